@@ -1,0 +1,128 @@
+/*
+ * p2c_hip.h -- C-ABI of the MI355X (gfx950) proposal-to-certificate library
+ * (libp2c_hip.so).  Plain pointers and sizes only; every entry point returns
+ * 0 on success or a negative P2C_E* code, with a message available from
+ * p2c_last_error() (thread-local).  Device-pointer entry points take
+ * caller-owned device buffers and the caller's hipStream_t (passed as void*),
+ * enqueue work and return without synchronising unless stated otherwise.
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   src/cpp/certificate/certificate_cuda.h:10-26   (batch certificate, cross-check)
+ *   src/cpp/certificate/certificate_engine.h:101-127 (interval certificate, queue)
+ *   src/cpp/rt_candidate/optix_candidate_tracer.cpp:834-889 (device candidate emit)
+ *   src/tools/scene_object_envelope_native_ti_walltime.cpp:256-302, 603-696 (VF/EE sets)
+ *   src/tools/tight_inclusion_full_query_benchmark.cpp:375-397 (ticcd call site)
+ */
+#ifndef P2C_HIP_H
+#define P2C_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ------------------------------------------------------ */
+#define P2C_OK 0
+#define P2C_EINVAL -1      /* invalid argument / config (reference ValueError text) */
+#define P2C_EHIP -2        /* HIP runtime failure */
+#define P2C_ECAPACITY -3   /* a caller-sized output buffer was too small */
+#define P2C_EUNSUPPORTED -4
+
+/* ---- status / family codes (runtime_contracts.h:22-33, certificate_engine.h:12-31) */
+#define P2C_STATUS_COLLISION 0
+#define P2C_STATUS_SEPARATION 1
+#define P2C_STATUS_UNDECIDED 2
+#define P2C_STATUS_INVALID 255 /* primitive failed validation (ids < 0 / non-finite) */
+#define P2C_FAMILY_POINT_TRIANGLE 1
+#define P2C_FAMILY_EDGE_EDGE 2
+
+/* LinearVertexTrajectory (certificate_engine.h:32-36): 56 bytes. */
+typedef struct p2c_traj {
+  int64_t feature_id;
+  double position_t0[3];
+  double position_t1[3];
+} p2c_traj;
+
+/* PointTriangleIntervalPrimitive / EdgeEdgeIntervalPrimitive
+ * (certificate_engine.h:38-54): 240 bytes, the same AoS layout the reference
+ * CUDA batch ships (certificate_cuda.cu:37-53).
+ *   PT: id_a = point_id, id_b = triangle_id, v = {point, tri_v0, tri_v1, tri_v2}
+ *   EE: id_a = edge_a_id, id_b = edge_b_id, v = {edge_a0, edge_a1, edge_b0, edge_b1} */
+typedef struct p2c_prim {
+  int64_t id_a;
+  int64_t id_b;
+  p2c_traj v[4];
+} p2c_prim;
+
+/* CertificateEngineConfig (certificate_engine.h:56-60). */
+typedef struct p2c_cert_config {
+  double eps_time;
+  double eps_space;
+  uint32_t max_subdivision_depth;
+  uint32_t reserved;
+} p2c_cert_config;
+
+/* PrimitiveIntervalResult (certificate_engine.h:69-81): 64 bytes. */
+typedef struct p2c_prim_result {
+  uint8_t status;
+  uint8_t witness_family;
+  uint16_t reason_code;
+  uint32_t covered_feature_mask;
+  double interval_t0;
+  double interval_t1;
+  double toi_upper;
+  double safe_margin_lb;
+  int64_t witness_id_a;
+  int64_t witness_id_b;
+  uint8_t next_refinement_mode;
+  uint8_t pad[3];
+  uint32_t nodes; /* subdivision nodes evaluated (work counter, not in the reference) */
+} p2c_prim_result;
+
+/* ---- library ------------------------------------------------------------ */
+const char* p2c_last_error(void);
+int p2c_version(void);
+/* 1 when a gfx950 device is visible and the kernels can launch. */
+int p2c_device_available(void);
+int p2c_stream_synchronize(void* stream);
+
+/* ---- reference-parity interval certificate ------------------------------
+ * Replaces EvaluatePointTriangleBatchCuda / EvaluateEdgeEdgeBatchCuda
+ * (certificate_cuda.h:11-20) with CertificateEngine CPU semantics
+ * (certificate_engine.cpp:300-386): depth-first left-then-right subdivision,
+ * first collision in that order wins, min margin over separated leaves.
+ * `prims` and `out` are device pointers of n elements.  A primitive that
+ * fails validation gets status P2C_STATUS_INVALID and reason_code 3.
+ * `invalid_count` (device uint32, may be NULL) is incremented per such one. */
+int p2c_cert_ref_batch(int family, const p2c_prim* prims, int64_t n, double interval_t0,
+                       double interval_t1, const p2c_cert_config* config,
+                       p2c_prim_result* out, uint32_t* invalid_count, void* stream);
+
+/* Closest distance of each primitive at its own time t[i] (device arrays of
+ * n), the distance functions of certificate_engine.cpp:186-198 evaluated on
+ * the device; exposed so FP64 lerp, sqrt and division can be checked
+ * bit-for-bit against the host.  No reference export corresponds to it. */
+int p2c_cert_ref_distance(int family, const p2c_prim* prims, const double* t, int64_t n,
+                          double* out, void* stream);
+
+/* Host-buffer twin of p2c_cert_ref_batch with the reference's ownership model
+ * (host vectors in and out, certificate_cuda.cu:456-552): allocates, copies,
+ * launches, synchronises.  Fails with the reference's validation message when
+ * any primitive is invalid. */
+int p2c_cert_ref_batch_host(int family, const p2c_prim* prims, int64_t n, double interval_t0,
+                            double interval_t1, const p2c_cert_config* config,
+                            p2c_prim_result* out);
+
+/* The same certificate evaluated on the host CPU from the same source as the
+ * device kernel (the reference's CPU entry points EvaluatePointTriangleInterval
+ * / EvaluateEdgeEdgeInterval, certificate_engine.h:101-110). */
+int p2c_cert_ref_eval_cpu(int family, const p2c_prim* prim, double interval_t0,
+                          double interval_t1, const p2c_cert_config* config,
+                          p2c_prim_result* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* P2C_HIP_H */

@@ -1,0 +1,134 @@
+"""CPU oracle for the proposal-to-certificate path -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this package, and only as the checker / CPU baseline; the shipped library
+never links or calls it.  See oracle.h for what each C file restates and how
+each is pinned.
+
+Two shared libraries are loaded lazily:
+  liboracle.so        the C restatement (oracle/*.c), always built by build()
+  _ref/libp2c_ref.so  the reference's own certificate_engine.cpp compiled in
+                      place (oracle/Makefile `ref`); only exists where the
+                      reference sources were present at build time.
+"""
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+REF_LIB = HERE / "_ref" / "libp2c_ref.so"
+
+TRAJ = np.dtype([("feature_id", "<i8"), ("p0", "<f8", (3,)), ("p1", "<f8", (3,))])
+PRIM = np.dtype([("id_a", "<i8"), ("id_b", "<i8"), ("v", TRAJ, (4,))])
+RESULT = np.dtype(
+    [
+        ("status", "<i4"),
+        ("covered_feature_mask", "<u4"),
+        ("interval_t0", "<f8"),
+        ("interval_t1", "<f8"),
+        ("toi_upper", "<f8"),
+        ("safe_margin_lb", "<f8"),
+        ("witness_family", "<i4"),
+        ("reason_code", "<i4"),
+        ("witness_id_a", "<i8"),
+        ("witness_id_b", "<i8"),
+        ("next_refinement_mode", "<i4"),
+        ("nodes", "<i4"),
+    ]
+)
+TUV_RESULT = np.dtype([("hit", "<i4"), ("capped", "<i4"), ("toi", "<f8"), ("boxes", "<i8")])
+assert PRIM.itemsize == 240 and RESULT.itemsize == 72 and TUV_RESULT.itemsize == 24
+
+_lib = None
+_ref = None
+_vp, _i64, _i32, _f64 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_double
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            raise ImportError(f"{LIB} not built (make -C oracle)")
+        _lib = ctypes.CDLL(str(LIB))
+        _lib.orc_ref_eval_batch.argtypes = [_i32, _vp, _i64, _f64, _f64, _f64, _f64, _i32, _vp]
+        _lib.orc_ref_eval_batch.restype = _i32
+        _lib.orc_pt_distance.argtypes = [_vp, _f64]
+        _lib.orc_pt_distance.restype = _f64
+        _lib.orc_ee_distance.argtypes = [_vp, _f64]
+        _lib.orc_ee_distance.restype = _f64
+        if hasattr(_lib, "orc_tuv_eval_batch"):
+            _lib.orc_tuv_eval_batch.argtypes = [_i32, _vp, _i64, _f64, _f64, _f64, _i64, _i64, _vp, _i32]
+            _lib.orc_tuv_eval_batch.restype = _i32
+            _lib.orc_tuv_params.argtypes = [_i32, _vp, _f64, _vp, _vp]
+            _lib.orc_tuv_params.restype = None
+    return _lib
+
+
+def ref_available() -> bool:
+    return REF_LIB.exists()
+
+
+def ref_lib() -> ctypes.CDLL:
+    global _ref
+    if _ref is None:
+        if not REF_LIB.exists():
+            raise ImportError(f"{REF_LIB} not built (make -C oracle ref, needs the reference sources)")
+        _ref = ctypes.CDLL(str(REF_LIB))
+        _ref.ref_eval_batch.argtypes = [_i32, _vp, _i64, _f64, _f64, _f64, _f64, _i32, _vp]
+        _ref.ref_eval_batch.restype = _i64
+        _ref.ref_engine_eval.argtypes = [ctypes.c_uint32, _f64, _f64, _vp, _i64, _vp, _i64, _f64, _f64, _i32, _vp]
+        _ref.ref_engine_eval.restype = _i32
+    return _ref
+
+
+def prims_from_positions(x: np.ndarray, ids: np.ndarray | None = None) -> np.ndarray:
+    """(n, 8, 3) positions (4 vertices at t0 then at t1) -> PRIM records."""
+    x = np.asarray(x, dtype=np.float64)
+    n = x.shape[0]
+    out = np.zeros(n, PRIM)
+    if ids is None:
+        ids = np.stack([np.arange(n) * 2, np.arange(n) * 2 + 1], axis=1)
+    out["id_a"], out["id_b"] = ids[:, 0], ids[:, 1]
+    for k in range(4):
+        out["v"][:, k]["feature_id"] = np.arange(n) * 4 + k
+        out["v"][:, k]["p0"] = x[:, k]
+        out["v"][:, k]["p1"] = x[:, 4 + k]
+    return out
+
+
+def ref_cert(family: int, prims: np.ndarray, t0=0.0, t1=1.0, eps_time=1.0e-4, eps_space=1.0e-6,
+             max_depth=32) -> np.ndarray:
+    """Restated reference certificate over a batch (oracle/ref_cert.c)."""
+    prims = np.ascontiguousarray(prims, dtype=PRIM)
+    out = np.zeros(prims.shape[0], RESULT)
+    rc = lib().orc_ref_eval_batch(family, prims.ctypes.data, prims.shape[0], t0, t1, eps_time, eps_space,
+                                  max_depth, out.ctypes.data)
+    if rc != 0:
+        raise ValueError(f"oracle rejected input (code {rc})")
+    return out
+
+
+def ref_cert_compiled(family: int, prims: np.ndarray, t0=0.0, t1=1.0, eps_time=1.0e-4, eps_space=1.0e-6,
+                      max_depth=32) -> np.ndarray:
+    """The reference's own compiled certificate_engine.cpp (oracle/_ref)."""
+    prims = np.ascontiguousarray(prims, dtype=PRIM)
+    out = np.zeros(prims.shape[0], RESULT)
+    rc = ref_lib().ref_eval_batch(family, prims.ctypes.data, prims.shape[0], t0, t1, eps_time, eps_space,
+                                  max_depth, out.ctypes.data)
+    if rc != 0:
+        raise ValueError(f"reference rejected primitive {rc - 1}")
+    return out
+
+
+def ref_engine_compiled(mask: int, pts: np.ndarray, ees: np.ndarray, t0=0.0, t1=1.0, eps_time=1.0e-4,
+                        eps_space=1.0e-6, max_depth=32) -> np.ndarray | None:
+    pts = np.ascontiguousarray(pts, dtype=PRIM)
+    ees = np.ascontiguousarray(ees, dtype=PRIM)
+    out = np.zeros(9, np.float64)
+    rc = ref_lib().ref_engine_eval(mask, t0, t1, pts.ctypes.data, pts.shape[0], ees.ctypes.data, ees.shape[0],
+                                   eps_time, eps_space, max_depth, out.ctypes.data)
+    return None if rc != 0 else out

@@ -1,0 +1,110 @@
+/*
+ * CPU oracle for the proposal-to-certificate path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the shipped library links, loads or
+ * calls this code: only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg use it, and only as the checker / CPU baseline.
+ *
+ * Contents (each function cites the reference lines it restates):
+ *   ref_cert.c    reference-parity interval certificate (CertificateEngine)
+ *   tuv_cert.c    (t,u,v) inclusion certificate, canonical best-first order
+ *   broadphase.c  swept boxes + sort-and-sweep VF/EE candidate sets + brute force
+ *   mlp.c         TinySTPF FP64 scorer and the 32-128-128-1 FP32 group scorer
+ *
+ * Pinning (how each function is checked against something other than itself):
+ *   ref_cert  : golden vectors written by the compiled reference
+ *               (oracle/_ref, tests/golden/ref_cert_*.npz) + the reference
+ *               KATs (toi 0.25, margin > 1.9).
+ *   tuv_cert  : parity with Tight-Inclusion itself is UNPINNED (library not
+ *               vendored); pinned by closed forms, exact-rational brute force
+ *               (sympy) on small cases and invariants (zero FN, toi <= exact).
+ *   broadphase: brute-force O(n^2) enumeration with the same set rules.
+ *   mlp       : torch fp32 reference / closed form.
+ */
+#ifndef P2C_ORACLE_H
+#define P2C_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  int64_t feature_id;
+  double p0[3];
+  double p1[3];
+} orc_traj;
+
+typedef struct {
+  int64_t id_a;
+  int64_t id_b;
+  orc_traj v[4];
+} orc_prim; /* PT: point, tri v0, v1, v2 ; EE: a0, a1, b0, b1 */
+
+typedef struct {
+  int32_t status;
+  uint32_t covered_feature_mask;
+  double interval_t0;
+  double interval_t1;
+  double toi_upper;
+  double safe_margin_lb;
+  int32_t witness_family;
+  int32_t reason_code;
+  int64_t witness_id_a;
+  int64_t witness_id_b;
+  int32_t next_refinement_mode;
+  int32_t nodes; /* subdivision nodes evaluated (work counter) */
+} orc_result;
+
+/* family: 1 = point-triangle, 2 = edge-edge.  Returns 0, or a negative
+ * validation code (-1 config, -2 interval, -3 ids, -4 non-finite). */
+int orc_ref_eval(int family, const orc_prim* prim, double t0, double t1, double eps_time,
+                 double eps_space, int max_depth, orc_result* out);
+int orc_ref_eval_batch(int family, const orc_prim* prims, int64_t n, double t0, double t1,
+                       double eps_time, double eps_space, int max_depth, orc_result* out);
+double orc_pt_distance(const orc_prim* prim, double t);
+double orc_ee_distance(const orc_prim* prim, double t);
+
+/* (t,u,v) inclusion certificate, canonical sequential order. */
+typedef struct {
+  int32_t hit;       /* 1 collision (or conservative cap), 0 separation */
+  int32_t capped;    /* 1 if the work cap ended the search */
+  double toi;        /* earliest certified time (inf when no hit) */
+  int64_t boxes;     /* boxes evaluated */
+} orc_tuv_result;
+
+int orc_tuv_eval(int is_ee, const double* v8x3 /* [8][3]: 4 at t0 then 4 at t1 */, double ms,
+                 double tolerance, double t_max, int64_t max_itr, int64_t heap_cap,
+                 orc_tuv_result* out);
+int orc_tuv_eval_batch(int is_ee, const double* v, int64_t n, double ms, double tolerance,
+                       double t_max, int64_t max_itr, int64_t heap_cap, orc_tuv_result* out,
+                       int nthreads);
+void orc_tuv_params(int is_ee, const double* v8x3, double tolerance, double err_out[3],
+                    double tol_out[3]);
+
+/* broad phase */
+void orc_swept_boxes(const double* x0, const double* x1, int64_t nv, const int32_t* faces,
+                     int64_t nf, const int32_t* edges, int64_t ne, double thickness,
+                     double* vbox, double* fbox, double* ebox);
+int64_t orc_sweep_vf(const double* vbox, const int32_t* vobj, int64_t nv, const double* fbox,
+                     const int32_t* faces, const int32_t* fobj, int64_t nf,
+                     const uint8_t* allowed, int32_t nobj, int32_t* out, int64_t cap);
+int64_t orc_sweep_ee(const double* ebox, const int32_t* edges, const int32_t* eobj, int64_t ne,
+                     const uint8_t* allowed, int32_t nobj, int32_t* out, int64_t cap);
+int64_t orc_brute_vf(const double* vbox, const int32_t* vobj, int64_t nv, const double* fbox,
+                     const int32_t* faces, const int32_t* fobj, int64_t nf,
+                     const uint8_t* allowed, int32_t nobj, int32_t* out, int64_t cap);
+int64_t orc_brute_ee(const double* ebox, const int32_t* edges, const int32_t* eobj, int64_t ne,
+                     const uint8_t* allowed, int32_t nobj, int32_t* out, int64_t cap);
+
+/* scorers */
+double orc_tiny_stpf(const double* mean, const double* std, const double* w0, const double* b0,
+                     const double* w1, const double* b1, const double* w2, double b2,
+                     const double* feature);
+void orc_mlp_score(const float* feat, int64_t n, const float* w0, const float* b0,
+                   const float* w1, const float* b1, const float* w2, float b2, float* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,285 @@
+/*
+ * Oracle (test infrastructure only): CPU restatement of the reference's
+ * interval certificate, src/cpp/certificate/certificate_engine.cpp.
+ *
+ * Written as a plain recursive C program, mirroring the reference's control
+ * flow (the shipped HIP kernel uses a different, stackless traversal), so that
+ * agreement between the two is evidence and not a copy checking itself.
+ * Compiled with -ffp-contract=off; every expression keeps the reference's
+ * operand order so FP64 results are bit-identical.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+#include "oracle.h"
+
+static const double kEps = 1.0e-14; /* certificate_engine.cpp:20 */
+
+/* Planted mistakes for the tests that show the golden vectors have teeth
+ * (0 = faithful): 1 right half first, 2 motion radius drops the point term,
+ * 3 lerp written p0 + t (p1 - p0), 4 margin keeps eps_space. */
+int orc_ref_mutation = 0;
+
+/* std::max / std::min / std::clamp semantics (NaN behaviour included). */
+static double smax(double a, double b) { return (a < b) ? b : a; }
+static double smin(double a, double b) { return (b < a) ? b : a; }
+static double sclamp(double v, double lo, double hi) { return (v < lo) ? lo : ((hi < v) ? hi : v); }
+
+typedef struct { double x, y, z; } V3;
+
+static V3 v_add(V3 a, V3 b) { V3 r = {a.x + b.x, a.y + b.y, a.z + b.z}; return r; }
+static V3 v_sub(V3 a, V3 b) { V3 r = {a.x - b.x, a.y - b.y, a.z - b.z}; return r; }
+static V3 v_scale(V3 a, double s) { V3 r = {a.x * s, a.y * s, a.z * s}; return r; }
+static double v_dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static V3 v_cross(V3 a, V3 b) {
+  V3 r = {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+  return r;
+}
+static double v_norm(V3 a) { return sqrt(v_dot(a, a)); }
+static V3 v_load(const double* p) { V3 r = {p[0], p[1], p[2]}; return r; }
+
+/* PositionAt, certificate_engine.cpp:67-69 */
+static V3 pos_at(const orc_traj* tr, double t) {
+  if (orc_ref_mutation == 3)
+    return v_add(v_load(tr->p0), v_scale(v_sub(v_load(tr->p1), v_load(tr->p0)), t));
+  return v_add(v_scale(v_load(tr->p0), 1.0 - t), v_scale(v_load(tr->p1), t));
+}
+
+/* MaxDisplacementFromMidpoint, :71-76 */
+static double max_disp(const orc_traj* tr, double t0, double t1) {
+  const double len = smax(0.0, t1 - t0);
+  return 0.5 * len * v_norm(v_sub(v_load(tr->p1), v_load(tr->p0)));
+}
+
+static double dist_pp(V3 a, V3 b) { return v_norm(v_sub(a, b)); } /* :78-80 */
+
+/* DistancePointSegment, :82-90 */
+static double dist_ps(V3 p, V3 a, V3 b) {
+  const V3 ab = v_sub(b, a);
+  const double ab2 = v_dot(ab, ab);
+  if (ab2 <= kEps) return dist_pp(p, a);
+  const double t = sclamp(v_dot(v_sub(p, a), ab) / ab2, 0.0, 1.0);
+  return dist_pp(p, v_add(a, v_scale(ab, t)));
+}
+
+/* DistanceSegmentSegment, :92-131 */
+static double dist_ss(V3 p1, V3 q1, V3 p2, V3 q2) {
+  const V3 d1 = v_sub(q1, p1);
+  const V3 d2 = v_sub(q2, p2);
+  const V3 r = v_sub(p1, p2);
+  const double a = v_dot(d1, d1);
+  const double e = v_dot(d2, d2);
+  const double f = v_dot(d2, r);
+  double s = 0.0, t = 0.0;
+  if (a <= kEps && e <= kEps) return dist_pp(p1, p2);
+  if (a <= kEps) {
+    t = sclamp(f / e, 0.0, 1.0);
+  } else {
+    const double c = v_dot(d1, r);
+    if (e <= kEps) {
+      s = sclamp(-c / a, 0.0, 1.0);
+    } else {
+      const double b = v_dot(d1, d2);
+      const double denom = a * e - b * b;
+      if (denom != 0.0) s = sclamp((b * f - c * e) / denom, 0.0, 1.0);
+      t = (b * s + f) / e;
+      if (t < 0.0) {
+        t = 0.0;
+        s = sclamp(-c / a, 0.0, 1.0);
+      } else if (t > 1.0) {
+        t = 1.0;
+        s = sclamp((b - c) / a, 0.0, 1.0);
+      }
+    }
+  }
+  return dist_pp(v_add(p1, v_scale(d1, s)), v_add(p2, v_scale(d2, t)));
+}
+
+/* DistancePointTriangle, :133-184 (Voronoi regions; degenerate fallback) */
+static double dist_pt(V3 p, V3 a, V3 b, V3 c) {
+  const V3 ab = v_sub(b, a);
+  const V3 ac = v_sub(c, a);
+  const V3 n = v_cross(ab, ac);
+  if (v_dot(n, n) <= kEps) {
+    /* std::min({x, y, z}) keeps the first smallest */
+    double m = dist_ps(p, a, b);
+    const double y = dist_ps(p, b, c);
+    const double z = dist_ps(p, c, a);
+    if (y < m) m = y;
+    if (z < m) m = z;
+    return m;
+  }
+  const V3 ap = v_sub(p, a);
+  const double d1 = v_dot(ab, ap);
+  const double d2 = v_dot(ac, ap);
+  if (d1 <= 0.0 && d2 <= 0.0) return dist_pp(p, a);
+  const V3 bp = v_sub(p, b);
+  const double d3 = v_dot(ab, bp);
+  const double d4 = v_dot(ac, bp);
+  if (d3 >= 0.0 && d4 <= d3) return dist_pp(p, b);
+  const double vc = d1 * d4 - d3 * d2;
+  if (vc <= 0.0 && d1 >= 0.0 && d3 <= 0.0) {
+    const double v = d1 / (d1 - d3);
+    return dist_pp(p, v_add(a, v_scale(ab, v)));
+  }
+  const V3 cp = v_sub(p, c);
+  const double d5 = v_dot(ab, cp);
+  const double d6 = v_dot(ac, cp);
+  if (d6 >= 0.0 && d5 <= d6) return dist_pp(p, c);
+  const double vb = d5 * d2 - d1 * d6;
+  if (vb <= 0.0 && d2 >= 0.0 && d6 <= 0.0) {
+    const double w = d2 / (d2 - d6);
+    return dist_pp(p, v_add(a, v_scale(ac, w)));
+  }
+  const double va = d3 * d6 - d5 * d4;
+  if (va <= 0.0 && (d4 - d3) >= 0.0 && (d5 - d6) >= 0.0) {
+    const double w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+    return dist_pp(p, v_add(b, v_scale(v_sub(c, b), w)));
+  }
+  return fabs(v_dot(ap, n) / v_norm(n));
+}
+
+double orc_pt_distance(const orc_prim* pr, double t) { /* :186-191 */
+  return dist_pt(pos_at(&pr->v[0], t), pos_at(&pr->v[1], t), pos_at(&pr->v[2], t),
+                 pos_at(&pr->v[3], t));
+}
+
+double orc_ee_distance(const orc_prim* pr, double t) { /* :193-198 */
+  return dist_ss(pos_at(&pr->v[0], t), pos_at(&pr->v[1], t), pos_at(&pr->v[2], t),
+                 pos_at(&pr->v[3], t));
+}
+
+/* PointTriangleMotionRadius / EdgeEdgeMotionRadius, :200-222 */
+static double motion_radius(int family, const orc_prim* pr, double t0, double t1) {
+  if (family == 1) {
+    const double pd = max_disp(&pr->v[0], t0, t1);
+    double td = max_disp(&pr->v[1], t0, t1); /* std::max({..}) keeps first largest */
+    const double y = max_disp(&pr->v[2], t0, t1);
+    const double z = max_disp(&pr->v[3], t0, t1);
+    if (td < y) td = y;
+    if (td < z) td = z;
+    if (orc_ref_mutation == 2) return td;
+    return pd + td;
+  }
+  const double ea = smax(max_disp(&pr->v[0], t0, t1), max_disp(&pr->v[1], t0, t1));
+  const double eb = smax(max_disp(&pr->v[2], t0, t1), max_disp(&pr->v[3], t0, t1));
+  return ea + eb;
+}
+
+typedef struct {
+  int family;
+  const orc_prim* pr;
+  double eps_time, eps_space;
+  int max_depth;
+  int nodes;
+} Ctx;
+
+static double dist(const Ctx* c, double t) {
+  return c->family == 1 ? orc_pt_distance(c->pr, t) : orc_ee_distance(c->pr, t);
+}
+
+/* MakePrimitiveResult, :266-277 */
+static orc_result make_result(const Ctx* c, int status, double t0, double t1) {
+  orc_result r;
+  memset(&r, 0, sizeof(r));
+  r.status = status;
+  r.covered_feature_mask = status == 1 ? (uint32_t)c->family : 0u;
+  r.interval_t0 = t0;
+  r.interval_t1 = t1;
+  r.toi_upper = t1;
+  r.safe_margin_lb = 0.0;
+  r.witness_family = 0;
+  r.witness_id_a = -1;
+  r.witness_id_b = -1;
+  r.reason_code = 0;
+  r.next_refinement_mode = 0;
+  return r;
+}
+
+/* EvaluateIntervalRecursive, :300-386 */
+static orc_result eval_rec(Ctx* c, double t0, double t1, int depth) {
+  c->nodes++;
+  const double mid = 0.5 * (t0 + t1);
+  const double st[3] = {t0, mid, t1};
+  const double sd[3] = {dist(c, t0), dist(c, mid), dist(c, t1)};
+  for (int i = 0; i < 3; ++i) {
+    if (sd[i] <= c->eps_space) {
+      orc_result r = make_result(c, 0, t0, t1);
+      r.toi_upper = st[i];
+      r.witness_family = c->family;
+      r.witness_id_a = c->pr->id_a;
+      r.witness_id_b = c->pr->id_b;
+      r.safe_margin_lb = 0.0;
+      return r;
+    }
+  }
+  /* lower_bound_fn: distance_fn(mid) - motion radius (:500-502, :536-538) */
+  const double lb = dist(c, mid) - motion_radius(c->family, c->pr, t0, t1);
+  if (lb > c->eps_space) {
+    orc_result r = make_result(c, 1, t0, t1);
+    r.safe_margin_lb = smax(0.0, orc_ref_mutation == 4 ? lb : lb - c->eps_space);
+    return r;
+  }
+  if (depth >= c->max_depth || t1 - t0 <= c->eps_time) {
+    orc_result r = make_result(c, 2, t0, t1);
+    r.reason_code = 2;
+    r.next_refinement_mode = 1;
+    return r;
+  }
+  if (orc_ref_mutation == 1) {
+    orc_result right = eval_rec(c, mid, t1, depth + 1);
+    if (right.status == 0) return right;
+    orc_result left = eval_rec(c, t0, mid, depth + 1);
+    if (left.status == 0) return left;
+    orc_result r = make_result(c, (left.status == 1 && right.status == 1) ? 1 : 2, t0, t1);
+    if (r.status == 1) r.safe_margin_lb = smin(left.safe_margin_lb, right.safe_margin_lb);
+    else { r.reason_code = 2; r.next_refinement_mode = 1; }
+    return r;
+  }
+  orc_result left = eval_rec(c, t0, mid, depth + 1);
+  if (left.status == 0) return left;
+  orc_result right = eval_rec(c, mid, t1, depth + 1);
+  if (right.status == 0) return right;
+  if (left.status == 1 && right.status == 1) {
+    orc_result r = make_result(c, 1, t0, t1);
+    r.safe_margin_lb = smin(left.safe_margin_lb, right.safe_margin_lb);
+    return r;
+  }
+  orc_result r = make_result(c, 2, t0, t1);
+  r.reason_code = 2;
+  r.next_refinement_mode = 1;
+  return r;
+}
+
+static int finite_traj(const orc_traj* t) {
+  for (int k = 0; k < 3; ++k)
+    if (!isfinite(t->p0[k]) || !isfinite(t->p1[k])) return 0;
+  return 1;
+}
+
+/* EvaluatePointTriangleInterval / EvaluateEdgeEdgeInterval, :481-551, with
+ * ValidateConfig :224-232, ValidateInterval :234-242, primitive checks :244-264 */
+int orc_ref_eval(int family, const orc_prim* prim, double t0, double t1, double eps_time,
+                 double eps_space, int max_depth, orc_result* out) {
+  if (!isfinite(eps_time) || eps_time <= 0.0) return -1;
+  if (!isfinite(eps_space) || eps_space <= 0.0) return -1;
+  if (!isfinite(t0) || !isfinite(t1)) return -2;
+  if (t0 < 0.0 || t1 > 1.0 || t0 > t1) return -2;
+  if (prim->id_a < 0 || prim->id_b < 0) return -3;
+  for (int i = 0; i < 4; ++i)
+    if (!finite_traj(&prim->v[i])) return -4;
+  Ctx c = {family, prim, eps_time, eps_space, max_depth, 0};
+  *out = eval_rec(&c, t0, t1, 0);
+  out->nodes = c.nodes;
+  return 0;
+}
+
+int orc_ref_eval_batch(int family, const orc_prim* prims, int64_t n, double t0, double t1,
+                       double eps_time, double eps_space, int max_depth, orc_result* out) {
+  for (int64_t i = 0; i < n; ++i) {
+    const int rc = orc_ref_eval(family, &prims[i], t0, t1, eps_time, eps_space, max_depth, &out[i]);
+    if (rc != 0) return rc;
+  }
+  return 0;
+}

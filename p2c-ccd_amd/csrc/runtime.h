@@ -1,0 +1,33 @@
+// Host-side runtime helpers shared by the library's translation units:
+// thread-local last-error text, HIP status checks, launch sizing.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "p2c_hip.h"
+
+namespace p2c {
+
+int set_error(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Grid for a grid-stride kernel over n items.
+inline unsigned grid_for(int64_t n, int block, int64_t max_blocks = int64_t(1) << 20) {
+  int64_t g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > max_blocks) g = max_blocks;
+  return static_cast<unsigned>(g);
+}
+
+}  // namespace p2c
+
+#define P2C_HIP_CHECK(expr)                         \
+  do {                                              \
+    hipError_t _e = (expr);                         \
+    if (_e != hipSuccess) return p2c::hip_fail(_e, #expr); \
+  } while (0)
