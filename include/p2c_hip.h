@@ -122,6 +122,54 @@ int p2c_cert_ref_eval_cpu(int family, const p2c_prim* prim, double interval_t0,
                           double interval_t1, const p2c_cert_config* config,
                           p2c_prim_result* out);
 
+/* ---- (t,u,v) inclusion certificate --------------------------------------
+ * Replaces the Tight-Inclusion call sites ticcd::vertexFaceCCD / edgeEdgeCCD
+ * (tight_inclusion_full_query_benchmark.cpp:375-397, with ms, tolerance,
+ * t_max = 1, max_itr of :24-40).  The certificate's definition is the
+ * library's (FP64 corner enclosure of the multilinear F(t,u,v), forward error
+ * bound, toi = least t_lo over done boxes); see the design notes.  Pairs are
+ * first expanded into an opaque per-pair workspace of P2C_TUV_PAIR_BYTES
+ * each; certification writes one p2c_tuv_result per pair.  A pair whose
+ * search exceeds max_itr boxes is appended to the replay list and must be
+ * finished with p2c_tuv_replay (the fallback-replay queue). */
+#define P2C_TUV_PAIR_BYTES 288
+#define P2C_TUV_FLAG_OVER_BUDGET 1u /* search stopped at max_itr; result provisional until replayed */
+#define P2C_TUV_FLAG_REPLAYED 2u    /* result produced by the canonical replay */
+#define P2C_TUV_FLAG_CAPPED 4u      /* the work cap bound: conservative hit at the cap's t */
+
+typedef struct p2c_tuv_result {
+  double toi;        /* earliest certified time; +inf when no hit */
+  uint64_t boxes;    /* boxes evaluated */
+  uint32_t hit;      /* 1 collision (or conservative cap), 0 separation */
+  uint32_t flags;    /* P2C_TUV_FLAG_* */
+  uint64_t reserved;
+} p2c_tuv_result;
+
+size_t p2c_tuv_pair_bytes(void);
+size_t p2c_tuv_heap_bytes(int64_t heap_cap);
+
+/* v: device (n, 8, 3) doubles, rows 0-3 the vertices at t=0 (VF: p, f0, f1,
+ * f2; EE: a0, a1, b0, b1), rows 4-7 the same at t=1. */
+int p2c_tuv_setup_positions(int is_ee, const double* v, int64_t n, double ms, double tolerance, void* pairs,
+                            void* stream);
+/* Gathered from mesh vertex arrays x0, x1 (device (V, 3) doubles) through
+ * idx4 (device (n, 4) int32 vertex ids in the same order as above). */
+int p2c_tuv_setup_indexed(int is_ee, const double* x0, const double* x1, const int32_t* idx4, int64_t n,
+                          double ms, double tolerance, void* pairs, void* stream);
+/* One lane per pair (the plain one-launch path). */
+int p2c_tuv_plain(const void* pairs, int64_t n, double tolerance, int64_t max_itr, p2c_tuv_result* out,
+                  uint32_t* replay_list, uint32_t* replay_count, uint32_t replay_cap, void* stream);
+/* Persistent lane-refill kernel over `grid_blocks` blocks of 256; `counter`
+ * is a device uint64 that must be zero at launch. */
+int p2c_tuv_persistent(const void* pairs, int64_t n, double tolerance, int64_t max_itr, p2c_tuv_result* out,
+                       unsigned long long* counter, uint32_t* replay_list, uint32_t* replay_count,
+                       uint32_t replay_cap, int grid_blocks, void* stream);
+/* Fallback replay of the listed pairs in canonical order; `heaps` holds
+ * slots * p2c_tuv_heap_bytes(heap_cap) bytes of device memory. */
+int p2c_tuv_replay(const void* pairs, const uint32_t* replay_list, const uint32_t* replay_count,
+                   uint32_t replay_cap, double tolerance, int64_t max_itr, void* heaps, int64_t heap_cap,
+                   int64_t slots, p2c_tuv_result* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -132,3 +132,22 @@ def ref_engine_compiled(mask: int, pts: np.ndarray, ees: np.ndarray, t0=0.0, t1=
     rc = ref_lib().ref_engine_eval(mask, t0, t1, pts.ctypes.data, pts.shape[0], ees.ctypes.data, ees.shape[0],
                                    eps_time, eps_space, max_depth, out.ctypes.data)
     return None if rc != 0 else out
+
+
+def tuv_cert(is_ee: bool, x: np.ndarray, ms=0.0, tolerance=1e-6, max_itr=1_000_000, heap_cap=1 << 20,
+             threads=1) -> np.ndarray:
+    """Canonical (t,u,v) inclusion certificate over (n, 8, 3) queries (oracle/tuv_cert.c)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros(x.shape[0], TUV_RESULT)
+    rc = lib().orc_tuv_eval_batch(int(is_ee), x.ctypes.data, x.shape[0], ms, tolerance, 1.0, max_itr, heap_cap,
+                                  out.ctypes.data, threads)
+    if rc != 0:
+        raise ValueError(f"oracle rejected input (code {rc})")
+    return out
+
+
+def tuv_params(is_ee: bool, x: np.ndarray, tolerance=1e-6):
+    err, inv = np.zeros(3), np.zeros(3)
+    lib().orc_tuv_params(int(is_ee), np.ascontiguousarray(x, np.float64).ctypes.data, tolerance, err.ctypes.data,
+                         inv.ctypes.data)
+    return err, inv

@@ -1,0 +1,388 @@
+// (t,u,v) inclusion certificate on gfx950 (FP64, no contraction).
+//
+//   setup       per pair: positions -> TuvPair (error bound, tolerances,
+//               split sequence); from a (n,8,3) query array or gathered from
+//               mesh vertex arrays through 4 vertex indices per pair.
+//   plain       one lane per pair, depth-first walk to completion.  Pairs
+//               whose walk exceeds max_itr boxes go to the replay list.
+//   persistent  a grid sized to the chip; every lane walks one pair box by
+//               box and pulls the next pair the moment its pair finishes
+//               (wave-aggregated atomics), so uneven subdivision depth does
+//               not idle the other 63 lanes of a wave.
+//   replay      the fallback-replay queue: pairs over budget are re-run in
+//               the canonical best-first order with a global-memory heap, the
+//               order that defines the result when the cap binds.
+//
+// Replaces the ticcd::vertexFaceCCD / edgeEdgeCCD call sites of the
+// reference (src/tools/tight_inclusion_full_query_benchmark.cpp:375-397;
+// src/tools/scene_object_envelope_native_ti_walltime.cpp:704-754).
+#include <cmath>
+
+#include "cert_tuv_math.h"
+#include "p2c_hip.h"
+#include "runtime.h"
+
+namespace p2c {
+namespace {
+
+static_assert(sizeof(TuvPair) == P2C_TUV_PAIR_BYTES, "TuvPair layout");
+static_assert(sizeof(p2c_tuv_result) == 32, "p2c_tuv_result layout");
+
+constexpr int kTuvBlock = 256;
+
+__device__ inline void write_result(p2c_tuv_result* out, uint64_t best, uint64_t boxes, uint32_t flags) {
+  p2c_tuv_result r;
+  r.hit = best != kTuvNoHit ? 1u : 0u;
+  r.toi = best != kTuvNoHit ? key_to_f64(best) : __builtin_inf();
+  r.boxes = boxes;
+  r.flags = flags;
+  r.reserved = 0;
+  *out = r;
+}
+
+__global__ __launch_bounds__(kTuvBlock) void tuv_setup_positions_kernel(int is_ee, const double* __restrict__ v,
+                                                                        int64_t n, double ms, double tol,
+                                                                        TuvPair* __restrict__ pairs) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    double x[8][3];
+    for (int r = 0; r < 8; ++r)
+      for (int k = 0; k < 3; ++k) x[r][k] = v[i * 24 + r * 3 + k];
+    TuvPair q;
+    tuv_setup(&q, x, is_ee, ms, tol);
+    pairs[i] = q;
+  }
+}
+
+__global__ __launch_bounds__(kTuvBlock) void tuv_setup_indexed_kernel(int is_ee, const double* __restrict__ x0,
+                                                                      const double* __restrict__ x1,
+                                                                      const int32_t* __restrict__ idx4, int64_t n,
+                                                                      double ms, double tol,
+                                                                      TuvPair* __restrict__ pairs) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    double x[8][3];
+    for (int r = 0; r < 4; ++r) {
+      const int64_t vi = idx4[i * 4 + r];
+      for (int k = 0; k < 3; ++k) {
+        x[r][k] = x0[vi * 3 + k];
+        x[4 + r][k] = x1[vi * 3 + k];
+      }
+    }
+    TuvPair q;
+    tuv_setup(&q, x, is_ee, ms, tol);
+    pairs[i] = q;
+  }
+}
+
+__device__ inline void flag_for_replay(uint32_t pair, uint32_t* list, uint32_t* count, uint32_t cap) {
+  const uint32_t slot = atomicAdd(count, 1u);
+  if (slot < cap) list[slot] = pair;
+}
+
+__global__ __launch_bounds__(kTuvBlock) void tuv_plain_kernel(const TuvPair* __restrict__ pairs, int64_t n,
+                                                              double tol, uint64_t max_itr,
+                                                              p2c_tuv_result* __restrict__ out,
+                                                              uint32_t* __restrict__ replay_list,
+                                                              uint32_t* __restrict__ replay_count,
+                                                              uint32_t replay_cap) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    const TuvPair q = pairs[i];
+    uint64_t best = kTuvNoHit, boxes = 0;
+    const bool complete = tuv_walk(q, 0, 0, 0, 0, tol, &best, &boxes, max_itr);
+    write_result(&out[i], best, boxes, complete ? 0u : P2C_TUV_FLAG_OVER_BUDGET);
+    if (!complete) flag_for_replay((uint32_t)i, replay_list, replay_count, replay_cap);
+  }
+}
+
+// One lane's resumable walk state (the loop body of tuv_walk, one box per step).
+struct LaneWalk {
+  uint64_t key[3];
+  uint64_t best, boxes;
+  int e[3];
+  uint32_t level;
+  bool visit;
+};
+
+// Advances one step; returns true when the pair's walk has finished.
+// Sets *over when the box budget ran out.
+__device__ inline bool lane_step(const TuvPair& q, LaneWalk& w, double tol, uint64_t budget, bool* over) {
+  if (w.visit) {
+    int outcome = 0;
+    const bool discard = (!q.is_ee && w.key[1] + w.key[2] > kOne63);
+    if (!discard && w.key[0] < w.best) {
+      if (w.boxes >= budget) {
+        *over = true;
+        return true;
+      }
+      ++w.boxes;
+      outcome = tuv_classify(q, w.key[0], w.key[1], w.key[2], w.e, w.level, tol);
+      if (outcome == 1 && w.key[0] < w.best) w.best = w.key[0];
+    }
+    if (outcome == 2) {
+      const int d = tuv_level_dim(q, w.level);
+      ++w.e[d];
+      ++w.level;
+      return false;
+    }
+  }
+  w.visit = false;
+  while (w.level > 0) {
+    const int d = tuv_level_dim(q, w.level - 1);
+    const uint64_t bit = kOne63 >> w.e[d];
+    if (w.key[d] & bit) {
+      w.key[d] &= ~bit;
+      --w.e[d];
+      --w.level;
+    } else {
+      w.key[d] |= bit;
+      w.visit = true;
+      return false;
+    }
+  }
+  return true;
+}
+
+__global__ __launch_bounds__(kTuvBlock) void tuv_persistent_kernel(const TuvPair* __restrict__ pairs, int64_t n,
+                                                                   double tol, uint64_t max_itr,
+                                                                   p2c_tuv_result* __restrict__ out,
+                                                                   unsigned long long* __restrict__ next_pair,
+                                                                   uint32_t* __restrict__ replay_list,
+                                                                   uint32_t* __restrict__ replay_count,
+                                                                   uint32_t replay_cap) {
+  const int lane = threadIdx.x & 63;
+  int64_t pair = -1;  // -1: needs a pair, -2: none left
+  TuvPair q;
+  LaneWalk w;
+  for (;;) {
+    const bool need = pair == -1;
+    const uint64_t need_mask = __ballot(need);
+    if (need_mask) {
+      const int leader = __ffsll((unsigned long long)need_mask) - 1;
+      unsigned long long base = 0;
+      if (lane == leader) base = atomicAdd(next_pair, (unsigned long long)__popcll(need_mask));
+      base = __shfl(base, leader);
+      if (need) {
+        const uint64_t rank = __popcll(need_mask & ((1ull << lane) - 1ull));
+        const int64_t p = (int64_t)(base + rank);
+        if (p < n) {
+          pair = p;
+          q = pairs[p];
+          w.key[0] = w.key[1] = w.key[2] = 0;
+          w.best = kTuvNoHit;
+          w.boxes = 0;
+          w.e[0] = w.e[1] = w.e[2] = 0;
+          w.level = 0;
+          w.visit = true;
+        } else {
+          pair = -2;
+        }
+      }
+    }
+    if (__ballot(pair >= 0) == 0) break;
+    if (pair >= 0) {
+      bool over = false;
+      if (lane_step(q, w, tol, max_itr, &over)) {
+        write_result(&out[pair], w.best, w.boxes, over ? P2C_TUV_FLAG_OVER_BUDGET : 0u);
+        if (over) flag_for_replay((uint32_t)pair, replay_list, replay_count, replay_cap);
+        pair = -1;
+      }
+    }
+  }
+}
+
+// ---- canonical best-first replay --------------------------------------
+struct HeapBox {
+  uint64_t kt, ku, kv;
+  uint32_t level;
+  uint32_t pad;
+};
+
+__device__ inline bool box_before(const HeapBox& a, const HeapBox& b) {
+  if (a.kt != b.kt) return a.kt < b.kt;
+  if (a.level != b.level) return a.level > b.level;
+  if (a.ku != b.ku) return a.ku < b.ku;
+  return a.kv < b.kv;
+}
+
+__device__ inline void heap_push(HeapBox* h, int64_t& n, const HeapBox& b) {
+  int64_t i = n++;
+  while (i > 0) {
+    const int64_t p = (i - 1) >> 1;
+    if (!box_before(b, h[p])) break;
+    h[i] = h[p];
+    i = p;
+  }
+  h[i] = b;
+}
+
+__device__ inline HeapBox heap_pop(HeapBox* h, int64_t& n) {
+  const HeapBox top = h[0];
+  const HeapBox last = h[--n];
+  int64_t i = 0;
+  for (;;) {
+    int64_t c = 2 * i + 1;
+    if (c >= n) break;
+    if (c + 1 < n && box_before(h[c + 1], h[c])) ++c;
+    if (!box_before(h[c], last)) break;
+    h[i] = h[c];
+    i = c;
+  }
+  if (n > 0) h[i] = last;
+  return top;
+}
+
+// Canonical order and caps (see the certificate definition): stops before the
+// first box of a new t_lo once more than max_itr boxes of smaller t_lo were
+// processed, or at 16 * max_itr boxes, or when the heap is full.
+__device__ void tuv_canonical(const TuvPair& q, double tol, int64_t max_itr, HeapBox* heap, int64_t heap_cap,
+                              uint64_t* best_out, uint64_t* boxes_out, bool* capped_out) {
+  int64_t hn = 0;
+  heap_push(heap, hn, HeapBox{0, 0, 0, 0, 0});
+  uint64_t best = kTuvNoHit, last_t = kTuvNoHit, cap_key = 0;
+  int64_t popped = 0;
+  bool capped = false;
+  const int64_t hard_cap = 16 * max_itr;
+  while (hn > 0) {
+    const HeapBox b = heap_pop(heap, hn);
+    if (b.kt >= best) break;
+    if (b.kt != last_t) {
+      last_t = b.kt;
+      if (popped > max_itr) {
+        capped = true;
+        cap_key = b.kt;
+        break;
+      }
+    }
+    if (popped >= hard_cap) {
+      capped = true;
+      cap_key = b.kt;
+      break;
+    }
+    ++popped;
+    int e[3];
+    tuv_exponents(q, b.level, e);
+    const int outcome = tuv_classify(q, b.kt, b.ku, b.kv, e, b.level, tol);
+    if (outcome == 0) continue;
+    if (outcome == 1) {
+      if (b.kt < best) best = b.kt;
+      continue;
+    }
+    const int d = tuv_level_dim(q, b.level);
+    const uint64_t half = kOne63 >> (e[d] + 1);
+    HeapBox lo = b, hi = b;
+    lo.level = hi.level = b.level + 1;
+    if (d == 0) hi.kt += half;
+    else if (d == 1) hi.ku += half;
+    else hi.kv += half;
+    const HeapBox ch[2] = {lo, hi};
+    for (int c = 0; c < 2; ++c) {
+      if (!q.is_ee && ch[c].ku + ch[c].kv > kOne63) continue;
+      if (hn >= heap_cap) {
+        capped = true;
+        cap_key = b.kt;
+        goto done;
+      }
+      heap_push(heap, hn, ch[c]);
+    }
+  }
+done:
+  *best_out = capped ? cap_key : best;
+  *boxes_out = (uint64_t)popped;
+  *capped_out = capped;
+}
+
+__global__ __launch_bounds__(64) void tuv_replay_kernel(const TuvPair* __restrict__ pairs,
+                                                        const uint32_t* __restrict__ list,
+                                                        const uint32_t* __restrict__ count, uint32_t list_cap,
+                                                        double tol, int64_t max_itr, HeapBox* __restrict__ heaps,
+                                                        int64_t heap_cap, int64_t slots,
+                                                        p2c_tuv_result* __restrict__ out) {
+  const int64_t slot = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (slot >= slots) return;
+  uint32_t m = *count;
+  if (m > list_cap) m = list_cap;
+  for (int64_t j = slot; j < (int64_t)m; j += slots) {
+    const uint32_t p = list[j];
+    const TuvPair q = pairs[p];
+    uint64_t best, boxes;
+    bool capped;
+    tuv_canonical(q, tol, max_itr, heaps + slot * heap_cap, heap_cap, &best, &boxes, &capped);
+    write_result(&out[p], best, boxes, P2C_TUV_FLAG_REPLAYED | (capped ? P2C_TUV_FLAG_CAPPED : 0u));
+  }
+}
+
+int check_tuv(double ms, double tol, double t_max, int64_t max_itr) {
+  if (!(tol > 0.0) || !std::isfinite(tol)) return set_error(P2C_EINVAL, "tolerance must be finite and positive");
+  if (!(ms >= 0.0) || !std::isfinite(ms)) return set_error(P2C_EINVAL, "minimum separation must be finite and >= 0");
+  if (t_max != 1.0) return set_error(P2C_EUNSUPPORTED, "t_max must be 1 (the full step)");
+  if (max_itr <= 0) return set_error(P2C_EINVAL, "max_itr must be positive");
+  return P2C_OK;
+}
+
+}  // namespace
+}  // namespace p2c
+
+extern "C" {
+
+size_t p2c_tuv_pair_bytes(void) { return sizeof(p2c::TuvPair); }
+
+size_t p2c_tuv_heap_bytes(int64_t heap_cap) { return sizeof(p2c::HeapBox) * size_t(heap_cap); }
+
+int p2c_tuv_setup_positions(int is_ee, const double* v, int64_t n, double ms, double tolerance, void* pairs,
+                            void* stream) {
+  if (int rc = p2c::check_tuv(ms, tolerance, 1.0, 1)) return rc;
+  if (n <= 0) return P2C_OK;
+  hipLaunchKernelGGL(p2c::tuv_setup_positions_kernel, dim3(p2c::grid_for(n, p2c::kTuvBlock)),
+                     dim3(p2c::kTuvBlock), 0, p2c::as_stream(stream), is_ee, v, n, ms, tolerance,
+                     static_cast<p2c::TuvPair*>(pairs));
+  P2C_HIP_CHECK(hipGetLastError());
+  return P2C_OK;
+}
+
+int p2c_tuv_setup_indexed(int is_ee, const double* x0, const double* x1, const int32_t* idx4, int64_t n,
+                          double ms, double tolerance, void* pairs, void* stream) {
+  if (int rc = p2c::check_tuv(ms, tolerance, 1.0, 1)) return rc;
+  if (n <= 0) return P2C_OK;
+  hipLaunchKernelGGL(p2c::tuv_setup_indexed_kernel, dim3(p2c::grid_for(n, p2c::kTuvBlock)),
+                     dim3(p2c::kTuvBlock), 0, p2c::as_stream(stream), is_ee, x0, x1, idx4, n, ms, tolerance,
+                     static_cast<p2c::TuvPair*>(pairs));
+  P2C_HIP_CHECK(hipGetLastError());
+  return P2C_OK;
+}
+
+int p2c_tuv_plain(const void* pairs, int64_t n, double tolerance, int64_t max_itr, p2c_tuv_result* out,
+                  uint32_t* replay_list, uint32_t* replay_count, uint32_t replay_cap, void* stream) {
+  if (int rc = p2c::check_tuv(0.0, tolerance, 1.0, max_itr)) return rc;
+  if (n <= 0) return P2C_OK;
+  hipLaunchKernelGGL(p2c::tuv_plain_kernel, dim3(p2c::grid_for(n, p2c::kTuvBlock)), dim3(p2c::kTuvBlock), 0,
+                     p2c::as_stream(stream), static_cast<const p2c::TuvPair*>(pairs), n, tolerance,
+                     (uint64_t)max_itr, out, replay_list, replay_count, replay_cap);
+  P2C_HIP_CHECK(hipGetLastError());
+  return P2C_OK;
+}
+
+int p2c_tuv_persistent(const void* pairs, int64_t n, double tolerance, int64_t max_itr, p2c_tuv_result* out,
+                       unsigned long long* counter, uint32_t* replay_list, uint32_t* replay_count,
+                       uint32_t replay_cap, int grid_blocks, void* stream) {
+  if (int rc = p2c::check_tuv(0.0, tolerance, 1.0, max_itr)) return rc;
+  if (n <= 0) return P2C_OK;
+  if (grid_blocks <= 0) return p2c::set_error(P2C_EINVAL, "grid_blocks must be positive");
+  hipLaunchKernelGGL(p2c::tuv_persistent_kernel, dim3(grid_blocks), dim3(p2c::kTuvBlock), 0,
+                     p2c::as_stream(stream), static_cast<const p2c::TuvPair*>(pairs), n, tolerance,
+                     (uint64_t)max_itr, out, counter, replay_list, replay_count, replay_cap);
+  P2C_HIP_CHECK(hipGetLastError());
+  return P2C_OK;
+}
+
+int p2c_tuv_replay(const void* pairs, const uint32_t* replay_list, const uint32_t* replay_count,
+                   uint32_t replay_cap, double tolerance, int64_t max_itr, void* heaps, int64_t heap_cap,
+                   int64_t slots, p2c_tuv_result* out, void* stream) {
+  if (int rc = p2c::check_tuv(0.0, tolerance, 1.0, max_itr)) return rc;
+  if (heap_cap < 8 || slots <= 0) return p2c::set_error(P2C_EINVAL, "replay needs heap_cap >= 8 and slots > 0");
+  hipLaunchKernelGGL(p2c::tuv_replay_kernel, dim3((unsigned)((slots + 63) / 64)), dim3(64), 0,
+                     p2c::as_stream(stream), static_cast<const p2c::TuvPair*>(pairs), replay_list, replay_count,
+                     replay_cap, tolerance, max_itr, static_cast<p2c::HeapBox*>(heaps), heap_cap, slots, out);
+  P2C_HIP_CHECK(hipGetLastError());
+  return P2C_OK;
+}
+
+}  // extern "C"

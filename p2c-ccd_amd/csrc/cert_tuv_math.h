@@ -1,0 +1,238 @@
+// (t,u,v) inclusion certificate: per-pair setup, box evaluation and the
+// stackless depth-first walk, shared by the gfx950 kernels.
+//
+// Method: the inclusion-based root finder of Wang et al., ACM TOG 2021 (the
+// library the reference calls, src/tools/tight_inclusion_full_query_benchmark
+// .cpp:375-397).  The certificate's definition -- multilinear corner
+// enclosure, forward error bound, tolerance-driven split order, "toi = least
+// t_lo over done boxes" -- is stated in full in the library's design notes
+// and implemented here operation for operation (FP64, no contraction).
+//
+// Box encoding: lower corners as dyadic keys scaled by 2^63 (kt, ku, kv) and
+// a level.  Every box at a level has the same widths, because the split
+// dimension is a function of the widths alone; the per-pair split sequence is
+// precomputed (2 bits per level), so a box is (keys, level) and the path to
+// it can be read back from the key bits.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define P2C_TUV_HD __host__ __device__ inline
+
+namespace p2c {
+
+constexpr double kTuvErrScale = 5.6e-15;   // >= 50 * 2^-53: first-order corner error per unit |x|
+constexpr double kTuvErrFloor = 1e-300;    // covers underflow in the corner evaluation
+constexpr int kTuvMaxExp = 52;             // per-dimension depth limit (keys stay exact in FP64)
+constexpr int kTuvMaxLevel = 3 * kTuvMaxExp;
+constexpr uint64_t kOne63 = 1ull << 63;
+constexpr uint64_t kTuvNoHit = ~0ull;
+
+// Per-pair constants (288 bytes).
+struct TuvPair {
+  double x0[4][3];   // vertex positions at t = 0
+  double d[4][3];    // x1 - x0
+  double err[3];     // corner-evaluation error bound + minimum separation
+  double inv_tol[3]; // 3 L_k / tolerance
+  uint64_t split[5]; // split dimension per level, 2 bits each (3 = none: done level)
+  uint32_t done_level;
+  uint32_t is_ee;
+};
+
+P2C_TUV_HD double key_to_f64(uint64_t k) { return (double)k * 0x1p-63; }
+
+// F at the 8 corners of box [tl,th]x[ul,uh]x[vl,vh]; per-coordinate min/max.
+P2C_TUV_HD void tuv_corner_range(const TuvPair& q, double tl, double th, double ul, double uh, double vl,
+                                 double vh, double mn[3], double mx[3]) {
+  const double ts[2] = {tl, th};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    double lo = __builtin_inf(), hi = -__builtin_inf();
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+      const double X0 = q.x0[0][k] + ts[it] * q.d[0][k];
+      const double X1 = q.x0[1][k] + ts[it] * q.d[1][k];
+      const double X2 = q.x0[2][k] + ts[it] * q.d[2][k];
+      const double X3 = q.x0[3][k] + ts[it] * q.d[3][k];
+      double f[4];
+      if (!q.is_ee) {
+        const double e1 = X2 - X1, e2 = X3 - X1;
+        const double w1l = X1 + ul * e1, w1h = X1 + uh * e1;
+        const double s2l = vl * e2, s2h = vh * e2;
+        f[0] = X0 - (w1l + s2l);
+        f[1] = X0 - (w1l + s2h);
+        f[2] = X0 - (w1h + s2l);
+        f[3] = X0 - (w1h + s2h);
+      } else {
+        const double ea = X1 - X0, eb = X3 - X2;
+        const double pal = X0 + ul * ea, pah = X0 + uh * ea;
+        const double pbl = X2 + vl * eb, pbh = X2 + vh * eb;
+        f[0] = pal - pbl;
+        f[1] = pal - pbh;
+        f[2] = pah - pbl;
+        f[3] = pah - pbh;
+      }
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        lo = f[c] < lo ? f[c] : lo;
+        hi = f[c] > hi ? f[c] : hi;
+      }
+    }
+    mn[k] = lo;
+    mx[k] = hi;
+  }
+}
+
+// Which dimension a box with exponents e splits along, or 3 when none may.
+P2C_TUV_HD int tuv_split_dim(const TuvPair& q, const int e[3]) {
+  int best = 3;
+  double br = 1.0;
+  for (int k = 0; k < 3; ++k) {
+    if (e[k] >= kTuvMaxExp) continue;
+    const double r = __builtin_ldexp(1.0, -e[k]) * q.inv_tol[k];
+    if (r > br) {
+      br = r;
+      best = k;
+    }
+  }
+  return best;
+}
+
+// Split dimension of `level`.  The word is chosen with constant indices so
+// the pair record stays in registers (a dynamic index would move it to
+// scratch memory).
+P2C_TUV_HD int tuv_level_dim(const TuvPair& q, uint32_t level) {
+  const uint32_t w = level >> 5;
+  const uint64_t word = w == 0 ? q.split[0] : w == 1 ? q.split[1] : w == 2 ? q.split[2] : w == 3 ? q.split[3]
+                                                                                               : q.split[4];
+  return (int)((word >> ((level & 31u) * 2u)) & 3ull);
+}
+
+// Per-pair constants from the 8 positions (rows 0-3 at t=0, 4-7 at t=1).
+P2C_TUV_HD void tuv_setup(TuvPair* q, const double v[8][3], int is_ee, double ms, double tolerance) {
+  q->is_ee = (uint32_t)is_ee;
+  for (int i = 0; i < 4; ++i)
+    for (int k = 0; k < 3; ++k) {
+      q->x0[i][k] = v[i][k];
+      q->d[i][k] = v[4 + i][k] - v[i][k];
+    }
+  for (int k = 0; k < 3; ++k) {
+    double m = 0.0;
+    for (int i = 0; i < 8; ++i) {
+      const double a = __builtin_fabs(v[i][k]);
+      m = a > m ? a : m;
+    }
+    q->err[k] = kTuvErrScale * m + kTuvErrFloor + ms;
+  }
+  // unit-cube corner values and the largest change along each axis
+  double F[2][2][2][3];
+  for (int it = 0; it < 2; ++it)
+    for (int iu = 0; iu < 2; ++iu)
+      for (int iv = 0; iv < 2; ++iv) {
+        double mn[3], mx[3];
+        tuv_corner_range(*q, it, it, iu, iu, iv, iv, mn, mx);
+        for (int k = 0; k < 3; ++k) F[it][iu][iv][k] = mn[k];
+      }
+  double L[3] = {0.0, 0.0, 0.0};
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b)
+      for (int k = 0; k < 3; ++k) {
+        const double dt = __builtin_fabs(F[1][a][b][k] - F[0][a][b][k]);
+        const double du = __builtin_fabs(F[a][1][b][k] - F[a][0][b][k]);
+        const double dv = __builtin_fabs(F[a][b][1][k] - F[a][b][0][k]);
+        L[0] = dt > L[0] ? dt : L[0];
+        L[1] = du > L[1] ? du : L[1];
+        L[2] = dv > L[2] ? dv : L[2];
+      }
+  for (int k = 0; k < 3; ++k) q->inv_tol[k] = (3.0 * L[k]) / tolerance;
+  for (int w = 0; w < 5; ++w) q->split[w] = ~0ull;  // 3 = no split
+  int e[3] = {0, 0, 0};
+  uint32_t level = 0;
+  for (;;) {
+    const int d = tuv_split_dim(*q, e);
+    if (d == 3) break;
+    q->split[level >> 5] &= ~(3ull << ((level & 31u) * 2u));
+    q->split[level >> 5] |= (uint64_t)d << ((level & 31u) * 2u);
+    ++e[d];
+    ++level;
+  }
+  q->done_level = level;
+}
+
+// Box outcome: 0 = no zero inside, 1 = done (certifies a hit at its t_lo),
+// 2 = undecided (split).
+P2C_TUV_HD int tuv_classify(const TuvPair& q, uint64_t kt, uint64_t ku, uint64_t kv, const int e[3],
+                            uint32_t level, double tolerance) {
+  const double tl = key_to_f64(kt), th = key_to_f64(kt + (kOne63 >> e[0]));
+  const double ul = key_to_f64(ku), uh = key_to_f64(ku + (kOne63 >> e[1]));
+  const double vl = key_to_f64(kv), vh = key_to_f64(kv + (kOne63 >> e[2]));
+  double mn[3], mx[3];
+  tuv_corner_range(q, tl, th, ul, uh, vl, vh, mn, mx);
+  bool zero_in = true, box_in = true, small = true;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const double eps = q.err[k];
+    zero_in = zero_in && !(mn[k] > eps || mx[k] < -eps);
+    box_in = box_in && !(mn[k] < -eps || mx[k] > eps);
+    small = small && !(mx[k] - mn[k] > tolerance);
+  }
+  if (!zero_in) return 0;
+  if (small || box_in || level >= q.done_level) return 1;
+  return 2;
+}
+
+// Exponents of the boxes at `level` (replays the split sequence).
+P2C_TUV_HD void tuv_exponents(const TuvPair& q, uint32_t level, int e[3]) {
+  e[0] = e[1] = e[2] = 0;
+  for (uint32_t l = 0; l < level; ++l) ++e[tuv_level_dim(q, l)];
+}
+
+// Depth-first walk of the subtree rooted at box (kt,ku,kv) of level `root`,
+// lower halves first, pruning boxes with t_lo >= *best.  Updates *best with
+// the least t_lo of a done box, adds evaluated boxes to *boxes, and stops
+// early (returning false) once *boxes exceeds `budget`.
+P2C_TUV_HD bool tuv_walk(const TuvPair& q, uint64_t kt, uint64_t ku, uint64_t kv, uint32_t root,
+                         double tolerance, uint64_t* best, uint64_t* boxes, uint64_t budget) {
+  int e[3];
+  tuv_exponents(q, root, e);
+  uint64_t key[3] = {kt, ku, kv};
+  uint32_t level = root;
+  bool visit = true;  // evaluate the current box
+  for (;;) {
+    if (visit) {
+      int outcome = 0;
+      const bool discard = (!q.is_ee && key[1] + key[2] > kOne63);
+      if (!discard && key[0] < *best) {
+        if (*boxes >= budget) return false;
+        ++*boxes;
+        outcome = tuv_classify(q, key[0], key[1], key[2], e, level, tolerance);
+        if (outcome == 1 && key[0] < *best) *best = key[0];
+      }
+      if (outcome == 2) {
+        const int d = tuv_level_dim(q, level);
+        ++e[d];
+        ++level;  // lower child: keys unchanged
+        continue;
+      }
+    }
+    // next box in pre-order within the subtree: climb while an upper child
+    visit = false;
+    while (level > root) {
+      const int d = tuv_level_dim(q, level - 1);
+      const uint64_t bit = kOne63 >> e[d];
+      if (key[d] & bit) {
+        key[d] &= ~bit;
+        --e[d];
+        --level;
+      } else {
+        key[d] |= bit;  // upper sibling
+        visit = true;
+        break;
+      }
+    }
+    if (!visit) return true;
+  }
+}
+
+}  // namespace p2c

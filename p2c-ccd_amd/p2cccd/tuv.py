@@ -1,0 +1,125 @@
+"""(t,u,v) inclusion certificate on the GPU: host-side driver over the C-ABI.
+
+`TuvCertifier` owns the device workspaces (per-pair records, results, the
+fallback-replay list and heaps) and runs one of two execution forms:
+  plain       one lane per pair, one launch (the per-stage baseline)
+  persistent  chip-sized grid, lanes refill as pairs finish
+Both are followed by the fallback replay of pairs that exceeded max_itr, so
+every pair ends with a final collide/separate decision.
+
+Parameters mirror the reference's Tight-Inclusion call sites
+(src/tools/tight_inclusion_full_query_benchmark.cpp:24-40): ms 0,
+tolerance 1e-6, t_max 1, max_itr 1e6.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _native as nat
+
+TUV_RESULT_DTYPE = np.dtype(
+    [("toi", "<f8"), ("boxes", "<u8"), ("hit", "<u4"), ("flags", "<u4"), ("reserved", "<u8")]
+)
+FLAG_OVER_BUDGET, FLAG_REPLAYED, FLAG_CAPPED = 1, 2, 4
+PAIR_BYTES = 288
+
+_vp, _i64, _i32, _u32, _f64 = nat._vp, nat._i64, nat._i32, nat._u32, nat._f64
+nat._sig("p2c_tuv_setup_positions", [_i32, _vp, _i64, _f64, _f64, _vp, _vp])
+nat._sig("p2c_tuv_setup_indexed", [_i32, _vp, _vp, _vp, _i64, _f64, _f64, _vp, _vp])
+nat._sig("p2c_tuv_plain", [_vp, _i64, _f64, _i64, _vp, _vp, _vp, _u32, _vp])
+nat._sig("p2c_tuv_persistent", [_vp, _i64, _f64, _i64, _vp, _vp, _vp, _vp, _u32, _i32, _vp])
+nat._sig("p2c_tuv_replay", [_vp, _vp, _vp, _u32, _f64, _i64, _vp, _i64, _i64, _vp, _vp])
+nat.lib.p2c_tuv_pair_bytes.restype = nat.ctypes.c_size_t
+assert nat.lib.p2c_tuv_pair_bytes() == PAIR_BYTES
+
+
+@dataclass
+class TuvConfig:
+    tolerance: float = 1.0e-6
+    ms: float = 0.0
+    max_itr: int = 1_000_000
+    heap_cap: int = 1 << 20      # canonical-replay heap entries per slot
+    replay_slots: int = 8        # pairs replayed concurrently
+    blocks_per_cu: int = 3       # persistent grid: resident blocks of 256 per CU
+
+
+def results_view(t: torch.Tensor) -> np.ndarray:
+    return t.cpu().numpy().view(TUV_RESULT_DTYPE)
+
+
+class TuvCertifier:
+    """Certifies batches of VF or EE pairs.  Workspaces grow to the largest
+    batch seen and are reused; nothing is allocated when a batch fits."""
+
+    def __init__(self, config: TuvConfig | None = None, device: str | torch.device = "cuda"):
+        self.cfg = config or TuvConfig()
+        self.device = torch.device(device)
+        self._cap = 0
+        props = torch.cuda.get_device_properties(self.device)
+        self.grid_blocks = int(props.multi_processor_count) * self.cfg.blocks_per_cu
+        self.heaps = torch.empty(self.cfg.replay_slots * self.cfg.heap_cap * 32, dtype=torch.uint8,
+                                 device=self.device)
+        self.counters = torch.zeros(4, dtype=torch.int64, device=self.device)  # [next_pair, replay_count, ...]
+
+    def _reserve(self, n: int) -> None:
+        if n <= self._cap:
+            return
+        cap = max(n, 1024)
+        self.pairs = torch.empty(cap * PAIR_BYTES, dtype=torch.uint8, device=self.device)
+        self.results = torch.empty(cap * TUV_RESULT_DTYPE.itemsize, dtype=torch.uint8, device=self.device)
+        self.replay_list = torch.empty(cap, dtype=torch.int32, device=self.device)
+        self._cap = cap
+
+    # -- stages ------------------------------------------------------------
+    def setup_positions(self, x: torch.Tensor, is_ee: bool, stream=None) -> int:
+        assert x.dtype == torch.float64 and x.is_cuda and x.shape[1:] == (8, 3) and x.is_contiguous()
+        n = x.shape[0]
+        self._reserve(n)
+        nat.call("p2c_tuv_setup_positions", int(is_ee), x.data_ptr(), n, self.cfg.ms, self.cfg.tolerance,
+                 self.pairs.data_ptr(), nat.stream_handle(stream))
+        return n
+
+    def setup_indexed(self, x0: torch.Tensor, x1: torch.Tensor, idx4: torch.Tensor, is_ee: bool, stream=None) -> int:
+        assert x0.dtype == torch.float64 and idx4.dtype == torch.int32 and idx4.shape[1] == 4
+        n = idx4.shape[0]
+        self._reserve(n)
+        nat.call("p2c_tuv_setup_indexed", int(is_ee), x0.data_ptr(), x1.data_ptr(), idx4.data_ptr(), n,
+                 self.cfg.ms, self.cfg.tolerance, self.pairs.data_ptr(), nat.stream_handle(stream))
+        return n
+
+    def certify(self, n: int, mode: str = "persistent", stream=None) -> None:
+        """Certifies the n pairs set up last; results land in self.results."""
+        s = nat.stream_handle(stream)
+        c = self.counters
+        c.zero_()
+        rc_ptr = c.data_ptr() + 8  # replay count (uint32 in the low half of slot 1)
+        if mode == "plain":
+            nat.call("p2c_tuv_plain", self.pairs.data_ptr(), n, self.cfg.tolerance, self.cfg.max_itr,
+                     self.results.data_ptr(), self.replay_list.data_ptr(), rc_ptr, self._cap, s)
+        elif mode == "persistent":
+            nat.call("p2c_tuv_persistent", self.pairs.data_ptr(), n, self.cfg.tolerance, self.cfg.max_itr,
+                     self.results.data_ptr(), c.data_ptr(), self.replay_list.data_ptr(), rc_ptr, self._cap,
+                     self.grid_blocks, s)
+        else:
+            raise ValueError(f"unknown certificate mode {mode!r}")
+        nat.call("p2c_tuv_replay", self.pairs.data_ptr(), self.replay_list.data_ptr(), rc_ptr, self._cap,
+                 self.cfg.tolerance, self.cfg.max_itr, self.heaps.data_ptr(), self.cfg.heap_cap,
+                 self.cfg.replay_slots, self.results.data_ptr(), s)
+
+    def replay_count(self) -> int:
+        return int(self.counters[1].item() & 0xFFFFFFFF)
+
+    def result_tensor(self, n: int) -> torch.Tensor:
+        return self.results[: n * TUV_RESULT_DTYPE.itemsize]
+
+    # -- convenience -------------------------------------------------------
+    def certify_positions(self, x, is_ee: bool, mode: str = "persistent") -> np.ndarray:
+        if isinstance(x, np.ndarray):
+            x = torch.from_numpy(np.ascontiguousarray(x, dtype=np.float64)).to(self.device)
+        n = self.setup_positions(x, is_ee)
+        self.certify(n, mode)
+        torch.cuda.synchronize(self.device)
+        return results_view(self.result_tensor(n))

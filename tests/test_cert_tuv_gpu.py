@@ -1,0 +1,55 @@
+"""gfx950 (t,u,v) inclusion certificate against the canonical oracle
+(oracle/tuv_cert.c): identical hit decisions, bit-identical TOI, on the plain
+20k-query workload, through both execution forms, and with a small work cap
+that forces the fallback-replay queue (which must then reproduce the
+canonical capped answer)."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from p2cccd import workloads
+from p2cccd.tuv import FLAG_REPLAYED, TuvCertifier, TuvConfig
+
+pytestmark = pytest.mark.gpu
+
+
+def _check(got, ref, label):
+    assert np.array_equal(got["hit"], ref["hit"]), label
+    assert np.array_equal(got["toi"], ref["toi"]), label
+
+
+@pytest.mark.parametrize("mode", ["plain", "persistent"])
+def test_plain_workload_matches_oracle(gpu, mode):
+    qs = workloads.make_query_set(10_000, 10_000)
+    cert = TuvCertifier()
+    for is_ee, x in ((False, qs.vf), (True, qs.ee)):
+        got = cert.certify_positions(x, is_ee, mode)
+        ref = oracle.tuv_cert(is_ee, x, threads=8)
+        _check(got, ref, (mode, is_ee))
+        assert cert.replay_count() == 0
+        assert got["hit"][(qs.vf_kind if not is_ee else qs.ee_kind) == 1].all()  # touching pairs all hit
+
+
+@pytest.mark.parametrize("mode", ["plain", "persistent"])
+def test_forced_replay_reproduces_capped_oracle(gpu, mode):
+    qs = workloads.make_query_set(2000, 2000, seed=31)
+    cfg = TuvConfig(max_itr=40, heap_cap=1 << 12, replay_slots=16)
+    cert = TuvCertifier(cfg)
+    for is_ee, x in ((False, qs.vf), (True, qs.ee)):
+        got = cert.certify_positions(x, is_ee, mode)
+        ref = oracle.tuv_cert(is_ee, x, max_itr=40, heap_cap=1 << 12, threads=8)
+        _check(got, ref, (mode, is_ee))
+        replayed = (got["flags"] & FLAG_REPLAYED) != 0
+        assert replayed.sum() == cert.replay_count() > 0
+        assert np.array_equal(got["boxes"][replayed], ref["boxes"][replayed])
+
+
+def test_tolerance_and_separation_variants(gpu):
+    qs = workloads.make_query_set(1500, 1500, seed=44)
+    for kw in (dict(tolerance=1e-4), dict(tolerance=1e-8), dict(ms=1e-3)):
+        cert = TuvCertifier(TuvConfig(tolerance=kw.get("tolerance", 1e-6), ms=kw.get("ms", 0.0)))
+        for is_ee, x in ((False, qs.vf), (True, qs.ee)):
+            got = cert.certify_positions(x, is_ee, "persistent")
+            ref = oracle.tuv_cert(is_ee, x, ms=kw.get("ms", 0.0), tolerance=kw.get("tolerance", 1e-6), threads=8)
+            _check(got, ref, kw)
