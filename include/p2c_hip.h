@@ -170,6 +170,41 @@ int p2c_tuv_replay(const void* pairs, const uint32_t* replay_list, const uint32_
                    uint32_t replay_cap, double tolerance, int64_t max_itr, void* heaps, int64_t heap_cap,
                    int64_t slots, p2c_tuv_result* out, void* stream);
 
+/* ---- swept-AABB LBVH broad phase ----------------------------------------
+ * Replaces the OptiX candidate emitter (optix_candidate_tracer.cpp:652-889,
+ * optix_candidate_program.cu:66-130) with the candidate-set semantics of the
+ * reference's sort-and-sweep (scene_object_envelope_native_ti_walltime.cpp:
+ * 256-302 boxes, 528-550 object envelopes, 595-696 VF/EE sets): candidate
+ * sets are identical as sets.  All pointers in p2c_bp_mesh are device
+ * pointers; topology (faces, edges, objects) is prepared once per mesh as in
+ * the reference's PrepareScene (:497-586).  At most 64 objects per scene. */
+typedef struct p2c_bp_mesh {
+  const int32_t* faces;          /* (F, 3) */
+  const int32_t* edges;          /* (E, 2), (min, max) */
+  const int32_t* vertex_object;  /* (V,), -1 for vertices in no face */
+  const int32_t* face_object;    /* (F,) */
+  const int32_t* edge_object;    /* (E,) */
+  int64_t n_vertices, n_faces, n_edges;
+  int32_t n_objects;
+  int32_t reserved;
+} p2c_bp_mesh;
+
+size_t p2c_bp_workspace_bytes(int64_t n_vertices, int64_t n_faces, int64_t n_edges, int32_t n_objects);
+/* Swept boxes (FP64, +/- thickness), object envelopes, both LBVHs (faces for
+ * VF, edges for EE) and the vertex query order.  x0, x1: device (V, 3). */
+int p2c_bp_build(const p2c_bp_mesh* mesh, const double* x0, const double* x1, double thickness, void* workspace,
+                 size_t workspace_bytes, void* stream);
+/* Traversal count pass and scan; writes to the device array totals[3]: VF
+ * total, EE total, and the number of traversals that hit the step bound
+ * (always 0 for a well-formed tree). */
+int p2c_bp_count(const p2c_bp_mesh* mesh, void* workspace, uint64_t* totals, void* stream);
+/* Fill pass: VF pairs (vertex, face) and EE pairs (edge i, edge j), i < j,
+ * as int32 pairs; entries beyond a capacity are dropped (compare with the
+ * totals from p2c_bp_count).  Output order: queries in Morton order, leaves
+ * in traversal order. */
+int p2c_bp_emit(const p2c_bp_mesh* mesh, void* workspace, int32_t* vf_pairs, int64_t vf_cap, int32_t* ee_pairs,
+                int64_t ee_cap, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -151,3 +151,80 @@ def tuv_params(is_ee: bool, x: np.ndarray, tolerance=1e-6):
     lib().orc_tuv_params(int(is_ee), np.ascontiguousarray(x, np.float64).ctypes.data, tolerance, err.ctypes.data,
                          inv.ctypes.data)
     return err, inv
+
+
+# ---- broad phase ----------------------------------------------------------
+def _bp_sigs():
+    L = lib()
+    if getattr(L, "_bp_ready", False):
+        return L
+    L.orc_swept_boxes.argtypes = [_vp, _vp, _i64, _vp, _i64, _vp, _i64, _f64, _vp, _vp, _vp]
+    L.orc_swept_boxes.restype = None
+    for name in ("orc_sweep_vf", "orc_brute_vf"):
+        fn = getattr(L, name)
+        fn.argtypes = [_vp, _vp, _i64, _vp, _vp, _vp, _i64, _vp, _i32, _vp, _i64]
+        fn.restype = _i64
+    for name in ("orc_sweep_ee", "orc_brute_ee"):
+        fn = getattr(L, name)
+        fn.argtypes = [_vp, _vp, _vp, _i64, _vp, _i32, _vp, _i64]
+        fn.restype = _i64
+    L._bp_ready = True
+    return L
+
+
+def swept_boxes(x0, x1, faces, edges, thickness):
+    L = _bp_sigs()
+    x0, x1 = np.ascontiguousarray(x0, np.float64), np.ascontiguousarray(x1, np.float64)
+    faces, edges = np.ascontiguousarray(faces, np.int32), np.ascontiguousarray(edges, np.int32)
+    vb, fb, eb = np.zeros((len(x0), 6)), np.zeros((len(faces), 6)), np.zeros((len(edges), 6))
+    L.orc_swept_boxes(x0.ctypes.data, x1.ctypes.data, len(x0), faces.ctypes.data, len(faces), edges.ctypes.data,
+                      len(edges), thickness, vb.ctypes.data, fb.ctypes.data, eb.ctypes.data)
+    return vb, fb, eb
+
+
+def object_pair_allowed(vbox, vobj, nobj):
+    """Object swept boxes and the allowed matrix (PrepareScene :528-550):
+    self pairs allowed, distinct objects when their boxes overlap."""
+    ob = np.zeros((nobj, 6))
+    ob[:, :3], ob[:, 3:] = np.inf, -np.inf
+    for o in range(nobj):
+        m = vobj == o
+        if m.any():
+            ob[o, :3] = vbox[m, :3].min(0)
+            ob[o, 3:] = vbox[m, 3:].max(0)
+    allowed = np.zeros((nobj, nobj), np.uint8)
+    for a in range(nobj):
+        for b in range(a, nobj):
+            ok = a == b or bool(np.all(ob[a, :3] <= ob[b, 3:]) and np.all(ob[b, :3] <= ob[a, 3:]))
+            allowed[a, b] = allowed[b, a] = ok
+    return ob, allowed
+
+
+def candidates(scene, method="sweep", cap=None):
+    """VF and EE candidate pairs of a scene (sorted lexicographically)."""
+    L = _bp_sigs()
+    t = scene.topo
+    vb, fb, eb = swept_boxes(scene.x0, scene.x1, t.faces, t.edges, scene.thickness)
+    # the object box of PrepareScene is built from raw positions then expanded
+    # by the thickness; it equals the union of the expanded vertex boxes
+    _, allowed = object_pair_allowed(vb, t.vertex_object, t.object_count)
+    vobj, fobj, eobj = (np.ascontiguousarray(a, np.int32) for a in (t.vertex_object, t.face_object, t.edge_object))
+    out = {}
+    for kind in ("vf", "ee"):
+        c = cap or 1
+        while True:
+            buf = np.zeros((c, 2), np.int32)
+            if kind == "vf":
+                fn = L.orc_sweep_vf if method == "sweep" else L.orc_brute_vf
+                n = fn(vb.ctypes.data, vobj.ctypes.data, len(vb), fb.ctypes.data, t.faces.ctypes.data, fobj.ctypes.data,
+                       len(fb), allowed.ctypes.data, t.object_count, buf.ctypes.data, c)
+            else:
+                fn = L.orc_sweep_ee if method == "sweep" else L.orc_brute_ee
+                n = fn(eb.ctypes.data, t.edges.ctypes.data, eobj.ctypes.data, len(eb), allowed.ctypes.data,
+                       t.object_count, buf.ctypes.data, c)
+            if n <= c:
+                break
+            c = int(n)
+        pairs = buf[:n]
+        out[kind] = pairs[np.lexsort((pairs[:, 1], pairs[:, 0]))]
+    return out

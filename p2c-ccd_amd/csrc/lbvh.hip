@@ -1,0 +1,821 @@
+// Swept-AABB LBVH broad phase on gfx950: vertex-face and edge-edge candidate
+// pairs with the reference's set semantics.
+//
+// Candidate set (src/tools/scene_object_envelope_native_ti_walltime.cpp):
+//   boxes  :256-302  min/max of a primitive's vertices at t0 and t1, then
+//                    -/+ thickness (FP64, exact min/max, one rounding each side)
+//   VF     :651-671  (vertex v, face f), inclusive overlap (:270-274), object
+//                    pair allowed, not (same object and f contains v)
+//   EE     :673-696  (edge i < edge j), overlap, allowed, not (same object
+//                    and a shared vertex)
+//   objects:528-550  object envelopes; distinct objects pair only when their
+//                    envelopes overlap, self pairs are allowed
+// The reference enumerates with a single-threaded sort-and-sweep (:603-649);
+// here the enumeration is an LBVH:
+//   1. swept boxes + object envelopes (order-preserving u64 atomics in LDS)
+//   2. 30-bit Morton codes of box centres (ordering only: no box is culled
+//      by quantisation, every overlap test is the exact FP64 one)
+//   3. LSD radix sort, 8-bit digits, stable in-block ranking from wave
+//      ballots with per-wave digit counters staged in LDS
+//   4. Karras build (distinct 64-bit keys: code << 32 | position) with
+//      escape indices for stackless traversal; bottom-up FP64 refit with
+//      acquire/release counters
+//   5. stackless traversal, one lane per query in Morton order: count,
+//      exclusive scan, fill (deterministic output order)
+#include <cmath>
+
+#include "p2c_hip.h"
+#include "runtime.h"
+
+namespace p2c {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kMaxObjects = 64;
+constexpr int kSortItems = 16;  // keys per lane per radix tile
+constexpr int kSortTile = kBlock * kSortItems;
+
+struct Box6 {
+  double mn[3];
+  double mx[3];
+};
+
+struct Node {  // 64 bytes; internal: left/right children; leaf: left = right = -1, prim
+  double mn[3];
+  double mx[3];
+  int32_t left, right, escape, prim;
+};
+static_assert(sizeof(Node) == 64, "node layout");
+
+__device__ inline uint64_t f64_order_key(double x) {
+  const uint64_t b = __double_as_longlong(x);
+  return (b >> 63) ? ~b : (b | (1ull << 63));
+}
+__device__ inline double f64_from_order_key(uint64_t k) {
+  const uint64_t b = (k >> 63) ? (k & ~(1ull << 63)) : ~k;
+  return __longlong_as_double(b);
+}
+
+__device__ inline void box_reset(Box6& b) {
+  b.mn[0] = b.mn[1] = b.mn[2] = __builtin_inf();
+  b.mx[0] = b.mx[1] = b.mx[2] = -__builtin_inf();
+}
+// Extend with std::min / std::max semantics.
+__device__ inline void box_extend(Box6& b, const double* p) {
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    b.mn[k] = (p[k] < b.mn[k]) ? p[k] : b.mn[k];
+    b.mx[k] = (b.mx[k] < p[k]) ? p[k] : b.mx[k];
+  }
+}
+__device__ inline void box_expand(Box6& b, double d) {
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    b.mn[k] -= d;
+    b.mx[k] += d;
+  }
+}
+__device__ inline bool overlap(const Box6& a, const double* mn, const double* mx) {
+  return a.mn[0] <= mx[0] && mn[0] <= a.mx[0] && a.mn[1] <= mx[1] && mn[1] <= a.mx[1] && a.mn[2] <= mx[2] &&
+         mn[2] <= a.mx[2];
+}
+
+// ---- workspace ----------------------------------------------------------
+struct Layout {
+  size_t vbox, fbox, ebox, objkey, allowed, keys0, vals0, keys1, vals1, hist, fnodes, fparent, fflag, frcs,
+      fhi, enodes, eparent, eflag, ercs, ehi, qorder, eorder, counts, offsets, hist_off, scan_part, err, total;
+  int64_t sort_cap, hist_blocks;
+};
+
+inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+Layout make_layout(int64_t nv, int64_t nf, int64_t ne, int32_t nobj) {
+  Layout L{};
+  size_t o = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = o;
+    o = align256(o + (bytes ? bytes : 1));
+    return at;
+  };
+  const int64_t sn = std::max<int64_t>(std::max(nv, nf), std::max<int64_t>(ne, 1));
+  L.sort_cap = sn;
+  L.hist_blocks = (sn + kSortTile - 1) / kSortTile;
+  L.vbox = take(sizeof(Box6) * nv);
+  L.fbox = take(sizeof(Box6) * nf);
+  L.ebox = take(sizeof(Box6) * ne);
+  L.objkey = take(sizeof(uint64_t) * 6 * nobj);
+  L.allowed = take(size_t(nobj) * nobj);
+  L.keys0 = take(4 * sn);
+  L.vals0 = take(4 * sn);
+  L.keys1 = take(4 * sn);
+  L.vals1 = take(4 * sn);
+  L.hist = take(4 * 256 * L.hist_blocks);
+  L.fnodes = take(sizeof(Node) * std::max<int64_t>(2 * nf - 1, 1));
+  L.fparent = take(4 * std::max<int64_t>(2 * nf - 1, 1));
+  L.fflag = take(4 * std::max<int64_t>(nf, 1));
+  L.frcs = take(4 * std::max<int64_t>(nf, 1));
+  L.fhi = take(4 * std::max<int64_t>(nf, 1));
+  L.enodes = take(sizeof(Node) * std::max<int64_t>(2 * ne - 1, 1));
+  L.eparent = take(4 * std::max<int64_t>(2 * ne - 1, 1));
+  L.eflag = take(4 * std::max<int64_t>(ne, 1));
+  L.ercs = take(4 * std::max<int64_t>(ne, 1));
+  L.ehi = take(4 * std::max<int64_t>(ne, 1));
+  L.qorder = take(4 * std::max<int64_t>(nv, 1));
+  L.eorder = take(4 * std::max<int64_t>(ne, 1));
+  L.counts = take(4 * (nv + ne + 1));
+  L.offsets = take(8 * (nv + ne + 1));
+  L.hist_off = take(8 * 256 * L.hist_blocks);
+  const int64_t scan_items = std::max<int64_t>(nv + ne + 1, 256 * L.hist_blocks);
+  L.scan_part = take(8 * (scan_items / 1024 + 2));
+  L.err = take(16);
+  L.total = o;
+  return L;
+}
+
+template <typename T>
+T* at(void* ws, size_t off) {
+  return reinterpret_cast<T*>(static_cast<char*>(ws) + off);
+}
+
+// ---- 1. boxes and object envelopes -------------------------------------
+__global__ void objkey_init_kernel(uint64_t* objkey, int32_t nobj) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nobj * 6) objkey[i] = (i % 6) < 3 ? ~0ull : 0ull;
+}
+
+__global__ __launch_bounds__(kBlock) void vertex_box_kernel(const double* __restrict__ x0,
+                                                            const double* __restrict__ x1, int64_t nv, double th,
+                                                            const int32_t* __restrict__ vobj, int32_t nobj,
+                                                            Box6* __restrict__ vbox, uint64_t* __restrict__ objkey) {
+  __shared__ unsigned long long lkey[kMaxObjects * 6];
+  for (int i = threadIdx.x; i < nobj * 6; i += blockDim.x) lkey[i] = (i % 6) < 3 ? ~0ull : 0ull;
+  __syncthreads();
+  for (int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < nv; v += int64_t(gridDim.x) * blockDim.x) {
+    Box6 b;
+    box_reset(b);
+    box_extend(b, x0 + v * 3);
+    box_extend(b, x1 + v * 3);
+    box_expand(b, th);
+    vbox[v] = b;
+    const int32_t o = vobj[v];
+    if (o >= 0 && o < nobj) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        atomicMin(&lkey[o * 6 + k], (unsigned long long)f64_order_key(b.mn[k]));
+        atomicMax(&lkey[o * 6 + 3 + k], (unsigned long long)f64_order_key(b.mx[k]));
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nobj * 6; i += blockDim.x) {
+    if ((i % 6) < 3) atomicMin((unsigned long long*)&objkey[i], lkey[i]);
+    else atomicMax((unsigned long long*)&objkey[i], lkey[i]);
+  }
+}
+
+__device__ inline void object_box(const uint64_t* objkey, int o, double* mn, double* mx) {
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    mn[k] = f64_from_order_key(objkey[o * 6 + k]);
+    mx[k] = f64_from_order_key(objkey[o * 6 + 3 + k]);
+  }
+}
+
+__global__ void allowed_kernel(const uint64_t* __restrict__ objkey, int32_t nobj, uint8_t* __restrict__ allowed) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nobj * nobj) return;
+  const int a = i / nobj, b = i % nobj;
+  uint8_t ok = 1;
+  if (a != b) {
+    double amn[3], amx[3], bmn[3], bmx[3];
+    object_box(objkey, a, amn, amx);
+    object_box(objkey, b, bmn, bmx);
+    ok = (amn[0] <= bmx[0] && bmn[0] <= amx[0] && amn[1] <= bmx[1] && bmn[1] <= amx[1] && amn[2] <= bmx[2] &&
+          bmn[2] <= amx[2])
+             ? 1
+             : 0;
+  }
+  allowed[i] = ok;
+}
+
+__device__ inline uint32_t spread10(uint32_t x) {
+  x &= 0x3ffu;
+  x = (x | (x << 16)) & 0x030000ffu;
+  x = (x | (x << 8)) & 0x0300f00fu;
+  x = (x | (x << 4)) & 0x030c30c3u;
+  x = (x | (x << 2)) & 0x09249249u;
+  return x;
+}
+
+__device__ inline uint32_t morton_of(const Box6& b, const double* lo, const double* inv) {
+  uint32_t c[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    double u = (0.5 * (b.mn[k] + b.mx[k]) - lo[k]) * inv[k];
+    u = u < 0.0 ? 0.0 : (u > 1.0 ? 1.0 : u);
+    c[k] = (uint32_t)(u * 1023.0);
+  }
+  return (spread10(c[0]) << 2) | (spread10(c[1]) << 1) | spread10(c[2]);
+}
+
+__device__ inline void scene_bounds(const uint64_t* objkey, int32_t nobj, double* lo, double* inv) {
+  double hi[3];
+  for (int k = 0; k < 3; ++k) {
+    lo[k] = __builtin_inf();
+    hi[k] = -__builtin_inf();
+  }
+  for (int o = 0; o < nobj; ++o) {
+    double mn[3], mx[3];
+    object_box(objkey, o, mn, mx);
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = mn[k] < lo[k] ? mn[k] : lo[k];
+      hi[k] = mx[k] > hi[k] ? mx[k] : hi[k];
+    }
+  }
+  for (int k = 0; k < 3; ++k) {
+    const double ext = hi[k] - lo[k];
+    inv[k] = ext > 0.0 ? 1.0 / ext : 0.0;
+  }
+}
+
+// face (kind 0) or edge (kind 1) boxes + Morton keys; vertex keys (kind 2)
+__global__ __launch_bounds__(kBlock) void prim_box_kernel(int kind, const double* __restrict__ x0,
+                                                          const double* __restrict__ x1,
+                                                          const int32_t* __restrict__ idx, int64_t n, double th,
+                                                          const Box6* __restrict__ vbox,
+                                                          const uint64_t* __restrict__ objkey, int32_t nobj,
+                                                          Box6* __restrict__ box, uint32_t* __restrict__ keys,
+                                                          uint32_t* __restrict__ vals) {
+  double lo[3], inv[3];
+  scene_bounds(objkey, nobj, lo, inv);
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    Box6 b;
+    if (kind == 2) {
+      b = vbox[i];
+    } else {
+      box_reset(b);
+      const int m = kind == 0 ? 3 : 2;
+      for (int c = 0; c < m; ++c) {
+        const int64_t v = idx[i * m + c];
+        box_extend(b, x0 + v * 3);
+        box_extend(b, x1 + v * 3);
+      }
+      box_expand(b, th);
+      box[i] = b;
+    }
+    keys[i] = morton_of(b, lo, inv);
+    vals[i] = (uint32_t)i;
+  }
+}
+
+// ---- 2. radix sort -------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void radix_count_kernel(const uint32_t* __restrict__ keys, int64_t n, int shift,
+                                                             uint32_t* __restrict__ hist, int64_t nblk) {
+  __shared__ uint32_t cnt[256];
+  cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t base = int64_t(blockIdx.x) * kSortTile;
+  for (int j = 0; j < kSortItems; ++j) {
+    const int64_t e = base + int64_t(j) * kBlock + threadIdx.x;
+    if (e < n) atomicAdd(&cnt[(keys[e] >> shift) & 0xffu], 1u);
+  }
+  __syncthreads();
+  hist[int64_t(threadIdx.x) * nblk + blockIdx.x] = cnt[threadIdx.x];
+}
+
+// Stable scatter.  Wave w of a tile owns the contiguous chunk
+// [base + w*64*kSortItems, +64*kSortItems) and walks it 64 keys per round;
+// a key's rank among equal digits in its round comes from 8 ballots, its
+// offset in the wave from the wave's LDS digit counters, and the wave's
+// offset in the tile from a prefix over the waves' counters.
+__global__ __launch_bounds__(kBlock) void radix_scatter_kernel(const uint32_t* __restrict__ kin,
+                                                               const uint32_t* __restrict__ vin, int64_t n, int shift,
+                                                               const uint64_t* __restrict__ hist_off, int64_t nblk,
+                                                               uint32_t* __restrict__ kout,
+                                                               uint32_t* __restrict__ vout) {
+  constexpr int kWaves = kBlock / 64;
+  __shared__ uint32_t wcnt[kWaves][256];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int i = threadIdx.x; i < kWaves * 256; i += kBlock) (&wcnt[0][0])[i] = 0;
+  __syncthreads();
+  const int64_t chunk = int64_t(blockIdx.x) * kSortTile + int64_t(w) * 64 * kSortItems;
+  uint32_t key[kSortItems], val[kSortItems], pos[kSortItems];
+  const uint64_t lt = (1ull << lane) - 1ull;
+#pragma unroll
+  for (int j = 0; j < kSortItems; ++j) {
+    const int64_t e = chunk + int64_t(j) * 64 + lane;
+    const bool valid = e < n;
+    key[j] = valid ? kin[e] : 0u;
+    val[j] = valid ? vin[e] : 0u;
+    const uint32_t d = (key[j] >> shift) & 0xffu;
+    uint64_t same = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const uint64_t bal = __ballot(valid && ((d >> b) & 1u));
+      same &= ((d >> b) & 1u) ? bal : ~bal;
+    }
+    const uint32_t rank = (uint32_t)__popcll(same & lt);
+    const uint32_t base = valid ? wcnt[w][d] : 0u;
+    pos[j] = base + rank;
+    __builtin_amdgcn_wave_barrier();
+    if (valid && rank == 0) wcnt[w][d] = base + (uint32_t)__popcll(same);
+    __builtin_amdgcn_wave_barrier();
+  }
+  __syncthreads();
+  // exclusive prefix over waves, per digit (one thread per digit)
+  {
+    const int d = threadIdx.x;
+    uint32_t run = 0;
+#pragma unroll
+    for (int ww = 0; ww < kWaves; ++ww) {
+      const uint32_t c = wcnt[ww][d];
+      wcnt[ww][d] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kSortItems; ++j) {
+    const int64_t e = chunk + int64_t(j) * 64 + lane;
+    if (e < n) {
+      const uint32_t d = (key[j] >> shift) & 0xffu;
+      const uint64_t dst = hist_off[int64_t(d) * nblk + blockIdx.x] + wcnt[w][d] + pos[j];
+      kout[dst] = key[j];
+      vout[dst] = val[j];
+    }
+  }
+}
+
+// ---- exclusive scan (u32 -> u64), three phases --------------------------
+__device__ inline uint64_t wave_inclusive(uint64_t x) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint64_t y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  return x;
+}
+
+// Block-wide exclusive scan of 4 items per thread (1024 per block).
+__device__ inline void block_scan4(uint64_t v[4], uint64_t* block_total) {
+  __shared__ uint64_t wsum[kBlock / 64];
+  uint64_t s = 0;
+  uint64_t t[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    t[i] = s;
+    s += v[i];
+  }
+  const uint64_t inc = wave_inclusive(s);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 63) wsum[w] = inc;
+  __syncthreads();
+  uint64_t wbase = 0, tot = 0;
+  for (int i = 0; i < kBlock / 64; ++i) {
+    if (i < w) wbase += wsum[i];
+    tot += wsum[i];
+  }
+  const uint64_t excl = wbase + inc - s;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) v[i] = excl + t[i];
+  *block_total = tot;
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(kBlock) void scan_local_kernel(const uint32_t* __restrict__ in, int64_t n,
+                                                            uint64_t* __restrict__ out, uint64_t* __restrict__ part) {
+  const int64_t base = int64_t(blockIdx.x) * 1024 + threadIdx.x * 4;
+  uint64_t v[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) v[i] = base + i < n ? in[base + i] : 0;
+  uint64_t tot;
+  block_scan4(v, &tot);
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    if (base + i < n) out[base + i] = v[i];
+  if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kBlock) void scan_part_kernel(uint64_t* __restrict__ part, int64_t m,
+                                                           uint64_t* __restrict__ grand) {
+  uint64_t carry = 0;
+  for (int64_t c = 0; c < m; c += 1024) {
+    const int64_t base = c + threadIdx.x * 4;
+    uint64_t v[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) v[i] = base + i < m ? part[base + i] : 0;
+    uint64_t tot;
+    block_scan4(v, &tot);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (base + i < m) part[base + i] = carry + v[i];
+    carry += tot;
+  }
+  if (threadIdx.x == 0) *grand = carry;
+}
+
+__global__ __launch_bounds__(kBlock) void scan_add_kernel(uint64_t* __restrict__ out, int64_t n,
+                                                          const uint64_t* __restrict__ part) {
+  const int64_t base = int64_t(blockIdx.x) * 1024 + threadIdx.x * 4;
+  const uint64_t add = part[blockIdx.x];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    if (base + i < n) out[base + i] += add;
+}
+
+// ---- 3. Karras build, escapes, refit ------------------------------------
+__device__ inline int delta(const uint32_t* keys, int64_t n, int64_t i, int64_t j) {
+  if (j < 0 || j >= n) return -1;
+  const uint64_t a = (uint64_t(keys[i]) << 32) | uint64_t(i);
+  const uint64_t b = (uint64_t(keys[j]) << 32) | uint64_t(j);
+  return __clzll(a ^ b);
+}
+
+__global__ __launch_bounds__(kBlock) void karras_kernel(const uint32_t* __restrict__ keys, int64_t n,
+                                                        Node* __restrict__ nodes, int32_t* __restrict__ parent,
+                                                        int32_t* __restrict__ rcs, int32_t* __restrict__ hi_of) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n - 1; i += int64_t(gridDim.x) * blockDim.x) {
+    const int d = delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1) >= 0 ? 1 : -1;
+    const int dmin = delta(keys, n, i, i - d);
+    int64_t lmax = 2;
+    while (delta(keys, n, i, i + lmax * d) > dmin) lmax <<= 1;
+    int64_t l = 0;
+    for (int64_t t = lmax >> 1; t >= 1; t >>= 1)
+      if (delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
+    const int64_t j = i + l * d;
+    const int dnode = delta(keys, n, i, j);
+    int64_t s = 0, step = l;
+    do {
+      step = (step + 1) >> 1;
+      if (delta(keys, n, i, i + (s + step) * d) > dnode) s += step;
+    } while (step > 1);
+    const int64_t gamma = i + s * d + (d < 0 ? -1 : 0);
+    const int64_t lo = i < j ? i : j, hi = i < j ? j : i;
+    const int32_t left = lo == gamma ? int32_t(n - 1 + gamma) : int32_t(gamma);
+    const int32_t right = hi == gamma + 1 ? int32_t(n - 1 + gamma + 1) : int32_t(gamma + 1);
+    nodes[i].left = left;
+    nodes[i].right = right;
+    nodes[i].prim = -1;
+    parent[left] = int32_t(i);
+    parent[right] = int32_t(i);
+    rcs[gamma] = right;
+    hi_of[i] = int32_t(hi);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void leaves_kernel(const uint32_t* __restrict__ vals, int64_t n,
+                                                        const Box6* __restrict__ box, Node* __restrict__ nodes,
+                                                        const int32_t* __restrict__ rcs,
+                                                        const int32_t* __restrict__ hi_of,
+                                                        int32_t* __restrict__ parent, uint32_t* __restrict__ flag) {
+  for (int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; p < 2 * n - 1;
+       p += int64_t(gridDim.x) * blockDim.x) {
+    if (p < n - 1) {  // internal node p: escape from its range max; reset refit flag
+      const int64_t hi = hi_of[p];
+      nodes[p].escape = hi == n - 1 ? -1 : rcs[hi];
+      flag[p] = 0;
+    } else {
+      const int64_t q = p - (n - 1);
+      Node leaf;
+      const Box6 b = box[vals[q]];
+      for (int k = 0; k < 3; ++k) {
+        leaf.mn[k] = b.mn[k];
+        leaf.mx[k] = b.mx[k];
+      }
+      leaf.left = leaf.right = -1;
+      leaf.escape = q == n - 1 ? -1 : rcs[q];
+      leaf.prim = int32_t(vals[q]);
+      nodes[p] = leaf;
+      if (n == 1) parent[p] = -1;
+    }
+    if (p == 0 && n > 1) parent[0] = -1;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void refit_kernel(int64_t n, Node* __restrict__ nodes,
+                                                       const int32_t* __restrict__ parent,
+                                                       uint32_t* __restrict__ flag) {
+  for (int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; q < n; q += int64_t(gridDim.x) * blockDim.x) {
+    int32_t node = parent[n - 1 + q];
+    while (node >= 0) {
+      // The first child to arrive stops; the second sees both children's
+      // boxes (release by the first, acquire here) and builds the parent's.
+      const uint32_t prev =
+          __hip_atomic_fetch_add(&flag[node], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+      if (prev == 0) break;
+      const Node& a = nodes[nodes[node].left];
+      const Node& b = nodes[nodes[node].right];
+      for (int k = 0; k < 3; ++k) {
+        const double amn = a.mn[k], bmn = b.mn[k];
+        const double amx = a.mx[k], bmx = b.mx[k];
+        nodes[node].mn[k] = bmn < amn ? bmn : amn;
+        nodes[node].mx[k] = amx < bmx ? bmx : amx;
+      }
+      node = parent[node];
+    }
+  }
+}
+
+// ---- 4. traversal ------------------------------------------------------
+struct VfRule {
+  const int32_t* faces;
+  const int32_t* fobj;
+  const int32_t* vobj;
+  const uint8_t* allowed;
+  int32_t nobj;
+};
+
+struct EeRule {
+  const int32_t* edges;
+  const int32_t* eobj;
+  const uint8_t* allowed;
+  int32_t nobj;
+};
+
+__device__ inline bool pair_allowed(const uint8_t* allowed, int32_t nobj, int32_t a, int32_t b) {
+  if (a < 0 || b < 0 || a >= nobj || b >= nobj) return false;
+  return allowed[a * nobj + b] != 0;
+}
+
+// Walks the BVH for query box q, calling emit(prim) for every leaf whose box
+// overlaps; returns the number of accepted leaves.
+// A pre-order walk visits each of the 2n-1 nodes at most once; `max_steps`
+// bounds the loop regardless, and an exhausted bound is counted in *err.
+template <typename Accept, typename Emit>
+__device__ inline uint32_t traverse(const Node* __restrict__ nodes, const Box6& q, int64_t max_steps,
+                                    uint32_t* err, Accept accept, Emit emit) {
+  uint32_t count = 0;
+  int32_t node = 0;
+  int64_t steps = 0;
+  while (node >= 0) {
+    if (++steps > max_steps) {
+      atomicAdd(err, 1u);
+      break;
+    }
+    const Node& N = nodes[node];
+    const double mn[3] = {N.mn[0], N.mn[1], N.mn[2]};
+    const double mx[3] = {N.mx[0], N.mx[1], N.mx[2]};
+    const int32_t left = N.left, escape = N.escape, prim = N.prim;
+    if (overlap(q, mn, mx)) {
+      if (left < 0) {
+        if (accept(prim)) {
+          emit(prim, count);
+          ++count;
+        }
+        node = escape;
+      } else {
+        node = left;
+      }
+    } else {
+      node = escape;
+    }
+  }
+  return count;
+}
+
+template <bool kFill>
+__global__ __launch_bounds__(kBlock) void vf_query_kernel(const Node* __restrict__ fnodes, int64_t nf,
+                                                          const Box6* __restrict__ vbox,
+                                                          const uint32_t* __restrict__ qorder, int64_t nv, VfRule r,
+                                                          uint32_t* __restrict__ err, uint32_t* __restrict__ counts,
+                                                          const uint64_t* __restrict__ offsets,
+                                                          int32_t* __restrict__ out, int64_t cap) {
+  for (int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; s < nv; s += int64_t(gridDim.x) * blockDim.x) {
+    const int32_t v = int32_t(qorder[s]);
+    const int32_t ov = r.vobj[v];
+    uint32_t c = 0;
+    if (ov >= 0 && nf > 0) {
+      const Box6 q = vbox[v];
+      const uint64_t base = kFill ? offsets[s] : 0;
+      c = traverse(
+          fnodes, q, 2 * nf, err,
+          [&](int32_t f) {
+            const int32_t of = r.fobj[f];
+            if (!pair_allowed(r.allowed, r.nobj, ov, of)) return false;
+            if (ov == of) {
+              const int32_t* fv = r.faces + int64_t(f) * 3;
+              if (fv[0] == v || fv[1] == v || fv[2] == v) return false;
+            }
+            return true;
+          },
+          [&](int32_t f, uint32_t k) {
+            if (kFill && base + k < uint64_t(cap)) {
+              out[(base + k) * 2] = v;
+              out[(base + k) * 2 + 1] = f;
+            }
+          });
+    }
+    if (!kFill) counts[s] = c;
+  }
+}
+
+template <bool kFill>
+__global__ __launch_bounds__(kBlock) void ee_query_kernel(const Node* __restrict__ enodes, int64_t ne,
+                                                          const Box6* __restrict__ ebox, const uint32_t* __restrict__ eorder,
+                                                          EeRule r, uint32_t* __restrict__ err,
+                                                          uint32_t* __restrict__ counts,
+                                                          const uint64_t* __restrict__ offsets, int64_t nv,
+                                                          int32_t* __restrict__ out, int64_t cap) {
+  for (int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; s < ne; s += int64_t(gridDim.x) * blockDim.x) {
+    const int32_t i = int32_t(eorder[s]);
+    const Box6 q = ebox[i];
+    const int32_t oi = r.eobj[i];
+    const int32_t a0 = r.edges[int64_t(i) * 2], a1 = r.edges[int64_t(i) * 2 + 1];
+    const uint64_t base = kFill ? offsets[nv + s] - offsets[nv] : 0;
+    const uint32_t c = traverse(
+        enodes, q, 2 * ne, err,
+        [&](int32_t j) {
+          if (i >= j) return false;
+          const int32_t oj = r.eobj[j];
+          if (!pair_allowed(r.allowed, r.nobj, oi, oj)) return false;
+          if (oi == oj) {
+            const int32_t b0 = r.edges[int64_t(j) * 2], b1 = r.edges[int64_t(j) * 2 + 1];
+            if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) return false;
+          }
+          return true;
+        },
+        [&](int32_t j, uint32_t k) {
+          if (kFill && base + k < uint64_t(cap)) {
+            out[(base + k) * 2] = i;
+            out[(base + k) * 2 + 1] = j;
+          }
+        });
+    if (!kFill) counts[s] = c;
+  }
+}
+
+__global__ void totals_kernel(const uint64_t* __restrict__ offsets, int64_t nv, int64_t ne,
+                              const uint32_t* __restrict__ err, uint64_t* __restrict__ totals) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    totals[0] = offsets[nv];
+    totals[1] = offsets[nv + ne] - offsets[nv];
+    totals[2] = *err;
+  }
+}
+
+__global__ void zero_u32_kernel(uint32_t* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// ---- host-side drivers ----------------------------------------------------
+int check_mesh(const p2c_bp_mesh* m) {
+  if (m == nullptr) return set_error(P2C_EINVAL, "mesh descriptor is required");
+  if (m->n_vertices < 0 || m->n_faces < 0 || m->n_edges < 0)
+    return set_error(P2C_EINVAL, "mesh sizes must be non-negative");
+  if (m->n_objects < 1 || m->n_objects > kMaxObjects)
+    return set_error(P2C_EUNSUPPORTED, "object count must be in [1, 64] per scene");
+  if (m->n_faces > (int64_t(1) << 30) || m->n_edges > (int64_t(1) << 30) || m->n_vertices > (int64_t(1) << 30))
+    return set_error(P2C_EUNSUPPORTED, "meshes above 2^30 primitives are not supported");
+  return P2C_OK;
+}
+
+int radix_sort(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, int64_t n, uint32_t* hist, uint64_t* hist_off,
+               uint64_t* part, hipStream_t s) {
+  if (n <= 1) return P2C_OK;
+  const int64_t nblk = (n + kSortTile - 1) / kSortTile;
+  for (int pass = 0; pass < 4; ++pass) {
+    const int shift = pass * 8;
+    hipLaunchKernelGGL(radix_count_kernel, dim3(unsigned(nblk)), dim3(kBlock), 0, s, k0, n, shift, hist, nblk);
+    const int64_t m = 256 * nblk;
+    const int64_t sb = (m + 1023) / 1024;
+    hipLaunchKernelGGL(scan_local_kernel, dim3(unsigned(sb)), dim3(kBlock), 0, s, hist, m, hist_off, part);
+    hipLaunchKernelGGL(scan_part_kernel, dim3(1), dim3(kBlock), 0, s, part, sb, part + sb);
+    hipLaunchKernelGGL(scan_add_kernel, dim3(unsigned(sb)), dim3(kBlock), 0, s, hist_off, m, part);
+    hipLaunchKernelGGL(radix_scatter_kernel, dim3(unsigned(nblk)), dim3(kBlock), 0, s, k0, v0, n, shift, hist_off,
+                       nblk, k1, v1);
+    std::swap(k0, k1);
+    std::swap(v0, v1);
+  }
+  P2C_HIP_CHECK(hipGetLastError());
+  return P2C_OK;  // 4 passes: result back in the first buffers
+}
+
+}  // namespace
+}  // namespace p2c
+
+extern "C" {
+
+size_t p2c_bp_workspace_bytes(int64_t nv, int64_t nf, int64_t ne, int32_t nobj) {
+  return p2c::make_layout(nv, nf, ne, nobj).total;
+}
+
+int p2c_bp_build(const p2c_bp_mesh* m, const double* x0, const double* x1, double thickness, void* ws,
+                 size_t ws_bytes, void* stream) {
+  using namespace p2c;
+  if (int rc = check_mesh(m)) return rc;
+  if (!std::isfinite(thickness) || thickness < 0.0)
+    return set_error(P2C_EINVAL, "thickness must be finite and non-negative");
+  const Layout L = make_layout(m->n_vertices, m->n_faces, m->n_edges, m->n_objects);
+  if (ws == nullptr || ws_bytes < L.total) return set_error(P2C_ECAPACITY, "broad-phase workspace is too small");
+  hipStream_t s = as_stream(stream);
+  const int64_t nv = m->n_vertices, nf = m->n_faces, ne = m->n_edges;
+  const int32_t nobj = m->n_objects;
+  uint64_t* objkey = at<uint64_t>(ws, L.objkey);
+  Box6* vbox = at<Box6>(ws, L.vbox);
+  hipLaunchKernelGGL(objkey_init_kernel, dim3(1), dim3(kMaxObjects * 6), 0, s, objkey, nobj);
+  hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, at<uint32_t>(ws, L.err), 4);
+  if (nv > 0)
+    hipLaunchKernelGGL(vertex_box_kernel, dim3(grid_for(nv, kBlock, 2048)), dim3(kBlock), 0, s, x0, x1, nv, thickness,
+                       m->vertex_object, nobj, vbox, objkey);
+  hipLaunchKernelGGL(allowed_kernel, dim3(unsigned((nobj * nobj + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, objkey,
+                     nobj, at<uint8_t>(ws, L.allowed));
+  uint32_t* k0 = at<uint32_t>(ws, L.keys0);
+  uint32_t* v0 = at<uint32_t>(ws, L.vals0);
+  uint32_t* k1 = at<uint32_t>(ws, L.keys1);
+  uint32_t* v1 = at<uint32_t>(ws, L.vals1);
+  uint32_t* hist = at<uint32_t>(ws, L.hist);
+  uint64_t* hist_off = at<uint64_t>(ws, L.hist_off);
+  uint64_t* part = at<uint64_t>(ws, L.scan_part);
+  struct Tree {
+    int kind;
+    const int32_t* idx;
+    int64_t n;
+    size_t box, nodes, parent, flag, rcs, hi;
+  } trees[2] = {{0, m->faces, nf, L.fbox, L.fnodes, L.fparent, L.fflag, L.frcs, L.fhi},
+                {1, m->edges, ne, L.ebox, L.enodes, L.eparent, L.eflag, L.ercs, L.ehi}};
+  for (const Tree& t : trees) {
+    if (t.n == 0) continue;
+    hipLaunchKernelGGL(prim_box_kernel, dim3(grid_for(t.n, kBlock)), dim3(kBlock), 0, s, t.kind, x0, x1, t.idx, t.n,
+                       thickness, vbox, objkey, nobj, at<Box6>(ws, t.box), k0, v0);
+    if (int rc = radix_sort(k0, v0, k1, v1, t.n, hist, hist_off, part, s)) return rc;
+    Node* nodes = at<Node>(ws, t.nodes);
+    int32_t* parent = at<int32_t>(ws, t.parent);
+    uint32_t* flag = at<uint32_t>(ws, t.flag);
+    if (t.n > 1)
+      hipLaunchKernelGGL(karras_kernel, dim3(grid_for(t.n - 1, kBlock)), dim3(kBlock), 0, s, k0, t.n, nodes, parent,
+                         at<int32_t>(ws, t.rcs), at<int32_t>(ws, t.hi));
+    hipLaunchKernelGGL(leaves_kernel, dim3(grid_for(2 * t.n - 1, kBlock)), dim3(kBlock), 0, s, v0, t.n,
+                       at<Box6>(ws, t.box), nodes, at<int32_t>(ws, t.rcs), at<int32_t>(ws, t.hi), parent, flag);
+    if (t.n > 1)
+      hipLaunchKernelGGL(refit_kernel, dim3(grid_for(t.n, kBlock)), dim3(kBlock), 0, s, t.n, nodes, parent, flag);
+    // the edge leaf order (sorted edge ids) is also the EE query order
+    if (t.kind == 1)
+      P2C_HIP_CHECK(hipMemcpyAsync(at<uint32_t>(ws, L.eorder), v0, sizeof(uint32_t) * t.n, hipMemcpyDeviceToDevice, s));
+  }
+  // vertex query order: vertices sorted by Morton code of their swept box
+  if (nv > 0) {
+    hipLaunchKernelGGL(prim_box_kernel, dim3(grid_for(nv, kBlock)), dim3(kBlock), 0, s, 2, x0, x1, nullptr, nv,
+                       thickness, vbox, objkey, nobj, nullptr, k0, v0);
+    if (int rc = radix_sort(k0, v0, k1, v1, nv, hist, hist_off, part, s)) return rc;
+    P2C_HIP_CHECK(hipMemcpyAsync(at<uint32_t>(ws, L.qorder), v0, sizeof(uint32_t) * nv, hipMemcpyDeviceToDevice, s));
+  }
+  P2C_HIP_CHECK(hipGetLastError());
+  return P2C_OK;
+}
+
+int p2c_bp_count(const p2c_bp_mesh* m, void* ws, uint64_t* totals, void* stream) {
+  using namespace p2c;
+  if (int rc = check_mesh(m)) return rc;
+  const Layout L = make_layout(m->n_vertices, m->n_faces, m->n_edges, m->n_objects);
+  hipStream_t s = as_stream(stream);
+  const int64_t nv = m->n_vertices, nf = m->n_faces, ne = m->n_edges;
+  uint32_t* counts = at<uint32_t>(ws, L.counts);
+  uint64_t* offsets = at<uint64_t>(ws, L.offsets);
+  VfRule vr{m->faces, m->face_object, m->vertex_object, at<uint8_t>(ws, L.allowed), m->n_objects};
+  EeRule er{m->edges, m->edge_object, at<uint8_t>(ws, L.allowed), m->n_objects};
+  if (nv > 0)
+    hipLaunchKernelGGL(vf_query_kernel<false>, dim3(grid_for(nv, kBlock)), dim3(kBlock), 0, s,
+                       at<Node>(ws, L.fnodes), nf, at<Box6>(ws, L.vbox), at<uint32_t>(ws, L.qorder), nv, vr,
+                       at<uint32_t>(ws, L.err), counts, nullptr, nullptr, 0);
+  if (ne > 0)
+    hipLaunchKernelGGL(ee_query_kernel<false>, dim3(grid_for(ne, kBlock)), dim3(kBlock), 0, s,
+                       at<Node>(ws, L.enodes), ne, at<Box6>(ws, L.ebox), at<uint32_t>(ws, L.eorder), er,
+                       at<uint32_t>(ws, L.err), counts + nv, nullptr, nv, nullptr, 0);
+  const int64_t n = nv + ne;
+  const int64_t sb = (n + 1023) / 1024;
+  uint64_t* part = at<uint64_t>(ws, L.scan_part);
+  if (n > 0) {
+    hipLaunchKernelGGL(scan_local_kernel, dim3(unsigned(sb)), dim3(kBlock), 0, s, counts, n, offsets, part);
+    hipLaunchKernelGGL(scan_part_kernel, dim3(1), dim3(kBlock), 0, s, part, sb, offsets + n);
+    hipLaunchKernelGGL(scan_add_kernel, dim3(unsigned(sb)), dim3(kBlock), 0, s, offsets, n, part);
+  }
+  hipLaunchKernelGGL(totals_kernel, dim3(1), dim3(64), 0, s, offsets, nv, ne, at<uint32_t>(ws, L.err), totals);
+  P2C_HIP_CHECK(hipGetLastError());
+  return P2C_OK;
+}
+
+int p2c_bp_emit(const p2c_bp_mesh* m, void* ws, int32_t* vf_pairs, int64_t vf_cap, int32_t* ee_pairs,
+                int64_t ee_cap, void* stream) {
+  using namespace p2c;
+  if (int rc = check_mesh(m)) return rc;
+  const Layout L = make_layout(m->n_vertices, m->n_faces, m->n_edges, m->n_objects);
+  hipStream_t s = as_stream(stream);
+  const int64_t nv = m->n_vertices, nf = m->n_faces, ne = m->n_edges;
+  const uint64_t* offsets = at<uint64_t>(ws, L.offsets);
+  VfRule vr{m->faces, m->face_object, m->vertex_object, at<uint8_t>(ws, L.allowed), m->n_objects};
+  EeRule er{m->edges, m->edge_object, at<uint8_t>(ws, L.allowed), m->n_objects};
+  if (nv > 0 && vf_pairs != nullptr)
+    hipLaunchKernelGGL(vf_query_kernel<true>, dim3(grid_for(nv, kBlock)), dim3(kBlock), 0, s,
+                       at<Node>(ws, L.fnodes), nf, at<Box6>(ws, L.vbox), at<uint32_t>(ws, L.qorder), nv, vr,
+                       at<uint32_t>(ws, L.err), nullptr, offsets, vf_pairs, vf_cap);
+  if (ne > 0 && ee_pairs != nullptr)
+    hipLaunchKernelGGL(ee_query_kernel<true>, dim3(grid_for(ne, kBlock)), dim3(kBlock), 0, s,
+                       at<Node>(ws, L.enodes), ne, at<Box6>(ws, L.ebox), at<uint32_t>(ws, L.eorder), er,
+                       at<uint32_t>(ws, L.err), nullptr, offsets, nv, ee_pairs, ee_cap);
+  P2C_HIP_CHECK(hipGetLastError());
+  return P2C_OK;
+}
+
+}  // extern "C"

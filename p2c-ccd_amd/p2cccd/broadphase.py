@@ -1,0 +1,88 @@
+"""LBVH broad phase on the GPU: host driver over the p2c_bp_* C-ABI.
+
+A `BroadPhase` holds one mesh's topology on the device (faces, edges, object
+ids -- prepared once, as the reference's PrepareScene does) and a reusable
+workspace; each step builds both BVHs from that step's positions and emits
+the VF and EE candidate pairs.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _native as nat
+from .scenes import Scene, Topology
+
+
+class _Mesh(ctypes.Structure):
+    _fields_ = [
+        ("faces", ctypes.c_void_p),
+        ("edges", ctypes.c_void_p),
+        ("vertex_object", ctypes.c_void_p),
+        ("face_object", ctypes.c_void_p),
+        ("edge_object", ctypes.c_void_p),
+        ("n_vertices", ctypes.c_int64),
+        ("n_faces", ctypes.c_int64),
+        ("n_edges", ctypes.c_int64),
+        ("n_objects", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+_vp, _i64, _i32 = nat._vp, nat._i64, nat._i32
+nat.lib.p2c_bp_workspace_bytes.argtypes = [_i64, _i64, _i64, _i32]
+nat.lib.p2c_bp_workspace_bytes.restype = ctypes.c_size_t
+nat._sig("p2c_bp_build", [_vp, _vp, _vp, nat._f64, _vp, ctypes.c_size_t, _vp])
+nat._sig("p2c_bp_count", [_vp, _vp, _vp, _vp])
+nat._sig("p2c_bp_emit", [_vp, _vp, _vp, _i64, _vp, _i64, _vp])
+
+
+class BroadPhase:
+    def __init__(self, topo: Topology, n_vertices: int, device: str | torch.device = "cuda"):
+        self.device = torch.device(device)
+        dev = self.device
+
+        def up(a):
+            return torch.from_numpy(np.ascontiguousarray(a, np.int32)).to(dev)
+
+        self.faces, self.edges = up(topo.faces), up(topo.edges)
+        self.vobj, self.fobj, self.eobj = up(topo.vertex_object), up(topo.face_object), up(topo.edge_object)
+        self.n_vertices, self.n_faces, self.n_edges = n_vertices, topo.faces.shape[0], topo.edges.shape[0]
+        self.mesh = _Mesh(self.faces.data_ptr(), self.edges.data_ptr(), self.vobj.data_ptr(), self.fobj.data_ptr(),
+                          self.eobj.data_ptr(), self.n_vertices, self.n_faces, self.n_edges, topo.object_count, 0)
+        self.ws_bytes = int(nat.lib.p2c_bp_workspace_bytes(self.n_vertices, self.n_faces, self.n_edges,
+                                                           topo.object_count))
+        self.workspace = torch.empty(self.ws_bytes, dtype=torch.uint8, device=dev)
+        self.totals = torch.zeros(3, dtype=torch.int64, device=dev)
+
+    @classmethod
+    def for_scene(cls, scene: Scene, device="cuda") -> "BroadPhase":
+        return cls(scene.topo, scene.n_vertices, device)
+
+    def build(self, x0: torch.Tensor, x1: torch.Tensor, thickness: float, stream=None) -> None:
+        assert x0.dtype == torch.float64 and x0.is_cuda and x0.shape == (self.n_vertices, 3) and x0.is_contiguous()
+        assert x1.dtype == torch.float64 and x1.is_cuda and x1.shape == (self.n_vertices, 3) and x1.is_contiguous()
+        nat.call("p2c_bp_build", ctypes.addressof(self.mesh), x0.data_ptr(), x1.data_ptr(), float(thickness),
+                 self.workspace.data_ptr(), self.ws_bytes, nat.stream_handle(stream))
+
+    def count(self, stream=None) -> None:
+        nat.call("p2c_bp_count", ctypes.addressof(self.mesh), self.workspace.data_ptr(), self.totals.data_ptr(),
+                 nat.stream_handle(stream))
+
+    def emit(self, vf: torch.Tensor, ee: torch.Tensor, stream=None) -> None:
+        nat.call("p2c_bp_emit", ctypes.addressof(self.mesh), self.workspace.data_ptr(), vf.data_ptr(), vf.shape[0],
+                 ee.data_ptr(), ee.shape[0], nat.stream_handle(stream))
+
+    def candidates(self, x0: torch.Tensor, x1: torch.Tensor, thickness: float) -> tuple[torch.Tensor, torch.Tensor]:
+        """Build, count, read the totals (one host sync), allocate and emit."""
+        self.build(x0, x1, thickness)
+        self.count()
+        n_vf, n_ee, bad = (int(v) for v in self.totals.cpu())
+        if bad:
+            raise RuntimeError(f"{bad} BVH traversals exceeded their step bound")
+        vf = torch.empty((n_vf, 2), dtype=torch.int32, device=self.device)
+        ee = torch.empty((n_ee, 2), dtype=torch.int32, device=self.device)
+        self.emit(vf, ee)
+        return vf, ee
