@@ -156,14 +156,42 @@ int p2c_tuv_setup_positions(int is_ee, const double* v, int64_t n, double ms, do
  * idx4 (device (n, 4) int32 vertex ids in the same order as above). */
 int p2c_tuv_setup_indexed(int is_ee, const double* x0, const double* x1, const int32_t* idx4, int64_t n,
                           double ms, double tolerance, void* pairs, void* stream);
-/* One lane per pair (the plain one-launch path). */
-int p2c_tuv_plain(const void* pairs, int64_t n, double tolerance, int64_t max_itr, p2c_tuv_result* out,
-                  uint32_t* replay_list, uint32_t* replay_count, uint32_t replay_cap, void* stream);
-/* Persistent lane-refill kernel over `grid_blocks` blocks of 256; `counter`
- * is a device uint64 that must be zero at launch. */
-int p2c_tuv_persistent(const void* pairs, int64_t n, double tolerance, int64_t max_itr, p2c_tuv_result* out,
-                       unsigned long long* counter, uint32_t* replay_list, uint32_t* replay_count,
-                       uint32_t replay_cap, int grid_blocks, void* stream);
+/* Straight from broad-phase pair lists: VF pairs (vertex, face) with topo =
+ * faces (F, 3), or EE pairs (edge i, edge j) with topo = edges (E, 2).  With
+ * `totals` (the device array written by p2c_bp_count) the pair count is
+ * min(n, totals[is_ee]) and EE records are placed after the VF ones
+ * (offset totals[0]); without it, n pairs go to slot out_offset onwards.
+ * Records at or beyond slot out_cap are not written. */
+int p2c_tuv_setup_pairs(int is_ee, const double* x0, const double* x1, const int32_t* topo, const int32_t* pairs,
+                        int64_t n, const uint64_t* totals, int64_t out_offset, int64_t out_cap, double ms,
+                        double tolerance, void* tuv_pairs, void* stream);
+/* One lane per pair (the plain one-launch path).  With `totals` (device) the
+ * pair count is min(n, totals[0] + totals[1]). */
+int p2c_tuv_plain(const void* pairs, int64_t n, const uint64_t* totals, double tolerance, int64_t max_itr,
+                  p2c_tuv_result* out, uint32_t* replay_list, uint32_t* replay_count, uint32_t replay_cap,
+                  void* stream);
+/* Batched form, two kernels on `stream`: a persistent lane-refill pass over
+ * `grid_blocks` blocks of 256 in which a pair may take up to `light_budget`
+ * boxes, then a wave-cooperative pass in which the 64 lanes of a wave share
+ * each remaining pair's subdivision tree.  `counters` is 32 zero-initialised
+ * device bytes (next pair; replay count at byte 8; heavy count at byte 12;
+ * heavy cursor at byte 16).  heavy_list and replay_list hold list_cap
+ * entries.  `totals` as above. */
+int p2c_tuv_persistent(const void* pairs, int64_t n, const uint64_t* totals, double tolerance, int64_t max_itr,
+                       int64_t light_budget, p2c_tuv_result* out, void* counters, uint32_t* heavy_list,
+                       uint32_t* replay_list, uint32_t list_cap, int grid_blocks, void* stream);
+/* Batched form by rounds: round r walks each still-open pair for at most
+ * budgets[r] steps, one lane per pair, saving unfinished walks in `states`
+ * (n * p2c_tuv_state_bytes() device bytes) and compacting them into
+ * list_a / list_b (list_cap entries each); the last round's survivors go to
+ * the wave-cooperative pass.  `counters`: 64 zero-initialised device bytes
+ * (replay count at byte 8, heavy count at byte 12).  `budgets` is a host
+ * array of n_rounds (1..8) entries. */
+size_t p2c_tuv_state_bytes(void);
+int p2c_tuv_rounds(const void* pairs, int64_t n, const uint64_t* totals, double tolerance, int64_t max_itr,
+                   const uint32_t* budgets, int n_rounds, p2c_tuv_result* out, void* counters, void* states,
+                   uint32_t* list_a, uint32_t* list_b, uint32_t* replay_list, uint32_t list_cap, int grid_blocks,
+                   void* stream);
 /* Fallback replay of the listed pairs in canonical order; `heaps` holds
  * slots * p2c_tuv_heap_bytes(heap_cap) bytes of device memory. */
 int p2c_tuv_replay(const void* pairs, const uint32_t* replay_list, const uint32_t* replay_count,

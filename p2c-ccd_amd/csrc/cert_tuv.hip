@@ -73,17 +73,68 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_setup_indexed_kernel(int is_ee,
   }
 }
 
+// Pair count of a launch: the host bound, or min(bound, device totals) when
+// the counts were produced on the device (VF total, EE total).
+__device__ inline int64_t device_count(int64_t n, const uint64_t* totals) {
+  if (totals == nullptr) return n;
+  const int64_t t = int64_t(totals[0] + totals[1]);
+  return t < n ? t : n;
+}
+
+// VF pair (v, f) -> (v, f0, f1, f2); EE pair (i, j) -> (i0, i1, j0, j1).
+// With device totals, VF writes slots [0, n_vf) and EE [n_vf, n_vf + n_ee).
+__global__ __launch_bounds__(kTuvBlock) void tuv_setup_pairs_kernel(int is_ee, const double* __restrict__ x0,
+                                                                    const double* __restrict__ x1,
+                                                                    const int32_t* __restrict__ topo,
+                                                                    const int32_t* __restrict__ pairs, int64_t n,
+                                                                    const uint64_t* __restrict__ totals,
+                                                                    int64_t out_offset, int64_t out_cap, double ms,
+                                                                    double tol, TuvPair* __restrict__ out) {
+  int64_t m = n, off = out_offset;
+  if (totals != nullptr) {
+    const int64_t t = int64_t(totals[is_ee]);
+    m = t < n ? t : n;
+    off = is_ee ? int64_t(totals[0]) : 0;
+  }
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < m; i += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t a = pairs[i * 2], b = pairs[i * 2 + 1];
+    int64_t vid[4];
+    if (!is_ee) {
+      vid[0] = a;
+      vid[1] = topo[b * 3];
+      vid[2] = topo[b * 3 + 1];
+      vid[3] = topo[b * 3 + 2];
+    } else {
+      vid[0] = topo[a * 2];
+      vid[1] = topo[a * 2 + 1];
+      vid[2] = topo[b * 2];
+      vid[3] = topo[b * 2 + 1];
+    }
+    double x[8][3];
+    for (int r = 0; r < 4; ++r)
+      for (int k = 0; k < 3; ++k) {
+        x[r][k] = x0[vid[r] * 3 + k];
+        x[4 + r][k] = x1[vid[r] * 3 + k];
+      }
+    TuvPair q;
+    tuv_setup(&q, x, is_ee, ms, tol);
+    if (off + i < out_cap) out[off + i] = q;
+  }
+}
+
 __device__ inline void flag_for_replay(uint32_t pair, uint32_t* list, uint32_t* count, uint32_t cap) {
   const uint32_t slot = atomicAdd(count, 1u);
   if (slot < cap) list[slot] = pair;
 }
 
-__global__ __launch_bounds__(kTuvBlock) void tuv_plain_kernel(const TuvPair* __restrict__ pairs, int64_t n,
+__global__ __launch_bounds__(kTuvBlock) void tuv_plain_kernel(const TuvPair* __restrict__ pairs, int64_t n_host,
+                                                              const uint64_t* __restrict__ totals,
                                                               double tol, uint64_t max_itr,
                                                               p2c_tuv_result* __restrict__ out,
                                                               uint32_t* __restrict__ replay_list,
                                                               uint32_t* __restrict__ replay_count,
                                                               uint32_t replay_cap) {
+  const int64_t n = device_count(n_host, totals);
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
     const TuvPair q = pairs[i];
     uint64_t best = kTuvNoHit, boxes = 0;
@@ -141,13 +192,19 @@ __device__ inline bool lane_step(const TuvPair& q, LaneWalk& w, double tol, uint
   return true;
 }
 
-__global__ __launch_bounds__(kTuvBlock) void tuv_persistent_kernel(const TuvPair* __restrict__ pairs, int64_t n,
-                                                                   double tol, uint64_t max_itr,
+// Pass 1 of the batched form.  A pair whose walk needs more than
+// `light_budget` boxes is abandoned here and appended to the heavy list for
+// the wave-cooperative pass; its result is written there.
+__global__ __launch_bounds__(kTuvBlock) void tuv_persistent_kernel(const TuvPair* __restrict__ pairs,
+                                                                   int64_t n_host,
+                                                                   const uint64_t* __restrict__ totals,
+                                                                   double tol, uint64_t light_budget,
                                                                    p2c_tuv_result* __restrict__ out,
                                                                    unsigned long long* __restrict__ next_pair,
-                                                                   uint32_t* __restrict__ replay_list,
-                                                                   uint32_t* __restrict__ replay_count,
-                                                                   uint32_t replay_cap) {
+                                                                   uint32_t* __restrict__ heavy_list,
+                                                                   uint32_t* __restrict__ heavy_count,
+                                                                   uint32_t list_cap) {
+  const int64_t n = device_count(n_host, totals);
   const int lane = threadIdx.x & 63;
   int64_t pair = -1;  // -1: needs a pair, -2: none left
   TuvPair q;
@@ -180,11 +237,291 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_persistent_kernel(const TuvPair
     if (__ballot(pair >= 0) == 0) break;
     if (pair >= 0) {
       bool over = false;
-      if (lane_step(q, w, tol, max_itr, &over)) {
-        write_result(&out[pair], w.best, w.boxes, over ? P2C_TUV_FLAG_OVER_BUDGET : 0u);
-        if (over) flag_for_replay((uint32_t)pair, replay_list, replay_count, replay_cap);
+      if (lane_step(q, w, tol, light_budget, &over)) {
+        if (over) flag_for_replay((uint32_t)pair, heavy_list, heavy_count, list_cap);
+        else write_result(&out[pair], w.best, w.boxes, 0u);
         pair = -1;
       }
+    }
+  }
+}
+
+// ---- budgeted rounds ------------------------------------------------------
+// Round r walks each listed pair for at most `budget` steps, one lane per
+// pair, then either writes the pair's result or saves its walk state (48
+// bytes) and appends it to the next round's list.  Round 0 starts every pair
+// fresh; the last round hands the survivors to the wave-cooperative pass.
+// No atomics or refills in the hot loop, and consecutive lanes read
+// consecutive pair records.
+struct WalkState {
+  uint64_t key[3];
+  uint64_t best;
+  uint32_t boxes;
+  uint16_t level;
+  uint8_t e[3];
+  uint8_t visit;
+};
+static_assert(sizeof(WalkState) == 48, "walk state layout");
+
+__device__ inline void wave_append(uint32_t value, bool want, uint32_t* list, uint32_t* count, uint32_t cap) {
+  const uint64_t m = __ballot(want);
+  if (m == 0) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(m));
+  base = __shfl(base, leader);
+  if (want) {
+    const uint32_t slot = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    if (slot < cap) list[slot] = value;
+  }
+}
+
+__global__ __launch_bounds__(kTuvBlock) void tuv_round_kernel(const TuvPair* __restrict__ pairs, int64_t n_host,
+                                                              const uint64_t* __restrict__ totals,
+                                                              const uint32_t* __restrict__ list_in,
+                                                              const uint32_t* __restrict__ count_in,
+                                                              WalkState* __restrict__ states, double tol,
+                                                              uint32_t budget, uint64_t max_itr,
+                                                              p2c_tuv_result* __restrict__ out,
+                                                              uint32_t* __restrict__ list_out,
+                                                              uint32_t* __restrict__ count_out, int last_round,
+                                                              uint32_t* __restrict__ replay_list,
+                                                              uint32_t* __restrict__ replay_count,
+                                                              uint32_t list_cap) {
+  int64_t m;
+  if (list_in == nullptr) {
+    m = device_count(n_host, totals);
+  } else {
+    m = *count_in;
+    if (m > list_cap) m = list_cap;
+  }
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  // the loop bound is the same for every lane of a wave so wave_append's
+  // ballots see the whole wave
+  for (int64_t base = int64_t(blockIdx.x) * blockDim.x; base < m; base += stride) {
+    const int64_t j = base + threadIdx.x;
+    const bool valid = j < m;
+    uint32_t p = 0;
+    bool keep = false, over = false;
+    if (valid) {
+      p = list_in == nullptr ? (uint32_t)j : list_in[j];
+      const TuvPair q = pairs[p];
+      LaneWalk w;
+      if (list_in == nullptr) {
+        w.key[0] = w.key[1] = w.key[2] = 0;
+        w.best = kTuvNoHit;
+        w.boxes = 0;
+        w.e[0] = w.e[1] = w.e[2] = 0;
+        w.level = 0;
+        w.visit = true;
+      } else {
+        const WalkState st = states[p];
+        w.key[0] = st.key[0];
+        w.key[1] = st.key[1];
+        w.key[2] = st.key[2];
+        w.best = st.best;
+        w.boxes = st.boxes;
+        w.e[0] = st.e[0];
+        w.e[1] = st.e[1];
+        w.e[2] = st.e[2];
+        w.level = st.level;
+        w.visit = st.visit != 0;
+      }
+      bool done = false;
+      for (uint32_t it = 0; it < budget && !done; ++it) done = lane_step(q, w, tol, max_itr, &over);
+      if (done && !over) {
+        write_result(&out[p], w.best, w.boxes, 0u);
+      } else if (over) {
+        write_result(&out[p], w.best, w.boxes, P2C_TUV_FLAG_OVER_BUDGET);
+      } else {
+        keep = true;
+        if (!last_round) {
+          WalkState st;
+          st.key[0] = w.key[0];
+          st.key[1] = w.key[1];
+          st.key[2] = w.key[2];
+          st.best = w.best;
+          st.boxes = (uint32_t)w.boxes;
+          st.level = (uint16_t)w.level;
+          st.e[0] = (uint8_t)w.e[0];
+          st.e[1] = (uint8_t)w.e[1];
+          st.e[2] = (uint8_t)w.e[2];
+          st.visit = w.visit ? 1 : 0;
+          states[p] = st;
+        }
+      }
+    }
+    wave_append(p, valid && over, replay_list, replay_count, list_cap);
+    wave_append(p, valid && keep, list_out, count_out, list_cap);
+  }
+}
+
+// ---- pass 2: one wave per heavy pair ------------------------------------
+// The 64 lanes of a wave share one pair's box tree.  Each lane walks a
+// subtree depth-first; when lanes sit idle, one busy lane donates every
+// pending upper sibling on its path into the wave's LDS task queue and
+// re-roots its walk at its current box; idle lanes take tasks (last in,
+// first out).  The least done t_lo is shared by a wave min-reduction every
+// step, so all lanes prune with it.  Boxes are counted for the whole pair; a
+// pair over max_itr goes to the canonical replay.
+constexpr int kWaveQueue = 448;
+
+struct WaveTask {
+  uint64_t kt, ku, kv;
+  uint32_t level, pad;
+};
+
+__device__ inline uint64_t wave_min_u64(uint64_t x) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    const uint64_t y = __shfl_xor(x, o);
+    x = y < x ? y : x;
+  }
+  return x;
+}
+
+__global__ __launch_bounds__(kTuvBlock) void tuv_heavy_kernel(const TuvPair* __restrict__ pairs,
+                                                              const uint32_t* __restrict__ heavy_list,
+                                                              const uint32_t* __restrict__ heavy_count,
+                                                              uint32_t list_cap, uint32_t* __restrict__ heavy_next,
+                                                              double tol, uint64_t max_itr,
+                                                              p2c_tuv_result* __restrict__ out,
+                                                              uint32_t* __restrict__ replay_list,
+                                                              uint32_t* __restrict__ replay_count) {
+  __shared__ WaveTask queue[kTuvBlock / 64][kWaveQueue];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  WaveTask* wq = queue[wv];
+  const uint64_t lt = (1ull << lane) - 1ull;
+  uint32_t total = *heavy_count;
+  if (total > list_cap) total = list_cap;
+  for (;;) {
+    uint32_t h = 0;
+    if (lane == 0) h = atomicAdd(heavy_next, 1u);
+    h = __shfl(h, 0);
+    if (h >= total) break;
+    const uint32_t p = heavy_list[h];
+    const TuvPair q = pairs[p];
+    LaneWalk w;
+    uint32_t root = 0;
+    bool has = lane == 0;
+    w.key[0] = w.key[1] = w.key[2] = 0;
+    w.e[0] = w.e[1] = w.e[2] = 0;
+    w.level = 0;
+    w.visit = true;
+    w.best = kTuvNoHit;
+    w.boxes = 0;
+    uint64_t boxes = 0;
+    int qsize = 0;
+    bool over = false;
+    for (;;) {
+      // 1. one step of every busy lane's subtree walk
+      bool evaluated = false;
+      if (has) {
+        const uint64_t before = w.boxes;
+        // climbing stops at this lane's subtree root
+        if (w.visit) {
+          int outcome = 0;
+          const bool discard = (!q.is_ee && w.key[1] + w.key[2] > kOne63);
+          if (!discard && w.key[0] < w.best) {
+            ++w.boxes;
+            outcome = tuv_classify(q, w.key[0], w.key[1], w.key[2], w.e, w.level, tol);
+            if (outcome == 1 && w.key[0] < w.best) w.best = w.key[0];
+          }
+          if (outcome == 2) {
+            const int d = tuv_level_dim(q, w.level);
+            ++w.e[d];
+            ++w.level;
+          } else {
+            w.visit = false;
+          }
+        }
+        if (!w.visit) {
+          while (w.level > root) {
+            const int d = tuv_level_dim(q, w.level - 1);
+            const uint64_t bit = kOne63 >> w.e[d];
+            if (w.key[d] & bit) {
+              w.key[d] &= ~bit;
+              --w.e[d];
+              --w.level;
+            } else {
+              w.key[d] |= bit;
+              w.visit = true;
+              break;
+            }
+          }
+          if (!w.visit) has = false;
+        }
+        evaluated = w.boxes != before;
+      }
+      boxes += (uint64_t)__popcll(__ballot(evaluated));
+      w.best = wave_min_u64(w.best);
+      if (boxes > max_itr) {
+        over = true;
+        break;
+      }
+      // 2. donation when lanes are idle and the queue cannot feed them
+      const uint64_t busy = __ballot(has);
+      const int idle = 64 - __popcll(busy);
+      if (idle > qsize && qsize + kTuvMaxLevel < kWaveQueue) {
+        const uint64_t cand = __ballot(has && w.visit && w.level > root);
+        if (cand) {
+          const int donor = __ffsll((unsigned long long)cand) - 1;
+          int pushed = 0;
+          if (lane == donor) {
+            uint64_t k[3] = {w.key[0], w.key[1], w.key[2]};
+            int e[3] = {w.e[0], w.e[1], w.e[2]};
+            for (uint32_t l = w.level; l > root; --l) {
+              const int d = tuv_level_dim(q, l - 1);
+              const uint64_t bit = kOne63 >> e[d];
+              if (!(k[d] & bit)) {
+                WaveTask t;
+                t.kt = k[0];
+                t.ku = k[1];
+                t.kv = k[2];
+                if (d == 0) t.kt |= bit;
+                else if (d == 1) t.ku |= bit;
+                else t.kv |= bit;
+                t.level = l;
+                t.pad = 0;
+                wq[qsize + pushed] = t;
+                ++pushed;
+              }
+              k[d] &= ~bit;
+              --e[d];
+            }
+            root = w.level;
+          }
+          qsize += __shfl(pushed, donor);
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      // 3. idle lanes take queued subtrees, newest first
+      const uint64_t want = __ballot(!has);
+      const int n_take = min(__popcll(want), qsize);
+      if (!has) {
+        const int rank = __popcll(want & lt);
+        if (rank < n_take) {
+          const WaveTask t = wq[qsize - 1 - rank];
+          w.key[0] = t.kt;
+          w.key[1] = t.ku;
+          w.key[2] = t.kv;
+          w.level = t.level;
+          root = t.level;
+          tuv_exponents(q, t.level, w.e);
+          w.visit = true;
+          has = true;
+        }
+      }
+      qsize -= n_take;
+      __builtin_amdgcn_wave_barrier();
+      if (__ballot(has) == 0 && qsize == 0) break;
+    }
+    if (lane == 0) {
+      write_result(&out[p], w.best, boxes, P2C_TUV_FLAG_OVER_BUDGET * (over ? 1u : 0u));
+      if (over) flag_for_replay(p, replay_list, replay_count, list_cap);
     }
   }
 }
@@ -349,26 +686,84 @@ int p2c_tuv_setup_indexed(int is_ee, const double* x0, const double* x1, const i
   return P2C_OK;
 }
 
-int p2c_tuv_plain(const void* pairs, int64_t n, double tolerance, int64_t max_itr, p2c_tuv_result* out,
-                  uint32_t* replay_list, uint32_t* replay_count, uint32_t replay_cap, void* stream) {
+int p2c_tuv_setup_pairs(int is_ee, const double* x0, const double* x1, const int32_t* topo, const int32_t* pairs,
+                        int64_t n, const uint64_t* totals, int64_t out_offset, int64_t out_cap, double ms,
+                        double tolerance, void* tuv_pairs, void* stream) {
+  if (int rc = p2c::check_tuv(ms, tolerance, 1.0, 1)) return rc;
+  if (n <= 0) return P2C_OK;
+  hipLaunchKernelGGL(p2c::tuv_setup_pairs_kernel, dim3(p2c::grid_for(n, p2c::kTuvBlock)), dim3(p2c::kTuvBlock), 0,
+                     p2c::as_stream(stream), is_ee, x0, x1, topo, pairs, n, totals, out_offset, out_cap, ms,
+                     tolerance, static_cast<p2c::TuvPair*>(tuv_pairs));
+  P2C_HIP_CHECK(hipGetLastError());
+  return P2C_OK;
+}
+
+int p2c_tuv_plain(const void* pairs, int64_t n, const uint64_t* totals, double tolerance, int64_t max_itr,
+                  p2c_tuv_result* out, uint32_t* replay_list, uint32_t* replay_count, uint32_t replay_cap,
+                  void* stream) {
   if (int rc = p2c::check_tuv(0.0, tolerance, 1.0, max_itr)) return rc;
   if (n <= 0) return P2C_OK;
   hipLaunchKernelGGL(p2c::tuv_plain_kernel, dim3(p2c::grid_for(n, p2c::kTuvBlock)), dim3(p2c::kTuvBlock), 0,
-                     p2c::as_stream(stream), static_cast<const p2c::TuvPair*>(pairs), n, tolerance,
+                     p2c::as_stream(stream), static_cast<const p2c::TuvPair*>(pairs), n, totals, tolerance,
                      (uint64_t)max_itr, out, replay_list, replay_count, replay_cap);
   P2C_HIP_CHECK(hipGetLastError());
   return P2C_OK;
 }
 
-int p2c_tuv_persistent(const void* pairs, int64_t n, double tolerance, int64_t max_itr, p2c_tuv_result* out,
-                       unsigned long long* counter, uint32_t* replay_list, uint32_t* replay_count,
-                       uint32_t replay_cap, int grid_blocks, void* stream) {
+size_t p2c_tuv_state_bytes(void) { return sizeof(p2c::WalkState); }
+
+int p2c_tuv_persistent(const void* pairs, int64_t n, const uint64_t* totals, double tolerance, int64_t max_itr,
+                       int64_t light_budget, p2c_tuv_result* out, void* counters, uint32_t* heavy_list,
+                       uint32_t* replay_list, uint32_t list_cap, int grid_blocks, void* stream) {
   if (int rc = p2c::check_tuv(0.0, tolerance, 1.0, max_itr)) return rc;
   if (n <= 0) return P2C_OK;
   if (grid_blocks <= 0) return p2c::set_error(P2C_EINVAL, "grid_blocks must be positive");
-  hipLaunchKernelGGL(p2c::tuv_persistent_kernel, dim3(grid_blocks), dim3(p2c::kTuvBlock), 0,
-                     p2c::as_stream(stream), static_cast<const p2c::TuvPair*>(pairs), n, tolerance,
-                     (uint64_t)max_itr, out, counter, replay_list, replay_count, replay_cap);
+  if (light_budget <= 0) return p2c::set_error(P2C_EINVAL, "light_budget must be positive");
+  if (light_budget > max_itr) light_budget = max_itr;
+  // counters: [0] next pair (u64), [1] replay count (u32) | heavy count (u32), [2] heavy cursor (u32)
+  auto* c64 = static_cast<unsigned long long*>(counters);
+  auto* c32 = static_cast<uint32_t*>(counters);
+  hipStream_t s = p2c::as_stream(stream);
+  hipLaunchKernelGGL(p2c::tuv_persistent_kernel, dim3(grid_blocks), dim3(p2c::kTuvBlock), 0, s,
+                     static_cast<const p2c::TuvPair*>(pairs), n, totals, tolerance, (uint64_t)light_budget, out,
+                     c64, heavy_list, c32 + 3, list_cap);
+  hipLaunchKernelGGL(p2c::tuv_heavy_kernel, dim3(grid_blocks), dim3(p2c::kTuvBlock), 0, s,
+                     static_cast<const p2c::TuvPair*>(pairs), heavy_list, c32 + 3, list_cap, c32 + 4, tolerance,
+                     (uint64_t)max_itr, out, replay_list, c32 + 2);
+  P2C_HIP_CHECK(hipGetLastError());
+  return P2C_OK;
+}
+
+int p2c_tuv_rounds(const void* pairs, int64_t n, const uint64_t* totals, double tolerance, int64_t max_itr,
+                   const uint32_t* budgets, int n_rounds, p2c_tuv_result* out, void* counters, void* states,
+                   uint32_t* list_a, uint32_t* list_b, uint32_t* replay_list, uint32_t list_cap, int grid_blocks,
+                   void* stream) {
+  if (int rc = p2c::check_tuv(0.0, tolerance, 1.0, max_itr)) return rc;
+  if (n <= 0) return P2C_OK;
+  if (n_rounds < 1 || n_rounds > 8) return p2c::set_error(P2C_EINVAL, "1 to 8 rounds");
+  if (grid_blocks <= 0) return p2c::set_error(P2C_EINVAL, "grid_blocks must be positive");
+  for (int r = 0; r < n_rounds; ++r)
+    if (budgets[r] == 0) return p2c::set_error(P2C_EINVAL, "round budgets must be positive");
+  // counters (u32): [2] replay count, [3] heavy count (list of the last round),
+  // [4] heavy cursor, [5..12] per-round survivor counts
+  auto* c32 = static_cast<uint32_t*>(counters);
+  hipStream_t s = p2c::as_stream(stream);
+  auto* st = static_cast<p2c::WalkState*>(states);
+  const auto* tp = static_cast<const p2c::TuvPair*>(pairs);
+  uint32_t* lists[2] = {list_a, list_b};
+  for (int r = 0; r < n_rounds; ++r) {
+    const bool last = r == n_rounds - 1;
+    const uint32_t* lin = r == 0 ? nullptr : lists[(r - 1) & 1];
+    const uint32_t* cin = r == 0 ? nullptr : c32 + 5 + (r - 1);
+    uint32_t* lout = last ? lists[r & 1] : lists[r & 1];
+    uint32_t* cout = last ? c32 + 3 : c32 + 5 + r;
+    hipLaunchKernelGGL(p2c::tuv_round_kernel, dim3(grid_blocks), dim3(p2c::kTuvBlock), 0, s, tp, n, totals, lin, cin,
+                       st, tolerance, budgets[r], (uint64_t)max_itr, out, lout, cout, last ? 1 : 0, replay_list,
+                       c32 + 2, list_cap);
+  }
+  const uint32_t* heavy = lists[(n_rounds - 1) & 1];
+  hipLaunchKernelGGL(p2c::tuv_heavy_kernel, dim3(grid_blocks), dim3(p2c::kTuvBlock), 0, s, tp, heavy, c32 + 3,
+                     list_cap, c32 + 4, tolerance, (uint64_t)max_itr, out, replay_list, c32 + 2);
   P2C_HIP_CHECK(hipGetLastError());
   return P2C_OK;
 }

@@ -125,15 +125,27 @@ P2C_TUV_HD void tuv_setup(TuvPair* q, const double v[8][3], int is_ee, double ms
     }
     q->err[k] = kTuvErrScale * m + kTuvErrFloor + ms;
   }
-  // unit-cube corner values and the largest change along each axis
+  // unit-cube corner values (the operations of tuv_corner_range, one corner
+  // at a time) and the largest change along each axis
   double F[2][2][2][3];
   for (int it = 0; it < 2; ++it)
-    for (int iu = 0; iu < 2; ++iu)
-      for (int iv = 0; iv < 2; ++iv) {
-        double mn[3], mx[3];
-        tuv_corner_range(*q, it, it, iu, iu, iv, iv, mn, mx);
-        for (int k = 0; k < 3; ++k) F[it][iu][iv][k] = mn[k];
-      }
+    for (int k = 0; k < 3; ++k) {
+      const double t = it;
+      const double X0 = q->x0[0][k] + t * q->d[0][k];
+      const double X1 = q->x0[1][k] + t * q->d[1][k];
+      const double X2 = q->x0[2][k] + t * q->d[2][k];
+      const double X3 = q->x0[3][k] + t * q->d[3][k];
+      for (int iu = 0; iu < 2; ++iu)
+        for (int iv = 0; iv < 2; ++iv) {
+          const double u = iu, v = iv;
+          if (!is_ee) {
+            const double e1 = X2 - X1, e2 = X3 - X1;
+            F[it][iu][iv][k] = X0 - ((X1 + u * e1) + v * e2);
+          } else {
+            F[it][iu][iv][k] = (X0 + u * (X1 - X0)) - (X2 + v * (X3 - X2));
+          }
+        }
+    }
   double L[3] = {0.0, 0.0, 0.0};
   for (int a = 0; a < 2; ++a)
     for (int b = 0; b < 2; ++b)

@@ -1,10 +1,13 @@
 """(t,u,v) inclusion certificate on the GPU: host-side driver over the C-ABI.
 
 `TuvCertifier` owns the device workspaces (per-pair records, results, the
-fallback-replay list and heaps) and runs one of two execution forms:
+fallback-replay list and heaps) and runs one of three execution forms:
   plain       one lane per pair, one launch (the per-stage baseline)
-  persistent  chip-sized grid, lanes refill as pairs finish
-Both are followed by the fallback replay of pairs that exceeded max_itr, so
+  rounds      budgeted one-lane-per-pair rounds that save unfinished walks and
+              compact them, then a wave-cooperative pass for the heaviest pairs
+  persistent  chip-sized grid whose lanes refill as pairs finish, then the
+              wave-cooperative pass
+All are followed by the fallback replay of pairs that exceeded max_itr, so
 every pair ends with a final collide/separate decision.
 
 Parameters mirror the reference's Tight-Inclusion call sites
@@ -29,8 +32,11 @@ PAIR_BYTES = 288
 _vp, _i64, _i32, _u32, _f64 = nat._vp, nat._i64, nat._i32, nat._u32, nat._f64
 nat._sig("p2c_tuv_setup_positions", [_i32, _vp, _i64, _f64, _f64, _vp, _vp])
 nat._sig("p2c_tuv_setup_indexed", [_i32, _vp, _vp, _vp, _i64, _f64, _f64, _vp, _vp])
-nat._sig("p2c_tuv_plain", [_vp, _i64, _f64, _i64, _vp, _vp, _vp, _u32, _vp])
-nat._sig("p2c_tuv_persistent", [_vp, _i64, _f64, _i64, _vp, _vp, _vp, _vp, _u32, _i32, _vp])
+nat._sig("p2c_tuv_setup_pairs", [_i32, _vp, _vp, _vp, _vp, _i64, _vp, _i64, _i64, _f64, _f64, _vp, _vp])
+nat._sig("p2c_tuv_plain", [_vp, _i64, _vp, _f64, _i64, _vp, _vp, _vp, _u32, _vp])
+nat._sig("p2c_tuv_persistent", [_vp, _i64, _vp, _f64, _i64, _i64, _vp, _vp, _vp, _vp, _u32, _i32, _vp])
+nat._sig("p2c_tuv_rounds", [_vp, _i64, _vp, _f64, _i64, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _u32, _i32, _vp])
+nat.lib.p2c_tuv_state_bytes.restype = nat.ctypes.c_size_t
 nat._sig("p2c_tuv_replay", [_vp, _vp, _vp, _u32, _f64, _i64, _vp, _i64, _i64, _vp, _vp])
 nat.lib.p2c_tuv_pair_bytes.restype = nat.ctypes.c_size_t
 assert nat.lib.p2c_tuv_pair_bytes() == PAIR_BYTES
@@ -44,6 +50,8 @@ class TuvConfig:
     heap_cap: int = 1 << 20      # canonical-replay heap entries per slot
     replay_slots: int = 8        # pairs replayed concurrently
     blocks_per_cu: int = 3       # persistent grid: resident blocks of 256 per CU
+    light_budget: int = 2048     # boxes a pair may take in the lane-refill pass before the wave pass
+    round_budgets: tuple = (8, 32, 128, 512)  # walk steps per pair per round before the wave pass
 
 
 def results_view(t: torch.Tensor) -> np.ndarray:
@@ -62,7 +70,13 @@ class TuvCertifier:
         self.grid_blocks = int(props.multi_processor_count) * self.cfg.blocks_per_cu
         self.heaps = torch.empty(self.cfg.replay_slots * self.cfg.heap_cap * 32, dtype=torch.uint8,
                                  device=self.device)
-        self.counters = torch.zeros(4, dtype=torch.int64, device=self.device)  # [next_pair, replay_count, ...]
+        # u32 view: [0:2] next pair, [2] replay count, [3] heavy count, [4] heavy cursor, [5:13] round survivors
+        self.counters = torch.zeros(8, dtype=torch.int64, device=self.device)
+        self.budgets = np.ascontiguousarray(self.cfg.round_budgets, dtype=np.uint32)
+        self.state_bytes = int(nat.lib.p2c_tuv_state_bytes())
+
+    def reserve(self, n: int) -> None:
+        self._reserve(n)
 
     def _reserve(self, n: int) -> None:
         if n <= self._cap:
@@ -71,6 +85,9 @@ class TuvCertifier:
         self.pairs = torch.empty(cap * PAIR_BYTES, dtype=torch.uint8, device=self.device)
         self.results = torch.empty(cap * TUV_RESULT_DTYPE.itemsize, dtype=torch.uint8, device=self.device)
         self.replay_list = torch.empty(cap, dtype=torch.int32, device=self.device)
+        self.heavy_list = torch.empty(cap, dtype=torch.int32, device=self.device)
+        self.round_list = torch.empty(cap, dtype=torch.int32, device=self.device)
+        self.states = torch.empty(cap * self.state_bytes, dtype=torch.uint8, device=self.device)
         self._cap = cap
 
     # -- stages ------------------------------------------------------------
@@ -90,33 +107,56 @@ class TuvCertifier:
                  self.cfg.ms, self.cfg.tolerance, self.pairs.data_ptr(), nat.stream_handle(stream))
         return n
 
-    def certify(self, n: int, mode: str = "persistent", stream=None) -> None:
-        """Certifies the n pairs set up last; results land in self.results."""
+    def certify(self, n: int, mode: str = "rounds", stream=None, totals: torch.Tensor | None = None,
+                events: tuple | None = None) -> None:
+        """Certifies the pairs set up last (n, or min(n, totals[0] + totals[1])
+        when device totals are given); results land in self.results.
+        `events` (start, end) are recorded around the certificate kernel."""
         s = nat.stream_handle(stream)
         c = self.counters
         c.zero_()
+        if events is not None:
+            events[0].record()
         rc_ptr = c.data_ptr() + 8  # replay count (uint32 in the low half of slot 1)
+        tp = 0 if totals is None else totals.data_ptr()
         if mode == "plain":
-            nat.call("p2c_tuv_plain", self.pairs.data_ptr(), n, self.cfg.tolerance, self.cfg.max_itr,
+            nat.call("p2c_tuv_plain", self.pairs.data_ptr(), n, tp, self.cfg.tolerance, self.cfg.max_itr,
                      self.results.data_ptr(), self.replay_list.data_ptr(), rc_ptr, self._cap, s)
+        elif mode == "rounds":
+            nat.call("p2c_tuv_rounds", self.pairs.data_ptr(), n, tp, self.cfg.tolerance, self.cfg.max_itr,
+                     self.budgets.ctypes.data, len(self.budgets), self.results.data_ptr(), c.data_ptr(),
+                     self.states.data_ptr(), self.round_list.data_ptr(), self.heavy_list.data_ptr(),
+                     self.replay_list.data_ptr(), self._cap, self.grid_blocks, s)
         elif mode == "persistent":
-            nat.call("p2c_tuv_persistent", self.pairs.data_ptr(), n, self.cfg.tolerance, self.cfg.max_itr,
-                     self.results.data_ptr(), c.data_ptr(), self.replay_list.data_ptr(), rc_ptr, self._cap,
-                     self.grid_blocks, s)
+            nat.call("p2c_tuv_persistent", self.pairs.data_ptr(), n, tp, self.cfg.tolerance, self.cfg.max_itr,
+                     self.cfg.light_budget, self.results.data_ptr(), c.data_ptr(), self.heavy_list.data_ptr(),
+                     self.replay_list.data_ptr(), self._cap, self.grid_blocks, s)
         else:
             raise ValueError(f"unknown certificate mode {mode!r}")
+        if events is not None:
+            events[1].record()
         nat.call("p2c_tuv_replay", self.pairs.data_ptr(), self.replay_list.data_ptr(), rc_ptr, self._cap,
                  self.cfg.tolerance, self.cfg.max_itr, self.heaps.data_ptr(), self.cfg.heap_cap,
                  self.cfg.replay_slots, self.results.data_ptr(), s)
 
+    def setup_pairs(self, x0: torch.Tensor, x1: torch.Tensor, topo: torch.Tensor, pairs: torch.Tensor, is_ee: bool,
+                    n: int, totals: torch.Tensor | None = None, out_offset: int = 0, stream=None) -> None:
+        """Pair records straight from broad-phase pair lists (see p2c_tuv_setup_pairs)."""
+        nat.call("p2c_tuv_setup_pairs", int(is_ee), x0.data_ptr(), x1.data_ptr(), topo.data_ptr(), pairs.data_ptr(),
+                 n, 0 if totals is None else totals.data_ptr(), out_offset, self._cap, self.cfg.ms,
+                 self.cfg.tolerance, self.pairs.data_ptr(), nat.stream_handle(stream))
+
     def replay_count(self) -> int:
         return int(self.counters[1].item() & 0xFFFFFFFF)
+
+    def heavy_count(self) -> int:
+        return int((self.counters[1].item() >> 32) & 0xFFFFFFFF)
 
     def result_tensor(self, n: int) -> torch.Tensor:
         return self.results[: n * TUV_RESULT_DTYPE.itemsize]
 
     # -- convenience -------------------------------------------------------
-    def certify_positions(self, x, is_ee: bool, mode: str = "persistent") -> np.ndarray:
+    def certify_positions(self, x, is_ee: bool, mode: str = "rounds") -> np.ndarray:
         if isinstance(x, np.ndarray):
             x = torch.from_numpy(np.ascontiguousarray(x, dtype=np.float64)).to(self.device)
         n = self.setup_positions(x, is_ee)

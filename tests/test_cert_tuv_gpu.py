@@ -19,7 +19,7 @@ def _check(got, ref, label):
     assert np.array_equal(got["toi"], ref["toi"]), label
 
 
-@pytest.mark.parametrize("mode", ["plain", "persistent"])
+@pytest.mark.parametrize("mode", ["plain", "persistent", "rounds"])
 def test_plain_workload_matches_oracle(gpu, mode):
     qs = workloads.make_query_set(10_000, 10_000)
     cert = TuvCertifier()
@@ -31,7 +31,7 @@ def test_plain_workload_matches_oracle(gpu, mode):
         assert got["hit"][(qs.vf_kind if not is_ee else qs.ee_kind) == 1].all()  # touching pairs all hit
 
 
-@pytest.mark.parametrize("mode", ["plain", "persistent"])
+@pytest.mark.parametrize("mode", ["plain", "persistent", "rounds"])
 def test_forced_replay_reproduces_capped_oracle(gpu, mode):
     qs = workloads.make_query_set(2000, 2000, seed=31)
     cfg = TuvConfig(max_itr=40, heap_cap=1 << 12, replay_slots=16)
@@ -53,3 +53,29 @@ def test_tolerance_and_separation_variants(gpu):
             got = cert.certify_positions(x, is_ee, "persistent")
             ref = oracle.tuv_cert(is_ee, x, ms=kw.get("ms", 0.0), tolerance=kw.get("tolerance", 1e-6), threads=8)
             _check(got, ref, kw)
+
+
+@pytest.mark.parametrize("light_budget", [1, 8, 64])
+def test_wave_cooperative_pass_matches_oracle(gpu, light_budget):
+    # A tiny lane budget sends most pairs through the wave-shared pass, whose
+    # box order differs from every other path; decisions and TOI must not.
+    qs = workloads.make_query_set(3000, 3000, seed=52)
+    cert = TuvCertifier(TuvConfig(light_budget=light_budget))
+    for is_ee, x in ((False, qs.vf), (True, qs.ee)):
+        got = cert.certify_positions(x, is_ee, "persistent")
+        ref = oracle.tuv_cert(is_ee, x, threads=8)
+        _check(got, ref, (light_budget, is_ee))
+        assert cert.heavy_count() > 0 and cert.replay_count() == 0
+
+
+@pytest.mark.parametrize("budgets", [(1,), (1, 2), (2, 4, 8, 16)])
+def test_rounds_with_tiny_budgets_match_oracle(gpu, budgets):
+    # Walks are cut after a step or two and resumed from their saved state in
+    # the next round; survivors of the last round restart in the wave pass.
+    qs = workloads.make_query_set(3000, 3000, seed=61)
+    cert = TuvCertifier(TuvConfig(round_budgets=budgets))
+    for is_ee, x in ((False, qs.vf), (True, qs.ee)):
+        got = cert.certify_positions(x, is_ee, "rounds")
+        ref = oracle.tuv_cert(is_ee, x, threads=8)
+        _check(got, ref, (budgets, is_ee))
+        assert cert.heavy_count() > 0
