@@ -336,7 +336,7 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_round_kernel(const TuvPair* __r
         write_result(&out[p], w.best, w.boxes, P2C_TUV_FLAG_OVER_BUDGET);
       } else {
         keep = true;
-        if (!last_round) {
+        {
           WalkState st;
           st.key[0] = w.key[0];
           st.key[1] = w.key[1];
@@ -359,12 +359,14 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_round_kernel(const TuvPair* __r
 
 // ---- pass 2: one wave per heavy pair ------------------------------------
 // The 64 lanes of a wave share one pair's box tree.  Each lane walks a
-// subtree depth-first; when lanes sit idle, one busy lane donates every
-// pending upper sibling on its path into the wave's LDS task queue and
-// re-roots its walk at its current box; idle lanes take tasks (last in,
-// first out).  The least done t_lo is shared by a wave min-reduction every
-// step, so all lanes prune with it.  Boxes are counted for the whole pair; a
-// pair over max_itr goes to the canonical replay.
+// subtree depth-first.  While lanes sit idle, every busy lane standing on a
+// fresh box donates all pending upper siblings on its path (shallowest taken
+// first) into the wave's LDS task queue and re-roots its walk at that box;
+// idle lanes take tasks.  The least done t_lo is shared through an LDS slot
+// (atomicMin) that every lane prunes with.  Boxes are counted for the whole
+// pair, including those a resumed walk already spent; a pair over max_itr
+// goes to the canonical replay.  Pairs handed over by the rounds resume
+// their saved walk (lane 0 holds it and donates its path at once).
 constexpr int kWaveQueue = 448;
 
 struct WaveTask {
@@ -372,24 +374,31 @@ struct WaveTask {
   uint32_t level, pad;
 };
 
-__device__ inline uint64_t wave_min_u64(uint64_t x) {
+// Exclusive wave prefix sum of 8-bit values through bit-sliced ballots.
+__device__ inline uint32_t wave_prefix_u8(uint32_t v, uint32_t* total) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t lt = (1ull << lane) - 1ull;
+  uint32_t pre = 0, tot = 0;
 #pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) {
-    const uint64_t y = __shfl_xor(x, o);
-    x = y < x ? y : x;
+  for (int b = 0; b < 8; ++b) {
+    const uint64_t m = __ballot((v >> b) & 1u);
+    pre += (uint32_t)__popcll(m & lt) << b;
+    tot += (uint32_t)__popcll(m) << b;
   }
-  return x;
+  *total = tot;
+  return pre;
 }
 
 __global__ __launch_bounds__(kTuvBlock) void tuv_heavy_kernel(const TuvPair* __restrict__ pairs,
                                                               const uint32_t* __restrict__ heavy_list,
                                                               const uint32_t* __restrict__ heavy_count,
                                                               uint32_t list_cap, uint32_t* __restrict__ heavy_next,
-                                                              double tol, uint64_t max_itr,
-                                                              p2c_tuv_result* __restrict__ out,
+                                                              const WalkState* __restrict__ resume, double tol,
+                                                              uint64_t max_itr, p2c_tuv_result* __restrict__ out,
                                                               uint32_t* __restrict__ replay_list,
                                                               uint32_t* __restrict__ replay_count) {
   __shared__ WaveTask queue[kTuvBlock / 64][kWaveQueue];
+  __shared__ unsigned long long best_slot[kTuvBlock / 64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   WaveTask* wq = queue[wv];
   const uint64_t lt = (1ull << lane) - 1ull;
@@ -405,28 +414,48 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_heavy_kernel(const TuvPair* __r
     LaneWalk w;
     uint32_t root = 0;
     bool has = lane == 0;
+    uint64_t boxes = 0;
     w.key[0] = w.key[1] = w.key[2] = 0;
     w.e[0] = w.e[1] = w.e[2] = 0;
     w.level = 0;
     w.visit = true;
     w.best = kTuvNoHit;
     w.boxes = 0;
-    uint64_t boxes = 0;
+    if (resume != nullptr) {
+      const WalkState st = resume[p];
+      boxes = st.boxes;
+      w.best = st.best;
+      if (lane == 0) {
+        w.key[0] = st.key[0];
+        w.key[1] = st.key[1];
+        w.key[2] = st.key[2];
+        w.e[0] = st.e[0];
+        w.e[1] = st.e[1];
+        w.e[2] = st.e[2];
+        w.level = st.level;
+        w.visit = st.visit != 0;
+      }
+    }
+    if (lane == 0) best_slot[wv] = w.best;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     int qsize = 0;
     bool over = false;
     for (;;) {
       // 1. one step of every busy lane's subtree walk
       bool evaluated = false;
       if (has) {
-        const uint64_t before = w.boxes;
-        // climbing stops at this lane's subtree root
         if (w.visit) {
           int outcome = 0;
           const bool discard = (!q.is_ee && w.key[1] + w.key[2] > kOne63);
           if (!discard && w.key[0] < w.best) {
-            ++w.boxes;
+            evaluated = true;
             outcome = tuv_classify(q, w.key[0], w.key[1], w.key[2], w.e, w.level, tol);
-            if (outcome == 1 && w.key[0] < w.best) w.best = w.key[0];
+            if (outcome == 1 && w.key[0] < w.best) {
+              w.best = w.key[0];
+              atomicMin(&best_slot[wv], (unsigned long long)w.key[0]);
+            }
           }
           if (outcome == 2) {
             const int d = tuv_level_dim(q, w.level);
@@ -452,53 +481,70 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_heavy_kernel(const TuvPair* __r
           }
           if (!w.visit) has = false;
         }
-        evaluated = w.boxes != before;
       }
       boxes += (uint64_t)__popcll(__ballot(evaluated));
-      w.best = wave_min_u64(w.best);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      {
+        const uint64_t shared_best = best_slot[wv];
+        if (shared_best < w.best) w.best = shared_best;
+      }
       if (boxes > max_itr) {
         over = true;
         break;
       }
-      // 2. donation when lanes are idle and the queue cannot feed them
-      const uint64_t busy = __ballot(has);
-      const int idle = 64 - __popcll(busy);
-      if (idle > qsize && qsize + kTuvMaxLevel < kWaveQueue) {
-        const uint64_t cand = __ballot(has && w.visit && w.level > root);
-        if (cand) {
-          const int donor = __ffsll((unsigned long long)cand) - 1;
-          int pushed = 0;
-          if (lane == donor) {
-            uint64_t k[3] = {w.key[0], w.key[1], w.key[2]};
-            int e[3] = {w.e[0], w.e[1], w.e[2]};
-            for (uint32_t l = w.level; l > root; --l) {
-              const int d = tuv_level_dim(q, l - 1);
-              const uint64_t bit = kOne63 >> e[d];
-              if (!(k[d] & bit)) {
-                WaveTask t;
-                t.kt = k[0];
-                t.ku = k[1];
-                t.kv = k[2];
-                if (d == 0) t.kt |= bit;
-                else if (d == 1) t.ku |= bit;
-                else t.kv |= bit;
-                t.level = l;
-                t.pad = 0;
-                wq[qsize + pushed] = t;
-                ++pushed;
-              }
-              k[d] &= ~bit;
-              --e[d];
-            }
-            root = w.level;
+      // 2. donation: every busy lane on a fresh box gives away its path
+      const int idle = 64 - __popcll(__ballot(has));
+      if (idle > qsize) {
+        uint32_t pend = 0;
+        if (has && w.visit) {
+          uint64_t k[3] = {w.key[0], w.key[1], w.key[2]};
+          int e[3] = {w.e[0], w.e[1], w.e[2]};
+          for (uint32_t l = w.level; l > root; --l) {
+            const int d = tuv_level_dim(q, l - 1);
+            const uint64_t bit = kOne63 >> e[d];
+            pend += (k[d] & bit) ? 0u : 1u;
+            k[d] &= ~bit;
+            --e[d];
           }
-          qsize += __shfl(pushed, donor);
         }
+        uint32_t tot = 0;
+        const uint32_t off = wave_prefix_u8(pend, &tot);
+        const bool fits = qsize + (int)(off + pend) <= kWaveQueue;
+        if (pend > 0 && fits) {
+          uint64_t k[3] = {w.key[0], w.key[1], w.key[2]};
+          int e[3] = {w.e[0], w.e[1], w.e[2]};
+          int slot = qsize + (int)off;
+          for (uint32_t l = w.level; l > root; --l) {  // deepest first: shallowest ends on top
+            const int d = tuv_level_dim(q, l - 1);
+            const uint64_t bit = kOne63 >> e[d];
+            if (!(k[d] & bit)) {
+              WaveTask t;
+              t.kt = k[0];
+              t.ku = k[1];
+              t.kv = k[2];
+              if (d == 0) t.kt |= bit;
+              else if (d == 1) t.ku |= bit;
+              else t.kv |= bit;
+              t.level = l;
+              t.pad = 0;
+              wq[slot++] = t;
+            }
+            k[d] &= ~bit;
+            --e[d];
+          }
+          root = w.level;
+        }
+        // lanes whose block did not fit pushed nothing; count what was pushed
+        uint32_t pushed_tot = 0;
+        wave_prefix_u8((pend > 0 && fits) ? pend : 0u, &pushed_tot);
+        qsize += (int)pushed_tot;
       }
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-      // 3. idle lanes take queued subtrees, newest first
+      // 3. idle lanes take queued subtrees, the top of the stack first
       const uint64_t want = __ballot(!has);
       const int n_take = min(__popcll(want), qsize);
       if (!has) {
@@ -516,13 +562,17 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_heavy_kernel(const TuvPair* __r
         }
       }
       qsize -= n_take;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
       __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
       if (__ballot(has) == 0 && qsize == 0) break;
     }
+    const uint64_t final_best = best_slot[wv] < w.best ? best_slot[wv] : w.best;
     if (lane == 0) {
-      write_result(&out[p], w.best, boxes, P2C_TUV_FLAG_OVER_BUDGET * (over ? 1u : 0u));
+      write_result(&out[p], final_best, boxes, over ? P2C_TUV_FLAG_OVER_BUDGET : 0u);
       if (over) flag_for_replay(p, replay_list, replay_count, list_cap);
     }
+    __builtin_amdgcn_wave_barrier();
   }
 }
 
@@ -728,8 +778,8 @@ int p2c_tuv_persistent(const void* pairs, int64_t n, const uint64_t* totals, dou
                      static_cast<const p2c::TuvPair*>(pairs), n, totals, tolerance, (uint64_t)light_budget, out,
                      c64, heavy_list, c32 + 3, list_cap);
   hipLaunchKernelGGL(p2c::tuv_heavy_kernel, dim3(grid_blocks), dim3(p2c::kTuvBlock), 0, s,
-                     static_cast<const p2c::TuvPair*>(pairs), heavy_list, c32 + 3, list_cap, c32 + 4, tolerance,
-                     (uint64_t)max_itr, out, replay_list, c32 + 2);
+                     static_cast<const p2c::TuvPair*>(pairs), heavy_list, c32 + 3, list_cap, c32 + 4, nullptr,
+                     tolerance, (uint64_t)max_itr, out, replay_list, c32 + 2);
   P2C_HIP_CHECK(hipGetLastError());
   return P2C_OK;
 }
@@ -763,7 +813,7 @@ int p2c_tuv_rounds(const void* pairs, int64_t n, const uint64_t* totals, double 
   }
   const uint32_t* heavy = lists[(n_rounds - 1) & 1];
   hipLaunchKernelGGL(p2c::tuv_heavy_kernel, dim3(grid_blocks), dim3(p2c::kTuvBlock), 0, s, tp, heavy, c32 + 3,
-                     list_cap, c32 + 4, tolerance, (uint64_t)max_itr, out, replay_list, c32 + 2);
+                     list_cap, c32 + 4, st, tolerance, (uint64_t)max_itr, out, replay_list, c32 + 2);
   P2C_HIP_CHECK(hipGetLastError());
   return P2C_OK;
 }
