@@ -18,7 +18,8 @@
  *  F(t,u,v) = VF: p(t) - ((f0(t) + u e1(t)) + v e2(t)),  e_i = f_i - f0
  *             EE: (a0(t) + u ea(t)) - (b0(t) + v eb(t))
  *  with x(t) = x0 + t (x1 - x0).  F is multilinear, so over a box its range
- *  per coordinate lies between the min and max of its 8 corner values.
+ *  per coordinate lies between the min and max of its 8 corner values
+ *  (IEEE minNum/maxNum: fmin / fmax, the same on the host and the device).
  *  err_k = 5.6e-15 * max|x_k| (+1e-300) bounds the FP64 error of one corner
  *  evaluation (first-order analysis gives 46 M eps for VF, 42 M eps for EE).
  *  A box is zero_in if every coordinate's corner range meets [-(err+ms), err+ms].
@@ -90,8 +91,8 @@ static void eval_box(const Tuv* q, double tl, double th, double ul, double uh, d
             const double pb = X[2][k] + vs[iv] * (X[3][k] - X[2][k]);
             f = pa - pb;
           }
-          if (f < mn[k]) mn[k] = f;
-          if (f > mx[k]) mx[k] = f;
+          mn[k] = fmin(mn[k], f);
+          mx[k] = fmax(mx[k], f);
         }
   }
 }

@@ -147,11 +147,21 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_plain_kernel(const TuvPair* __r
 // One lane's resumable walk state (the loop body of tuv_walk, one box per step).
 struct LaneWalk {
   uint64_t key[3];
+  double lo[3];  // the keys as exact doubles
   uint64_t best, boxes;
   int e[3];
   uint32_t level;
   bool visit;
 };
+
+__device__ inline void set_keys(LaneWalk& w, uint64_t kt, uint64_t ku, uint64_t kv) {
+  w.key[0] = kt;
+  w.key[1] = ku;
+  w.key[2] = kv;
+  w.lo[0] = key_to_f64(kt);
+  w.lo[1] = key_to_f64(ku);
+  w.lo[2] = key_to_f64(kv);
+}
 
 // Advances one step; returns true when the pair's walk has finished.
 // Sets *over when the box budget ran out.
@@ -165,12 +175,12 @@ __device__ inline bool lane_step(const TuvPair& q, LaneWalk& w, double tol, uint
         return true;
       }
       ++w.boxes;
-      outcome = tuv_classify(q, w.key[0], w.key[1], w.key[2], w.e, w.level, tol);
+      outcome = tuv_classify_at(q, w.lo[0], w.lo[1], w.lo[2], w.e, w.level, tol);
       if (outcome == 1 && w.key[0] < w.best) w.best = w.key[0];
     }
     if (outcome == 2) {
       const int d = tuv_level_dim(q, w.level);
-      ++w.e[d];
+      put3(w.e, d, get3(w.e, d) + 1);
       ++w.level;
       return false;
     }
@@ -178,13 +188,18 @@ __device__ inline bool lane_step(const TuvPair& q, LaneWalk& w, double tol, uint
   w.visit = false;
   while (w.level > 0) {
     const int d = tuv_level_dim(q, w.level - 1);
-    const uint64_t bit = kOne63 >> w.e[d];
-    if (w.key[d] & bit) {
-      w.key[d] &= ~bit;
-      --w.e[d];
+    const int ed = get3(w.e, d);
+    const uint64_t bit = kOne63 >> ed;
+    const double wd = __builtin_ldexp(1.0, -ed);
+    const uint64_t kd = get3(w.key, d);
+    if (kd & bit) {
+      put3(w.key, d, kd & ~bit);
+      put3(w.lo, d, get3(w.lo, d) - wd);
+      put3(w.e, d, ed - 1);
       --w.level;
     } else {
-      w.key[d] |= bit;
+      put3(w.key, d, kd | bit);
+      put3(w.lo, d, get3(w.lo, d) + wd);
       w.visit = true;
       return false;
     }
@@ -223,7 +238,7 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_persistent_kernel(const TuvPair
         if (p < n) {
           pair = p;
           q = pairs[p];
-          w.key[0] = w.key[1] = w.key[2] = 0;
+          set_keys(w, 0, 0, 0);
           w.best = kTuvNoHit;
           w.boxes = 0;
           w.e[0] = w.e[1] = w.e[2] = 0;
@@ -309,7 +324,7 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_round_kernel(const TuvPair* __r
       const TuvPair q = pairs[p];
       LaneWalk w;
       if (list_in == nullptr) {
-        w.key[0] = w.key[1] = w.key[2] = 0;
+        set_keys(w, 0, 0, 0);
         w.best = kTuvNoHit;
         w.boxes = 0;
         w.e[0] = w.e[1] = w.e[2] = 0;
@@ -317,9 +332,7 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_round_kernel(const TuvPair* __r
         w.visit = true;
       } else {
         const WalkState st = states[p];
-        w.key[0] = st.key[0];
-        w.key[1] = st.key[1];
-        w.key[2] = st.key[2];
+        set_keys(w, st.key[0], st.key[1], st.key[2]);
         w.best = st.best;
         w.boxes = st.boxes;
         w.e[0] = st.e[0];
@@ -415,7 +428,7 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_heavy_kernel(const TuvPair* __r
     uint32_t root = 0;
     bool has = lane == 0;
     uint64_t boxes = 0;
-    w.key[0] = w.key[1] = w.key[2] = 0;
+    set_keys(w, 0, 0, 0);
     w.e[0] = w.e[1] = w.e[2] = 0;
     w.level = 0;
     w.visit = true;
@@ -426,9 +439,7 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_heavy_kernel(const TuvPair* __r
       boxes = st.boxes;
       w.best = st.best;
       if (lane == 0) {
-        w.key[0] = st.key[0];
-        w.key[1] = st.key[1];
-        w.key[2] = st.key[2];
+        set_keys(w, st.key[0], st.key[1], st.key[2]);
         w.e[0] = st.e[0];
         w.e[1] = st.e[1];
         w.e[2] = st.e[2];
@@ -451,7 +462,7 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_heavy_kernel(const TuvPair* __r
           const bool discard = (!q.is_ee && w.key[1] + w.key[2] > kOne63);
           if (!discard && w.key[0] < w.best) {
             evaluated = true;
-            outcome = tuv_classify(q, w.key[0], w.key[1], w.key[2], w.e, w.level, tol);
+            outcome = tuv_classify_at(q, w.lo[0], w.lo[1], w.lo[2], w.e, w.level, tol);
             if (outcome == 1 && w.key[0] < w.best) {
               w.best = w.key[0];
               atomicMin(&best_slot[wv], (unsigned long long)w.key[0]);
@@ -459,7 +470,7 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_heavy_kernel(const TuvPair* __r
           }
           if (outcome == 2) {
             const int d = tuv_level_dim(q, w.level);
-            ++w.e[d];
+            put3(w.e, d, get3(w.e, d) + 1);
             ++w.level;
           } else {
             w.visit = false;
@@ -468,13 +479,18 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_heavy_kernel(const TuvPair* __r
         if (!w.visit) {
           while (w.level > root) {
             const int d = tuv_level_dim(q, w.level - 1);
-            const uint64_t bit = kOne63 >> w.e[d];
-            if (w.key[d] & bit) {
-              w.key[d] &= ~bit;
-              --w.e[d];
+            const int ed = get3(w.e, d);
+            const uint64_t bit = kOne63 >> ed;
+            const double wd = __builtin_ldexp(1.0, -ed);
+            const uint64_t kd = get3(w.key, d);
+            if (kd & bit) {
+              put3(w.key, d, kd & ~bit);
+              put3(w.lo, d, get3(w.lo, d) - wd);
+              put3(w.e, d, ed - 1);
               --w.level;
             } else {
-              w.key[d] |= bit;
+              put3(w.key, d, kd | bit);
+              put3(w.lo, d, get3(w.lo, d) + wd);
               w.visit = true;
               break;
             }
@@ -503,10 +519,12 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_heavy_kernel(const TuvPair* __r
           int e[3] = {w.e[0], w.e[1], w.e[2]};
           for (uint32_t l = w.level; l > root; --l) {
             const int d = tuv_level_dim(q, l - 1);
-            const uint64_t bit = kOne63 >> e[d];
-            pend += (k[d] & bit) ? 0u : 1u;
-            k[d] &= ~bit;
-            --e[d];
+            const int ed = get3(e, d);
+            const uint64_t bit = kOne63 >> ed;
+            const uint64_t kd = get3(k, d);
+            pend += (kd & bit) ? 0u : 1u;
+            put3(k, d, kd & ~bit);
+            put3(e, d, ed - 1);
           }
         }
         uint32_t tot = 0;
@@ -518,21 +536,20 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_heavy_kernel(const TuvPair* __r
           int slot = qsize + (int)off;
           for (uint32_t l = w.level; l > root; --l) {  // deepest first: shallowest ends on top
             const int d = tuv_level_dim(q, l - 1);
-            const uint64_t bit = kOne63 >> e[d];
-            if (!(k[d] & bit)) {
+            const int ed = get3(e, d);
+            const uint64_t bit = kOne63 >> ed;
+            const uint64_t kd = get3(k, d);
+            if (!(kd & bit)) {
               WaveTask t;
-              t.kt = k[0];
-              t.ku = k[1];
-              t.kv = k[2];
-              if (d == 0) t.kt |= bit;
-              else if (d == 1) t.ku |= bit;
-              else t.kv |= bit;
+              t.kt = k[0] | (d == 0 ? bit : 0ull);
+              t.ku = k[1] | (d == 1 ? bit : 0ull);
+              t.kv = k[2] | (d == 2 ? bit : 0ull);
               t.level = l;
               t.pad = 0;
               wq[slot++] = t;
             }
-            k[d] &= ~bit;
-            --e[d];
+            put3(k, d, kd & ~bit);
+            put3(e, d, ed - 1);
           }
           root = w.level;
         }
@@ -551,9 +568,7 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_heavy_kernel(const TuvPair* __r
         const int rank = __popcll(want & lt);
         if (rank < n_take) {
           const WaveTask t = wq[qsize - 1 - rank];
-          w.key[0] = t.kt;
-          w.key[1] = t.ku;
-          w.key[2] = t.kv;
+          set_keys(w, t.kt, t.ku, t.kv);
           w.level = t.level;
           root = t.level;
           tuv_exponents(q, t.level, w.e);
@@ -654,7 +669,7 @@ __device__ void tuv_canonical(const TuvPair& q, double tol, int64_t max_itr, Hea
       continue;
     }
     const int d = tuv_level_dim(q, b.level);
-    const uint64_t half = kOne63 >> (e[d] + 1);
+    const uint64_t half = kOne63 >> (get3(e, d) + 1);
     HeapBox lo = b, hi = b;
     lo.level = hi.level = b.level + 1;
     if (d == 0) hi.kt += half;

@@ -75,8 +75,8 @@ P2C_TUV_HD void tuv_corner_range(const TuvPair& q, double tl, double th, double 
       }
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        lo = f[c] < lo ? f[c] : lo;
-        hi = f[c] > hi ? f[c] : hi;
+        lo = __builtin_fmin(lo, f[c]);
+        hi = __builtin_fmax(hi, f[c]);
       }
     }
     mn[k] = lo;
@@ -97,6 +97,20 @@ P2C_TUV_HD int tuv_split_dim(const TuvPair& q, const int e[3]) {
     }
   }
   return best;
+}
+
+// Element d of a 3-vector through constant indices only: a dynamic index
+// would move the array out of registers (to scratch, or to LDS when the
+// compiler promotes it).
+template <typename T>
+P2C_TUV_HD T get3(const T v[3], int d) {
+  return d == 0 ? v[0] : (d == 1 ? v[1] : v[2]);
+}
+template <typename T>
+P2C_TUV_HD void put3(T v[3], int d, T x) {
+  v[0] = d == 0 ? x : v[0];
+  v[1] = d == 1 ? x : v[1];
+  v[2] = d == 2 ? x : v[2];
 }
 
 // Split dimension of `level`.  The word is chosen with constant indices so
@@ -158,27 +172,39 @@ P2C_TUV_HD void tuv_setup(TuvPair* q, const double v[8][3], int is_ee, double ms
         L[2] = dv > L[2] ? dv : L[2];
       }
   for (int k = 0; k < 3; ++k) q->inv_tol[k] = (3.0 * L[k]) / tolerance;
-  for (int w = 0; w < 5; ++w) q->split[w] = ~0ull;  // 3 = no split
+  // split sequence, 2 bits per level (3 = no split), built in registers
+  uint64_t w0 = ~0ull, w1 = ~0ull, w2 = ~0ull, w3 = ~0ull, w4 = ~0ull;
   int e[3] = {0, 0, 0};
   uint32_t level = 0;
   for (;;) {
     const int d = tuv_split_dim(*q, e);
     if (d == 3) break;
-    q->split[level >> 5] &= ~(3ull << ((level & 31u) * 2u));
-    q->split[level >> 5] |= (uint64_t)d << ((level & 31u) * 2u);
-    ++e[d];
+    const uint32_t wi = level >> 5, sh = (level & 31u) * 2u;
+    const uint64_t clear = ~(3ull << sh), set = (uint64_t)d << sh;
+    w0 = wi == 0 ? ((w0 & clear) | set) : w0;
+    w1 = wi == 1 ? ((w1 & clear) | set) : w1;
+    w2 = wi == 2 ? ((w2 & clear) | set) : w2;
+    w3 = wi == 3 ? ((w3 & clear) | set) : w3;
+    w4 = wi == 4 ? ((w4 & clear) | set) : w4;
+    put3(e, d, get3(e, d) + 1);
     ++level;
   }
+  q->split[0] = w0;
+  q->split[1] = w1;
+  q->split[2] = w2;
+  q->split[3] = w3;
+  q->split[4] = w4;
   q->done_level = level;
 }
 
 // Box outcome: 0 = no zero inside, 1 = done (certifies a hit at its t_lo),
-// 2 = undecided (split).
-P2C_TUV_HD int tuv_classify(const TuvPair& q, uint64_t kt, uint64_t ku, uint64_t kv, const int e[3],
-                            uint32_t level, double tolerance) {
-  const double tl = key_to_f64(kt), th = key_to_f64(kt + (kOne63 >> e[0]));
-  const double ul = key_to_f64(ku), uh = key_to_f64(ku + (kOne63 >> e[1]));
-  const double vl = key_to_f64(kv), vh = key_to_f64(kv + (kOne63 >> e[2]));
+// 2 = undecided (split).  (tl, ul, vl) are the box's lower corner as exact
+// doubles; its widths are 2^-e.
+P2C_TUV_HD int tuv_classify_at(const TuvPair& q, double tl, double ul, double vl, const int e[3], uint32_t level,
+                               double tolerance) {
+  const double th = tl + __builtin_ldexp(1.0, -e[0]);
+  const double uh = ul + __builtin_ldexp(1.0, -e[1]);
+  const double vh = vl + __builtin_ldexp(1.0, -e[2]);
   double mn[3], mx[3];
   tuv_corner_range(q, tl, th, ul, uh, vl, vh, mn, mx);
   bool zero_in = true, box_in = true, small = true;
@@ -194,10 +220,24 @@ P2C_TUV_HD int tuv_classify(const TuvPair& q, uint64_t kt, uint64_t ku, uint64_t
   return 2;
 }
 
+// The same from dyadic keys (lower corners scaled by 2^63).
+P2C_TUV_HD int tuv_classify(const TuvPair& q, uint64_t kt, uint64_t ku, uint64_t kv, const int e[3],
+                            uint32_t level, double tolerance) {
+  return tuv_classify_at(q, key_to_f64(kt), key_to_f64(ku), key_to_f64(kv), e, level, tolerance);
+}
+
 // Exponents of the boxes at `level` (replays the split sequence).
 P2C_TUV_HD void tuv_exponents(const TuvPair& q, uint32_t level, int e[3]) {
-  e[0] = e[1] = e[2] = 0;
-  for (uint32_t l = 0; l < level; ++l) ++e[tuv_level_dim(q, l)];
+  int e0 = 0, e1 = 0, e2 = 0;
+  for (uint32_t l = 0; l < level; ++l) {
+    const int d = tuv_level_dim(q, l);
+    e0 += d == 0;
+    e1 += d == 1;
+    e2 += d == 2;
+  }
+  e[0] = e0;
+  e[1] = e1;
+  e[2] = e2;
 }
 
 // Depth-first walk of the subtree rooted at box (kt,ku,kv) of level `root`,
@@ -209,6 +249,7 @@ P2C_TUV_HD bool tuv_walk(const TuvPair& q, uint64_t kt, uint64_t ku, uint64_t kv
   int e[3];
   tuv_exponents(q, root, e);
   uint64_t key[3] = {kt, ku, kv};
+  double lo[3] = {key_to_f64(kt), key_to_f64(ku), key_to_f64(kv)};  // exact: <= 52 significant bits
   uint32_t level = root;
   bool visit = true;  // evaluate the current box
   for (;;) {
@@ -218,13 +259,13 @@ P2C_TUV_HD bool tuv_walk(const TuvPair& q, uint64_t kt, uint64_t ku, uint64_t kv
       if (!discard && key[0] < *best) {
         if (*boxes >= budget) return false;
         ++*boxes;
-        outcome = tuv_classify(q, key[0], key[1], key[2], e, level, tolerance);
+        outcome = tuv_classify_at(q, lo[0], lo[1], lo[2], e, level, tolerance);
         if (outcome == 1 && key[0] < *best) *best = key[0];
       }
       if (outcome == 2) {
         const int d = tuv_level_dim(q, level);
-        ++e[d];
-        ++level;  // lower child: keys unchanged
+        put3(e, d, get3(e, d) + 1);
+        ++level;  // lower child: corner unchanged
         continue;
       }
     }
@@ -232,13 +273,18 @@ P2C_TUV_HD bool tuv_walk(const TuvPair& q, uint64_t kt, uint64_t ku, uint64_t kv
     visit = false;
     while (level > root) {
       const int d = tuv_level_dim(q, level - 1);
-      const uint64_t bit = kOne63 >> e[d];
-      if (key[d] & bit) {
-        key[d] &= ~bit;
-        --e[d];
+      const int ed = get3(e, d);
+      const uint64_t bit = kOne63 >> ed;
+      const double w = __builtin_ldexp(1.0, -ed);
+      const uint64_t kd = get3(key, d);
+      if (kd & bit) {
+        put3(key, d, kd & ~bit);
+        put3(lo, d, get3(lo, d) - w);
+        put3(e, d, ed - 1);
         --level;
       } else {
-        key[d] |= bit;  // upper sibling
+        put3(key, d, kd | bit);  // upper sibling
+        put3(lo, d, get3(lo, d) + w);
         visit = true;
         break;
       }
