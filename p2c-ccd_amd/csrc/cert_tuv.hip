@@ -144,67 +144,106 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_plain_kernel(const TuvPair* __r
   }
 }
 
-// One lane's resumable walk state (the loop body of tuv_walk, one box per step).
+// One lane's resumable walk over a pair's box tree.  `ep` packs the box's
+// exponents (e_t | e_u << 8 | e_v << 16; its widths are 2^-e).  `ready`: the
+// lane stands on a box still to be evaluated that passed box_skipped;
+// otherwise it is part-way through moving to the next such box.
 struct LaneWalk {
   uint64_t key[3];
-  double lo[3];  // the keys as exact doubles
-  uint64_t best, boxes;
-  int e[3];
+  uint64_t best;
+  uint32_t boxes;
+  uint32_t ep;
   uint32_t level;
-  bool visit;
+  bool ready;
 };
 
-__device__ inline void set_keys(LaneWalk& w, uint64_t kt, uint64_t ku, uint64_t kv) {
-  w.key[0] = kt;
-  w.key[1] = ku;
-  w.key[2] = kv;
-  w.lo[0] = key_to_f64(kt);
-  w.lo[1] = key_to_f64(ku);
-  w.lo[2] = key_to_f64(kv);
+__device__ inline int exp_of(uint32_t ep, int d) { return (int)((ep >> (8 * d)) & 0xffu); }
+
+__device__ inline uint32_t pack_exponents(const int e[3]) {
+  return (uint32_t)e[0] | ((uint32_t)e[1] << 8) | ((uint32_t)e[2] << 16);
 }
 
-// Advances one step; returns true when the pair's walk has finished.
-// Sets *over when the box budget ran out.
-__device__ inline bool lane_step(const TuvPair& q, LaneWalk& w, double tol, uint64_t budget, bool* over) {
-  if (w.visit) {
-    int outcome = 0;
-    const bool discard = (!q.is_ee && w.key[1] + w.key[2] > kOne63);
-    if (!discard && w.key[0] < w.best) {
-      if (w.boxes >= budget) {
-        *over = true;
-        return true;
-      }
-      ++w.boxes;
-      outcome = tuv_classify_at(q, w.lo[0], w.lo[1], w.lo[2], w.e, w.level, tol);
-      if (outcome == 1 && w.key[0] < w.best) w.best = w.key[0];
-    }
-    if (outcome == 2) {
-      const int d = tuv_level_dim(q, w.level);
-      put3(w.e, d, get3(w.e, d) + 1);
-      ++w.level;
-      return false;
-    }
-  }
-  w.visit = false;
-  while (w.level > 0) {
-    const int d = tuv_level_dim(q, w.level - 1);
-    const int ed = get3(w.e, d);
-    const uint64_t bit = kOne63 >> ed;
-    const double wd = __builtin_ldexp(1.0, -ed);
-    const uint64_t kd = get3(w.key, d);
-    if (kd & bit) {
-      put3(w.key, d, kd & ~bit);
-      put3(w.lo, d, get3(w.lo, d) - wd);
-      put3(w.e, d, ed - 1);
-      --w.level;
-    } else {
-      put3(w.key, d, kd | bit);
-      put3(w.lo, d, get3(w.lo, d) + wd);
-      w.visit = true;
-      return false;
-    }
-  }
+__device__ inline void start_walk(LaneWalk& w) {
+  w.key[0] = w.key[1] = w.key[2] = 0;
+  w.best = kTuvNoHit;
+  w.boxes = 0;
+  w.ep = 0;
+  w.level = 0;
+  w.ready = true;
+}
+
+// True when the box (and with it its whole subtree) needs no evaluation.
+__device__ inline bool box_skipped(const TuvPair& q, const LaneWalk& w) {
+  return (!q.is_ee && w.key[1] + w.key[2] > kOne63) || !(w.key[0] < w.best);
+}
+
+// Evaluates the box the lane stands on: 0 reject, 1 done, 2 split.
+__device__ inline int evaluate(const TuvPair& q, const LaneWalk& w, double tol) {
+  const int e[3] = {exp_of(w.ep, 0), exp_of(w.ep, 1), exp_of(w.ep, 2)};
+  return tuv_classify_at(q, key_to_f64(w.key[0]), key_to_f64(w.key[1]), key_to_f64(w.key[2]), e, w.level, tol);
+}
+
+// Moves to the lower child (same lower corner, so it passes box_skipped too).
+__device__ inline void descend(const TuvPair& q, LaneWalk& w) {
+  w.ep += 1u << (8 * tuv_level_dim(q, w.level));
+  ++w.level;
+}
+
+// One move of the pre-order walk towards the next box after the current
+// box's subtree: climbs one level out of an upper child, or steps from a
+// lower child to its upper sibling (ready unless skipped; a skipped sibling
+// is climbed out of by the next move).  False when the subtree rooted at
+// level `root` is exhausted.  Branch-free apart from the exit.
+__device__ inline bool advance_once(const TuvPair& q, LaneWalk& w, uint32_t root) {
+  if (w.level <= root) return false;
+  const int d = tuv_level_dim(q, w.level - 1);
+  const uint64_t bit = kOne63 >> exp_of(w.ep, d);
+  const uint64_t b0 = d == 0 ? bit : 0ull, b1 = d == 1 ? bit : 0ull, b2 = d == 2 ? bit : 0ull;
+  const bool upper = ((w.key[0] & b0) | (w.key[1] & b1) | (w.key[2] & b2)) != 0ull;
+  w.key[0] ^= b0;
+  w.key[1] ^= b1;
+  w.key[2] ^= b2;
+  w.ep -= upper ? (1u << (8 * d)) : 0u;
+  w.level -= upper ? 1u : 0u;
+  w.ready = !upper && !box_skipped(q, w);
   return true;
+}
+
+// Moves until ready; false when the subtree rooted at `root` is exhausted.
+__device__ inline bool walk_advance(const TuvPair& q, LaneWalk& w, uint32_t root) {
+  while (!w.ready)
+    if (!advance_once(q, w, root)) return false;
+  return true;
+}
+
+// One step of a lane's walk: evaluates the box it stands on (when ready),
+// then makes at most kAdvanceMoves moves towards the next box, so one lane's
+// long climb costs the wave a few cheap moves per step rather than a serial
+// loop.  Returns true when the walk has finished; sets *over (leaving the box
+// unevaluated) when the box budget ran out.
+constexpr int kAdvanceMoves = 2;
+
+__device__ inline bool lane_step(const TuvPair& q, LaneWalk& w, double tol, uint64_t budget, bool* over) {
+  if (w.ready) {
+    if (w.boxes >= budget) {
+      *over = true;
+      return true;
+    }
+    ++w.boxes;
+    const int outcome = evaluate(q, w, tol);
+    if (outcome == 2) {
+      descend(q, w);
+      return false;
+    }
+    if (outcome == 1 && w.key[0] < w.best) w.best = w.key[0];
+    w.ready = false;
+  }
+#pragma unroll
+  for (int m = 0; m < kAdvanceMoves; ++m) {
+    if (!advance_once(q, w, 0)) return true;
+    if (w.ready) return false;
+  }
+  return false;
 }
 
 // Pass 1 of the batched form.  A pair whose walk needs more than
@@ -238,12 +277,7 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_persistent_kernel(const TuvPair
         if (p < n) {
           pair = p;
           q = pairs[p];
-          set_keys(w, 0, 0, 0);
-          w.best = kTuvNoHit;
-          w.boxes = 0;
-          w.e[0] = w.e[1] = w.e[2] = 0;
-          w.level = 0;
-          w.visit = true;
+          start_walk(w);
         } else {
           pair = -2;
         }
@@ -274,9 +308,35 @@ struct WalkState {
   uint32_t boxes;
   uint16_t level;
   uint8_t e[3];
-  uint8_t visit;
+  uint8_t ready;
 };
 static_assert(sizeof(WalkState) == 48, "walk state layout");
+
+__device__ inline void save_walk(WalkState* st, const LaneWalk& w) {
+  WalkState s;
+  s.key[0] = w.key[0];
+  s.key[1] = w.key[1];
+  s.key[2] = w.key[2];
+  s.best = w.best;
+  s.boxes = w.boxes;
+  s.level = (uint16_t)w.level;
+  s.e[0] = (uint8_t)exp_of(w.ep, 0);
+  s.e[1] = (uint8_t)exp_of(w.ep, 1);
+  s.e[2] = (uint8_t)exp_of(w.ep, 2);
+  s.ready = w.ready ? 1 : 0;
+  *st = s;
+}
+
+__device__ inline void load_walk(LaneWalk& w, const WalkState& s) {
+  w.key[0] = s.key[0];
+  w.key[1] = s.key[1];
+  w.key[2] = s.key[2];
+  w.best = s.best;
+  w.boxes = s.boxes;
+  w.level = s.level;
+  w.ep = (uint32_t)s.e[0] | ((uint32_t)s.e[1] << 8) | ((uint32_t)s.e[2] << 16);
+  w.ready = s.ready != 0;
+}
 
 __device__ inline void wave_append(uint32_t value, bool want, uint32_t* list, uint32_t* count, uint32_t cap) {
   const uint64_t m = __ballot(want);
@@ -323,24 +383,8 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_round_kernel(const TuvPair* __r
       p = list_in == nullptr ? (uint32_t)j : list_in[j];
       const TuvPair q = pairs[p];
       LaneWalk w;
-      if (list_in == nullptr) {
-        set_keys(w, 0, 0, 0);
-        w.best = kTuvNoHit;
-        w.boxes = 0;
-        w.e[0] = w.e[1] = w.e[2] = 0;
-        w.level = 0;
-        w.visit = true;
-      } else {
-        const WalkState st = states[p];
-        set_keys(w, st.key[0], st.key[1], st.key[2]);
-        w.best = st.best;
-        w.boxes = st.boxes;
-        w.e[0] = st.e[0];
-        w.e[1] = st.e[1];
-        w.e[2] = st.e[2];
-        w.level = st.level;
-        w.visit = st.visit != 0;
-      }
+      if (list_in == nullptr) start_walk(w);
+      else load_walk(w, states[p]);
       bool done = false;
       for (uint32_t it = 0; it < budget && !done; ++it) done = lane_step(q, w, tol, max_itr, &over);
       if (done && !over) {
@@ -349,20 +393,7 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_round_kernel(const TuvPair* __r
         write_result(&out[p], w.best, w.boxes, P2C_TUV_FLAG_OVER_BUDGET);
       } else {
         keep = true;
-        {
-          WalkState st;
-          st.key[0] = w.key[0];
-          st.key[1] = w.key[1];
-          st.key[2] = w.key[2];
-          st.best = w.best;
-          st.boxes = (uint32_t)w.boxes;
-          st.level = (uint16_t)w.level;
-          st.e[0] = (uint8_t)w.e[0];
-          st.e[1] = (uint8_t)w.e[1];
-          st.e[2] = (uint8_t)w.e[2];
-          st.visit = w.visit ? 1 : 0;
-          states[p] = st;
-        }
+        save_walk(&states[p], w);
       }
     }
     wave_append(p, valid && over, replay_list, replay_count, list_cap);
@@ -370,16 +401,85 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_round_kernel(const TuvPair* __r
   }
 }
 
+// ---- stream: one lane per pair, refilled in batches -----------------------
+// A lane whose pair has finished idles until `refill_at` lanes of its wave
+// are idle; then every idle lane takes the next pair of the wave's claimed
+// chunk at once, so one memory latency serves many lanes and busy lanes never
+// wait on a refill.  A wave claims kStreamChunk pairs with one atomic.  A
+// pair that needs more than `light_budget` boxes saves its walk (the box it
+// stands on is not yet evaluated) and goes to the heavy list.
+constexpr int kStreamChunk = 256;
+
+__global__ __launch_bounds__(kTuvBlock) void tuv_stream_kernel(const TuvPair* __restrict__ pairs, int64_t n_host,
+                                                               const uint64_t* __restrict__ totals, double tol,
+                                                               uint64_t light_budget, uint32_t refill_at,
+                                                               p2c_tuv_result* __restrict__ out,
+                                                               unsigned long long* __restrict__ cursor,
+                                                               WalkState* __restrict__ states,
+                                                               uint32_t* __restrict__ heavy_list,
+                                                               uint32_t* __restrict__ heavy_count,
+                                                               uint32_t list_cap) {
+  const int64_t n = device_count(n_host, totals);
+  const int lane = threadIdx.x & 63;
+  const uint64_t lt = (1ull << lane) - 1ull;
+  int64_t next = 0, end = 0;  // the wave's claimed chunk (wave-uniform)
+  bool drained = false;       // no pairs left to claim (wave-uniform)
+  int64_t pair = -1;
+  TuvPair q;
+  LaneWalk w;
+  for (;;) {
+    uint64_t idle = __ballot(pair < 0);
+    if (!drained && (uint32_t)__popcll(idle) >= refill_at) {
+      while (idle != 0) {
+        if (next >= end) {
+          unsigned long long base = 0;
+          if (lane == 0) base = atomicAdd(cursor, (unsigned long long)kStreamChunk);
+          base = __shfl(base, 0);
+          next = (int64_t)base;
+          end = next + kStreamChunk < n ? next + kStreamChunk : n;
+          if (next >= n) {
+            drained = true;
+            break;
+          }
+        }
+        const int64_t avail = end - next;
+        const int rank = __popcll(idle & lt);
+        if (pair < 0 && rank < avail) {
+          pair = next + rank;
+          q = pairs[pair];
+          start_walk(w);
+        }
+        const int64_t took = avail < __popcll(idle) ? avail : (int64_t)__popcll(idle);
+        next += took;
+        idle = __ballot(pair < 0);
+      }
+    }
+    if (__ballot(pair >= 0) == 0) break;  // all idle after a refill attempt: nothing left
+    if (pair >= 0) {
+      bool over = false;
+      if (lane_step(q, w, tol, light_budget, &over)) {
+        if (over) {
+          save_walk(&states[pair], w);
+          flag_for_replay((uint32_t)pair, heavy_list, heavy_count, list_cap);
+        } else {
+          write_result(&out[pair], w.best, w.boxes, 0u);
+        }
+        pair = -1;
+      }
+    }
+  }
+}
+
 // ---- pass 2: one wave per heavy pair ------------------------------------
 // The 64 lanes of a wave share one pair's box tree.  Each lane walks a
-// subtree depth-first.  While lanes sit idle, every busy lane standing on a
-// fresh box donates all pending upper siblings on its path (shallowest taken
-// first) into the wave's LDS task queue and re-roots its walk at that box;
-// idle lanes take tasks.  The least done t_lo is shared through an LDS slot
-// (atomicMin) that every lane prunes with.  Boxes are counted for the whole
-// pair, including those a resumed walk already spent; a pair over max_itr
-// goes to the canonical replay.  Pairs handed over by the rounds resume
-// their saved walk (lane 0 holds it and donates its path at once).
+// subtree depth-first.  While lanes sit idle, every busy lane donates all
+// pending upper siblings on its path (shallowest taken first) into the
+// wave's LDS task queue and re-roots its walk at its current box; idle lanes
+// take tasks.  The least done t_lo is shared through an LDS slot (atomicMin)
+// that every lane prunes with.  Boxes are counted for the whole pair,
+// including those a resumed walk already spent; a pair over max_itr goes to
+// the canonical replay.  Pairs handed over by the other passes resume their
+// saved walk (lane 0 holds it and donates its path at once).
 constexpr int kWaveQueue = 448;
 
 struct WaveTask {
@@ -400,6 +500,12 @@ __device__ inline uint32_t wave_prefix_u8(uint32_t v, uint32_t* total) {
   }
   *total = tot;
   return pre;
+}
+
+__device__ inline void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
 __global__ __launch_bounds__(kTuvBlock) void tuv_heavy_kernel(const TuvPair* __restrict__ pairs,
@@ -428,80 +534,43 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_heavy_kernel(const TuvPair* __r
     uint32_t root = 0;
     bool has = lane == 0;
     uint64_t boxes = 0;
-    set_keys(w, 0, 0, 0);
-    w.e[0] = w.e[1] = w.e[2] = 0;
-    w.level = 0;
-    w.visit = true;
-    w.best = kTuvNoHit;
-    w.boxes = 0;
+    start_walk(w);
     if (resume != nullptr) {
       const WalkState st = resume[p];
       boxes = st.boxes;
       w.best = st.best;
       if (lane == 0) {
-        set_keys(w, st.key[0], st.key[1], st.key[2]);
-        w.e[0] = st.e[0];
-        w.e[1] = st.e[1];
-        w.e[2] = st.e[2];
-        w.level = st.level;
-        w.visit = st.visit != 0;
+        load_walk(w, st);
+        has = walk_advance(q, w, 0);
       }
     }
     if (lane == 0) best_slot[wv] = w.best;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    wave_sync_lds();
     int qsize = 0;
     bool over = false;
     for (;;) {
-      // 1. one step of every busy lane's subtree walk
+      // 1. one step of every busy lane's subtree walk: evaluate, then descend
+      //    or advance (a box pruned meanwhile by the shared best is skipped)
       bool evaluated = false;
       if (has) {
-        if (w.visit) {
-          int outcome = 0;
-          const bool discard = (!q.is_ee && w.key[1] + w.key[2] > kOne63);
-          if (!discard && w.key[0] < w.best) {
-            evaluated = true;
-            outcome = tuv_classify_at(q, w.lo[0], w.lo[1], w.lo[2], w.e, w.level, tol);
-            if (outcome == 1 && w.key[0] < w.best) {
-              w.best = w.key[0];
-              atomicMin(&best_slot[wv], (unsigned long long)w.key[0]);
-            }
-          }
-          if (outcome == 2) {
-            const int d = tuv_level_dim(q, w.level);
-            put3(w.e, d, get3(w.e, d) + 1);
-            ++w.level;
-          } else {
-            w.visit = false;
+        int outcome = 0;
+        if (w.key[0] < w.best) {
+          evaluated = true;
+          outcome = evaluate(q, w, tol);
+          if (outcome == 1 && w.key[0] < w.best) {
+            w.best = w.key[0];
+            atomicMin(&best_slot[wv], (unsigned long long)w.key[0]);
           }
         }
-        if (!w.visit) {
-          while (w.level > root) {
-            const int d = tuv_level_dim(q, w.level - 1);
-            const int ed = get3(w.e, d);
-            const uint64_t bit = kOne63 >> ed;
-            const double wd = __builtin_ldexp(1.0, -ed);
-            const uint64_t kd = get3(w.key, d);
-            if (kd & bit) {
-              put3(w.key, d, kd & ~bit);
-              put3(w.lo, d, get3(w.lo, d) - wd);
-              put3(w.e, d, ed - 1);
-              --w.level;
-            } else {
-              put3(w.key, d, kd | bit);
-              put3(w.lo, d, get3(w.lo, d) + wd);
-              w.visit = true;
-              break;
-            }
-          }
-          if (!w.visit) has = false;
+        if (outcome == 2) {
+          descend(q, w);
+        } else {
+          w.ready = false;
+          has = walk_advance(q, w, root);
         }
       }
       boxes += (uint64_t)__popcll(__ballot(evaluated));
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      wave_sync_lds();
       {
         const uint64_t shared_best = best_slot[wv];
         if (shared_best < w.best) w.best = shared_best;
@@ -510,21 +579,20 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_heavy_kernel(const TuvPair* __r
         over = true;
         break;
       }
-      // 2. donation: every busy lane on a fresh box gives away its path
+      // 2. donation: every busy lane gives away the pending siblings on its path
       const int idle = 64 - __popcll(__ballot(has));
       if (idle > qsize) {
         uint32_t pend = 0;
-        if (has && w.visit) {
+        if (has) {
           uint64_t k[3] = {w.key[0], w.key[1], w.key[2]};
-          int e[3] = {w.e[0], w.e[1], w.e[2]};
+          uint32_t ep = w.ep;
           for (uint32_t l = w.level; l > root; --l) {
             const int d = tuv_level_dim(q, l - 1);
-            const int ed = get3(e, d);
-            const uint64_t bit = kOne63 >> ed;
+            const uint64_t bit = kOne63 >> exp_of(ep, d);
             const uint64_t kd = get3(k, d);
             pend += (kd & bit) ? 0u : 1u;
             put3(k, d, kd & ~bit);
-            put3(e, d, ed - 1);
+            ep -= 1u << (8 * d);
           }
         }
         uint32_t tot = 0;
@@ -532,12 +600,11 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_heavy_kernel(const TuvPair* __r
         const bool fits = qsize + (int)(off + pend) <= kWaveQueue;
         if (pend > 0 && fits) {
           uint64_t k[3] = {w.key[0], w.key[1], w.key[2]};
-          int e[3] = {w.e[0], w.e[1], w.e[2]};
+          uint32_t ep = w.ep;
           int slot = qsize + (int)off;
           for (uint32_t l = w.level; l > root; --l) {  // deepest first: shallowest ends on top
             const int d = tuv_level_dim(q, l - 1);
-            const int ed = get3(e, d);
-            const uint64_t bit = kOne63 >> ed;
+            const uint64_t bit = kOne63 >> exp_of(ep, d);
             const uint64_t kd = get3(k, d);
             if (!(kd & bit)) {
               WaveTask t;
@@ -549,7 +616,7 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_heavy_kernel(const TuvPair* __r
               wq[slot++] = t;
             }
             put3(k, d, kd & ~bit);
-            put3(e, d, ed - 1);
+            ep -= 1u << (8 * d);
           }
           root = w.level;
         }
@@ -558,9 +625,7 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_heavy_kernel(const TuvPair* __r
         wave_prefix_u8((pend > 0 && fits) ? pend : 0u, &pushed_tot);
         qsize += (int)pushed_tot;
       }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      wave_sync_lds();
       // 3. idle lanes take queued subtrees, the top of the stack first
       const uint64_t want = __ballot(!has);
       const int n_take = min(__popcll(want), qsize);
@@ -568,18 +633,20 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_heavy_kernel(const TuvPair* __r
         const int rank = __popcll(want & lt);
         if (rank < n_take) {
           const WaveTask t = wq[qsize - 1 - rank];
-          set_keys(w, t.kt, t.ku, t.kv);
+          w.key[0] = t.kt;
+          w.key[1] = t.ku;
+          w.key[2] = t.kv;
           w.level = t.level;
           root = t.level;
-          tuv_exponents(q, t.level, w.e);
-          w.visit = true;
-          has = true;
+          int e[3];
+          tuv_exponents(q, t.level, e);
+          w.ep = pack_exponents(e);
+          w.ready = true;
+          has = !box_skipped(q, w);  // a skipped task's whole subtree is skipped
         }
       }
       qsize -= n_take;
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      wave_sync_lds();
       if (__ballot(has) == 0 && qsize == 0) break;
     }
     const uint64_t final_best = best_slot[wv] < w.best ? best_slot[wv] : w.best;
@@ -828,6 +895,29 @@ int p2c_tuv_rounds(const void* pairs, int64_t n, const uint64_t* totals, double 
   }
   const uint32_t* heavy = lists[(n_rounds - 1) & 1];
   hipLaunchKernelGGL(p2c::tuv_heavy_kernel, dim3(grid_blocks), dim3(p2c::kTuvBlock), 0, s, tp, heavy, c32 + 3,
+                     list_cap, c32 + 4, st, tolerance, (uint64_t)max_itr, out, replay_list, c32 + 2);
+  P2C_HIP_CHECK(hipGetLastError());
+  return P2C_OK;
+}
+
+int p2c_tuv_stream(const void* pairs, int64_t n, const uint64_t* totals, double tolerance, int64_t max_itr,
+                   int64_t light_budget, int refill_at, p2c_tuv_result* out, void* counters, void* states,
+                   uint32_t* heavy_list, uint32_t* replay_list, uint32_t list_cap, int grid_blocks, void* stream) {
+  if (int rc = p2c::check_tuv(0.0, tolerance, 1.0, max_itr)) return rc;
+  if (n <= 0) return P2C_OK;
+  if (grid_blocks <= 0) return p2c::set_error(P2C_EINVAL, "grid_blocks must be positive");
+  if (light_budget <= 0) return p2c::set_error(P2C_EINVAL, "light_budget must be positive");
+  if (refill_at < 1 || refill_at > 64) return p2c::set_error(P2C_EINVAL, "refill_at must be in 1..64");
+  if (light_budget > max_itr) light_budget = max_itr;
+  // counters: [0] next pair (u64), [1] replay count (u32) | heavy count (u32), [2] heavy cursor (u32)
+  auto* c64 = static_cast<unsigned long long*>(counters);
+  auto* c32 = static_cast<uint32_t*>(counters);
+  auto* st = static_cast<p2c::WalkState*>(states);
+  const auto* tp = static_cast<const p2c::TuvPair*>(pairs);
+  hipStream_t s = p2c::as_stream(stream);
+  hipLaunchKernelGGL(p2c::tuv_stream_kernel, dim3(grid_blocks), dim3(p2c::kTuvBlock), 0, s, tp, n, totals, tolerance,
+                     (uint64_t)light_budget, (uint32_t)refill_at, out, c64, st, heavy_list, c32 + 3, list_cap);
+  hipLaunchKernelGGL(p2c::tuv_heavy_kernel, dim3(grid_blocks), dim3(p2c::kTuvBlock), 0, s, tp, heavy_list, c32 + 3,
                      list_cap, c32 + 4, st, tolerance, (uint64_t)max_itr, out, replay_list, c32 + 2);
   P2C_HIP_CHECK(hipGetLastError());
   return P2C_OK;

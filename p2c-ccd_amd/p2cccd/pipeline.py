@@ -6,8 +6,9 @@ Two execution forms with identical answers:
   plain    one launch per stage; the host reads the candidate counts back to
            size the pair buffers, and the certificate runs one lane per pair.
   batched  every stage enqueued back to back with the counts kept on the
-           device, the certificate as the persistent lane-refill kernel, the
-           whole step captured once in a HIP graph and replayed.
+           device, the certificate as the stream kernel (batched lane refill)
+           plus the wave-cooperative pass, the whole step captured once in a
+           HIP graph and replayed.
 
 This is the reference's all-exact path (src/tools/scene_object_envelope_native
 _ti_walltime.cpp:1239-1281: every candidate certified) on the device.
@@ -46,7 +47,7 @@ class CcdPipeline:
         self.thickness = scene.thickness
         self.cert = TuvCertifier(config, self.device)
         self.headroom = headroom
-        self.batched_mode = "rounds"
+        self.batched_mode = "stream"
         self.cap_vf = self.cap_ee = 0
         self.vf = self.ee = None
         self.graph: torch.cuda.CUDAGraph | None = None

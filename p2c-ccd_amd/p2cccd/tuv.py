@@ -1,12 +1,15 @@
 """(t,u,v) inclusion certificate on the GPU: host-side driver over the C-ABI.
 
 `TuvCertifier` owns the device workspaces (per-pair records, results, the
-fallback-replay list and heaps) and runs one of three execution forms:
+fallback-replay list and heaps) and runs one of four execution forms:
   plain       one lane per pair, one launch (the per-stage baseline)
+  stream      chip-sized grid, one lane per pair; a wave refills its idle lanes
+              together once `refill_at` of them are idle; pairs over
+              `light_budget` boxes save their walk for the wave-cooperative pass
   rounds      budgeted one-lane-per-pair rounds that save unfinished walks and
-              compact them, then a wave-cooperative pass for the heaviest pairs
-  persistent  chip-sized grid whose lanes refill as pairs finish, then the
-              wave-cooperative pass
+              compact them, then the wave-cooperative pass
+  persistent  chip-sized grid whose lanes refill one by one as pairs finish,
+              then the wave-cooperative pass
 All are followed by the fallback replay of pairs that exceeded max_itr, so
 every pair ends with a final collide/separate decision.
 
@@ -35,6 +38,7 @@ nat._sig("p2c_tuv_setup_indexed", [_i32, _vp, _vp, _vp, _i64, _f64, _f64, _vp, _
 nat._sig("p2c_tuv_setup_pairs", [_i32, _vp, _vp, _vp, _vp, _i64, _vp, _i64, _i64, _f64, _f64, _vp, _vp])
 nat._sig("p2c_tuv_plain", [_vp, _i64, _vp, _f64, _i64, _vp, _vp, _vp, _u32, _vp])
 nat._sig("p2c_tuv_persistent", [_vp, _i64, _vp, _f64, _i64, _i64, _vp, _vp, _vp, _vp, _u32, _i32, _vp])
+nat._sig("p2c_tuv_stream", [_vp, _i64, _vp, _f64, _i64, _i64, _i32, _vp, _vp, _vp, _vp, _vp, _u32, _i32, _vp])
 nat._sig("p2c_tuv_rounds", [_vp, _i64, _vp, _f64, _i64, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _u32, _i32, _vp])
 nat.lib.p2c_tuv_state_bytes.restype = nat.ctypes.c_size_t
 nat._sig("p2c_tuv_replay", [_vp, _vp, _vp, _u32, _f64, _i64, _vp, _i64, _i64, _vp, _vp])
@@ -50,7 +54,8 @@ class TuvConfig:
     heap_cap: int = 1 << 20      # canonical-replay heap entries per slot
     replay_slots: int = 8        # pairs replayed concurrently
     blocks_per_cu: int = 3       # persistent grid: resident blocks of 256 per CU
-    light_budget: int = 2048     # boxes a pair may take in the lane-refill pass before the wave pass
+    light_budget: int = 1024     # boxes a pair may take in the stream/persistent pass before the wave pass
+    refill_at: int = 8           # stream: idle lanes of a wave that trigger a refill
     round_budgets: tuple = (8, 32, 128, 512)  # walk steps per pair per round before the wave pass
 
 
@@ -107,7 +112,7 @@ class TuvCertifier:
                  self.cfg.ms, self.cfg.tolerance, self.pairs.data_ptr(), nat.stream_handle(stream))
         return n
 
-    def certify(self, n: int, mode: str = "rounds", stream=None, totals: torch.Tensor | None = None,
+    def certify(self, n: int, mode: str = "stream", stream=None, totals: torch.Tensor | None = None,
                 events: tuple | None = None) -> None:
         """Certifies the pairs set up last (n, or min(n, totals[0] + totals[1])
         when device totals are given); results land in self.results.
@@ -127,6 +132,11 @@ class TuvCertifier:
                      self.budgets.ctypes.data, len(self.budgets), self.results.data_ptr(), c.data_ptr(),
                      self.states.data_ptr(), self.round_list.data_ptr(), self.heavy_list.data_ptr(),
                      self.replay_list.data_ptr(), self._cap, self.grid_blocks, s)
+        elif mode == "stream":
+            nat.call("p2c_tuv_stream", self.pairs.data_ptr(), n, tp, self.cfg.tolerance, self.cfg.max_itr,
+                     self.cfg.light_budget, self.cfg.refill_at, self.results.data_ptr(), c.data_ptr(),
+                     self.states.data_ptr(), self.heavy_list.data_ptr(), self.replay_list.data_ptr(), self._cap,
+                     self.grid_blocks, s)
         elif mode == "persistent":
             nat.call("p2c_tuv_persistent", self.pairs.data_ptr(), n, tp, self.cfg.tolerance, self.cfg.max_itr,
                      self.cfg.light_budget, self.results.data_ptr(), c.data_ptr(), self.heavy_list.data_ptr(),
@@ -156,7 +166,7 @@ class TuvCertifier:
         return self.results[: n * TUV_RESULT_DTYPE.itemsize]
 
     # -- convenience -------------------------------------------------------
-    def certify_positions(self, x, is_ee: bool, mode: str = "rounds") -> np.ndarray:
+    def certify_positions(self, x, is_ee: bool, mode: str = "stream") -> np.ndarray:
         if isinstance(x, np.ndarray):
             x = torch.from_numpy(np.ascontiguousarray(x, dtype=np.float64)).to(self.device)
         n = self.setup_positions(x, is_ee)

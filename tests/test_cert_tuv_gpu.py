@@ -19,7 +19,7 @@ def _check(got, ref, label):
     assert np.array_equal(got["toi"], ref["toi"]), label
 
 
-@pytest.mark.parametrize("mode", ["plain", "persistent", "rounds"])
+@pytest.mark.parametrize("mode", ["plain", "stream", "persistent", "rounds"])
 def test_plain_workload_matches_oracle(gpu, mode):
     qs = workloads.make_query_set(10_000, 10_000)
     cert = TuvCertifier()
@@ -31,7 +31,7 @@ def test_plain_workload_matches_oracle(gpu, mode):
         assert got["hit"][(qs.vf_kind if not is_ee else qs.ee_kind) == 1].all()  # touching pairs all hit
 
 
-@pytest.mark.parametrize("mode", ["plain", "persistent", "rounds"])
+@pytest.mark.parametrize("mode", ["plain", "stream", "persistent", "rounds"])
 def test_forced_replay_reproduces_capped_oracle(gpu, mode):
     qs = workloads.make_query_set(2000, 2000, seed=31)
     cfg = TuvConfig(max_itr=40, heap_cap=1 << 12, replay_slots=16)
@@ -79,3 +79,35 @@ def test_rounds_with_tiny_budgets_match_oracle(gpu, budgets):
         ref = oracle.tuv_cert(is_ee, x, threads=8)
         _check(got, ref, (budgets, is_ee))
         assert cert.heavy_count() > 0
+
+
+@pytest.mark.parametrize("refill_at", [1, 8, 64])
+@pytest.mark.parametrize("light_budget", [4, 1024])
+def test_stream_refill_and_handover_match_oracle(gpu, refill_at, light_budget):
+    # refill_at 1 refills a lane as soon as it idles, 64 only when the whole
+    # wave is idle; light budget 4 sends most pairs, mid-walk, to the wave pass.
+    qs = workloads.make_query_set(3000, 3000, seed=67)
+    cert = TuvCertifier(TuvConfig(light_budget=light_budget, refill_at=refill_at))
+    for is_ee, x in ((False, qs.vf), (True, qs.ee)):
+        got = cert.certify_positions(x, is_ee, "stream")
+        ref = oracle.tuv_cert(is_ee, x, threads=8)
+        _check(got, ref, (refill_at, light_budget, is_ee))
+        assert cert.replay_count() == 0
+        if light_budget == 4:
+            assert cert.heavy_count() > 0
+
+
+def test_single_lane_forms_walk_the_same_boxes(gpu):
+    # Without a hand-over to the wave pass, the stream and rounds forms walk
+    # each pair's boxes in the plain kernel's depth-first order, so even the
+    # per-pair box counts agree.
+    qs = workloads.make_query_set(4000, 4000, seed=73)
+    for is_ee, x in ((False, qs.vf), (True, qs.ee)):
+        plain = TuvCertifier().certify_positions(x, is_ee, "plain")
+        big = TuvConfig(light_budget=1_000_000, round_budgets=(1_000_000,))
+        for mode in ("stream", "rounds"):
+            c = TuvCertifier(big)
+            got = c.certify_positions(x, is_ee, mode)
+            assert c.heavy_count() == 0
+            _check(got, plain, (mode, is_ee))
+            assert np.array_equal(got["boxes"], plain["boxes"]), (mode, is_ee)
