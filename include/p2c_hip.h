@@ -195,12 +195,19 @@ int p2c_tuv_rounds(const void* pairs, int64_t n, const uint64_t* totals, double 
 /* Batched form by streaming: a chip-sized grid, one lane per pair; idle lanes
  * are refilled together once `refill_at` (1..64) lanes of a wave are idle.
  * Pairs needing more than `light_budget` boxes save their walk in `states`
- * (n * p2c_tuv_state_bytes() bytes) and finish in the wave-cooperative pass.
- * `counters`: 64 zero-initialised device bytes, laid out as for
- * p2c_tuv_persistent. */
+ * (n * p2c_tuv_state_bytes() bytes) and are listed in `heavy_list`, as are
+ * the walks of a wave left with `handover_at` (0..64) or fewer busy lanes
+ * once every pair has been claimed; p2c_tuv_wave finishes them.  `counters`:
+ * 64 zero-initialised device bytes, laid out as for p2c_tuv_persistent. */
 int p2c_tuv_stream(const void* pairs, int64_t n, const uint64_t* totals, double tolerance, int64_t max_itr,
-                   int64_t light_budget, int refill_at, p2c_tuv_result* out, void* counters, void* states,
-                   uint32_t* heavy_list, uint32_t* replay_list, uint32_t list_cap, int grid_blocks, void* stream);
+                   int64_t light_budget, int refill_at, int handover_at, p2c_tuv_result* out, void* counters,
+                   void* states, uint32_t* heavy_list, uint32_t list_cap, int grid_blocks, void* stream);
+/* Wave-cooperative pass over the pairs listed by p2c_tuv_stream (same
+ * counters, states and list): one wave per pair, resuming the saved walks.
+ * Pairs over max_itr are appended to `replay_list` for p2c_tuv_replay. */
+int p2c_tuv_wave(const void* pairs, void* counters, const void* states, const uint32_t* heavy_list,
+                 uint32_t list_cap, double tolerance, int64_t max_itr, p2c_tuv_result* out, uint32_t* replay_list,
+                 int grid_blocks, void* stream);
 /* Fallback replay of the listed pairs in canonical order; `heaps` holds
  * slots * p2c_tuv_heap_bytes(heap_cap) bytes of device memory. */
 int p2c_tuv_replay(const void* pairs, const uint32_t* replay_list, const uint32_t* replay_count,

@@ -311,6 +311,7 @@ struct WalkState {
   uint8_t ready;
 };
 static_assert(sizeof(WalkState) == 48, "walk state layout");
+static_assert(kTuvMaxLevel < 256, "pending sibling counts fit 8 bits");
 
 __device__ inline void save_walk(WalkState* st, const LaneWalk& w) {
   WalkState s;
@@ -406,13 +407,16 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_round_kernel(const TuvPair* __r
 // are idle; then every idle lane takes the next pair of the wave's claimed
 // chunk at once, so one memory latency serves many lanes and busy lanes never
 // wait on a refill.  A wave claims kStreamChunk pairs with one atomic.  A
-// pair that needs more than `light_budget` boxes saves its walk (the box it
-// stands on is not yet evaluated) and goes to the heavy list.
+// pair that needs more than `light_budget` boxes saves its walk and goes to
+// the heavy list.  Once no pairs are left to claim, a wave whose busy lanes
+// number `handover_at` or fewer hands them all to the heavy list too, so a
+// few long walks do not hold a nearly empty wave.
 constexpr int kStreamChunk = 256;
 
 __global__ __launch_bounds__(kTuvBlock) void tuv_stream_kernel(const TuvPair* __restrict__ pairs, int64_t n_host,
                                                                const uint64_t* __restrict__ totals, double tol,
                                                                uint64_t light_budget, uint32_t refill_at,
+                                                               uint32_t handover_at,
                                                                p2c_tuv_result* __restrict__ out,
                                                                unsigned long long* __restrict__ cursor,
                                                                WalkState* __restrict__ states,
@@ -454,7 +458,15 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_stream_kernel(const TuvPair* __
         idle = __ballot(pair < 0);
       }
     }
-    if (__ballot(pair >= 0) == 0) break;  // all idle after a refill attempt: nothing left
+    const uint64_t busy = __ballot(pair >= 0);
+    if (busy == 0) break;  // all idle after a refill attempt: nothing left
+    if (drained && (uint32_t)__popcll(busy) <= handover_at) {
+      if (pair >= 0) {
+        save_walk(&states[pair], w);
+        flag_for_replay((uint32_t)pair, heavy_list, heavy_count, list_cap);
+      }
+      break;
+    }
     if (pair >= 0) {
       bool over = false;
       if (lane_step(q, w, tol, light_budget, &over)) {
@@ -472,19 +484,21 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_stream_kernel(const TuvPair* __
 
 // ---- pass 2: one wave per heavy pair ------------------------------------
 // The 64 lanes of a wave share one pair's box tree.  Each lane walks a
-// subtree depth-first.  While lanes sit idle, every busy lane donates all
-// pending upper siblings on its path (shallowest taken first) into the
-// wave's LDS task queue and re-roots its walk at its current box; idle lanes
-// take tasks.  The least done t_lo is shared through an LDS slot (atomicMin)
-// that every lane prunes with.  Boxes are counted for the whole pair,
-// including those a resumed walk already spent; a pair over max_itr goes to
-// the canonical replay.  Pairs handed over by the other passes resume their
-// saved walk (lane 0 holds it and donates its path at once).
-constexpr int kWaveQueue = 448;
+// subtree depth-first (one evaluation and at most kAdvanceMoves moves per
+// step, as in lane_step).  While lanes sit idle, every lane standing on a
+// box donates all pending upper siblings on its path (shallowest taken
+// first) into the wave's LDS task stack and re-roots its walk at its current
+// box; idle lanes take tasks.  The least done t_lo is shared through an LDS
+// slot (atomicMin) that every lane prunes with.  Boxes are counted for the
+// whole pair, including those a resumed walk already spent; a pair over
+// max_itr goes to the canonical replay.  Pairs handed over by the other
+// passes resume their saved walk (lane 0 holds it and donates its path at
+// once).
+constexpr int kWaveQueue = 384;
 
 struct WaveTask {
   uint64_t kt, ku, kv;
-  uint32_t level, pad;
+  uint32_t level, ep;
 };
 
 // Exclusive wave prefix sum of 8-bit values through bit-sliced ballots.
@@ -506,6 +520,22 @@ __device__ inline void wave_sync_lds() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// Pending upper siblings on the path between level `root` (exponents
+// root_ep) and the current box: the path's lower-child choices, i.e. the
+// levels minus the key bits set below the root.
+__device__ inline uint32_t pending_siblings(const LaneWalk& w, uint32_t root, uint32_t root_ep) {
+  uint32_t set = 0;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int hi = 63 - exp_of(root_ep, d);  // key bits below the root's splits on d: [lo, hi)
+    const int lo = 63 - exp_of(w.ep, d);
+    const uint64_t below_hi = hi >= 64 ? ~0ull : ((1ull << hi) - 1ull);
+    const uint64_t below_lo = (1ull << lo) - 1ull;
+    set += (uint32_t)__popcll(get3(w.key, d) & (below_hi & ~below_lo));
+  }
+  return (w.level - root) - set;
 }
 
 __global__ __launch_bounds__(kTuvBlock) void tuv_heavy_kernel(const TuvPair* __restrict__ pairs,
@@ -531,28 +561,25 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_heavy_kernel(const TuvPair* __r
     const uint32_t p = heavy_list[h];
     const TuvPair q = pairs[p];
     LaneWalk w;
-    uint32_t root = 0;
-    bool has = lane == 0;
+    uint32_t root = 0, root_ep = 0;
+    bool has = lane == 0;  // the lane holds a walk (standing on a box, or moving to the next)
     uint64_t boxes = 0;
     start_walk(w);
     if (resume != nullptr) {
       const WalkState st = resume[p];
       boxes = st.boxes;
       w.best = st.best;
-      if (lane == 0) {
-        load_walk(w, st);
-        has = walk_advance(q, w, 0);
-      }
+      if (lane == 0) load_walk(w, st);
     }
     if (lane == 0) best_slot[wv] = w.best;
     wave_sync_lds();
     int qsize = 0;
     bool over = false;
     for (;;) {
-      // 1. one step of every busy lane's subtree walk: evaluate, then descend
-      //    or advance (a box pruned meanwhile by the shared best is skipped)
+      // 1. evaluate (a box pruned meanwhile by the shared best is skipped),
+      //    then descend, or make at most kAdvanceMoves moves
       bool evaluated = false;
-      if (has) {
+      if (has && w.ready) {
         int outcome = 0;
         if (w.key[0] < w.best) {
           evaluated = true;
@@ -562,11 +589,17 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_heavy_kernel(const TuvPair* __r
             atomicMin(&best_slot[wv], (unsigned long long)w.key[0]);
           }
         }
-        if (outcome == 2) {
-          descend(q, w);
-        } else {
-          w.ready = false;
-          has = walk_advance(q, w, root);
+        if (outcome == 2) descend(q, w);
+        else w.ready = false;
+      }
+      if (has && !w.ready) {
+#pragma unroll
+        for (int m = 0; m < kAdvanceMoves; ++m) {
+          if (!advance_once(q, w, root)) {
+            has = false;
+            break;
+          }
+          if (w.ready) break;
         }
       }
       boxes += (uint64_t)__popcll(__ballot(evaluated));
@@ -579,26 +612,16 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_heavy_kernel(const TuvPair* __r
         over = true;
         break;
       }
-      // 2. donation: every busy lane gives away the pending siblings on its path
       const int idle = 64 - __popcll(__ballot(has));
+      if (idle == 0) continue;
+      // 2. donation: lanes standing on a box give away the pending siblings on their path
       if (idle > qsize) {
-        uint32_t pend = 0;
-        if (has) {
-          uint64_t k[3] = {w.key[0], w.key[1], w.key[2]};
-          uint32_t ep = w.ep;
-          for (uint32_t l = w.level; l > root; --l) {
-            const int d = tuv_level_dim(q, l - 1);
-            const uint64_t bit = kOne63 >> exp_of(ep, d);
-            const uint64_t kd = get3(k, d);
-            pend += (kd & bit) ? 0u : 1u;
-            put3(k, d, kd & ~bit);
-            ep -= 1u << (8 * d);
-          }
-        }
+        // at most kTuvMaxLevel (156) pending siblings: within wave_prefix_u8's range
+        const uint32_t pend = has && w.ready ? pending_siblings(w, root, root_ep) : 0u;
         uint32_t tot = 0;
         const uint32_t off = wave_prefix_u8(pend, &tot);
-        const bool fits = qsize + (int)(off + pend) <= kWaveQueue;
-        if (pend > 0 && fits) {
+        const bool push = pend > 0 && qsize + (int)(off + pend) <= kWaveQueue;
+        if (push) {
           uint64_t k[3] = {w.key[0], w.key[1], w.key[2]};
           uint32_t ep = w.ep;
           int slot = qsize + (int)off;
@@ -612,41 +635,43 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_heavy_kernel(const TuvPair* __r
               t.ku = k[1] | (d == 1 ? bit : 0ull);
               t.kv = k[2] | (d == 2 ? bit : 0ull);
               t.level = l;
-              t.pad = 0;
+              t.ep = ep;
               wq[slot++] = t;
             }
             put3(k, d, kd & ~bit);
             ep -= 1u << (8 * d);
           }
           root = w.level;
+          root_ep = w.ep;
         }
         // lanes whose block did not fit pushed nothing; count what was pushed
         uint32_t pushed_tot = 0;
-        wave_prefix_u8((pend > 0 && fits) ? pend : 0u, &pushed_tot);
+        wave_prefix_u8(push ? pend : 0u, &pushed_tot);
         qsize += (int)pushed_tot;
+        wave_sync_lds();
       }
-      wave_sync_lds();
       // 3. idle lanes take queued subtrees, the top of the stack first
       const uint64_t want = __ballot(!has);
       const int n_take = min(__popcll(want), qsize);
-      if (!has) {
-        const int rank = __popcll(want & lt);
-        if (rank < n_take) {
-          const WaveTask t = wq[qsize - 1 - rank];
-          w.key[0] = t.kt;
-          w.key[1] = t.ku;
-          w.key[2] = t.kv;
-          w.level = t.level;
-          root = t.level;
-          int e[3];
-          tuv_exponents(q, t.level, e);
-          w.ep = pack_exponents(e);
-          w.ready = true;
-          has = !box_skipped(q, w);  // a skipped task's whole subtree is skipped
+      if (n_take > 0) {
+        if (!has) {
+          const int rank = __popcll(want & lt);
+          if (rank < n_take) {
+            const WaveTask t = wq[qsize - 1 - rank];
+            w.key[0] = t.kt;
+            w.key[1] = t.ku;
+            w.key[2] = t.kv;
+            w.level = t.level;
+            w.ep = t.ep;
+            root = t.level;
+            root_ep = t.ep;
+            w.ready = true;
+            has = !box_skipped(q, w);  // a skipped task's whole subtree is skipped
+          }
         }
+        qsize -= n_take;
+        wave_sync_lds();
       }
-      qsize -= n_take;
-      wave_sync_lds();
       if (__ballot(has) == 0 && qsize == 0) break;
     }
     const uint64_t final_best = best_slot[wv] < w.best ? best_slot[wv] : w.best;
@@ -901,13 +926,14 @@ int p2c_tuv_rounds(const void* pairs, int64_t n, const uint64_t* totals, double 
 }
 
 int p2c_tuv_stream(const void* pairs, int64_t n, const uint64_t* totals, double tolerance, int64_t max_itr,
-                   int64_t light_budget, int refill_at, p2c_tuv_result* out, void* counters, void* states,
-                   uint32_t* heavy_list, uint32_t* replay_list, uint32_t list_cap, int grid_blocks, void* stream) {
+                   int64_t light_budget, int refill_at, int handover_at, p2c_tuv_result* out, void* counters,
+                   void* states, uint32_t* heavy_list, uint32_t list_cap, int grid_blocks, void* stream) {
   if (int rc = p2c::check_tuv(0.0, tolerance, 1.0, max_itr)) return rc;
   if (n <= 0) return P2C_OK;
   if (grid_blocks <= 0) return p2c::set_error(P2C_EINVAL, "grid_blocks must be positive");
   if (light_budget <= 0) return p2c::set_error(P2C_EINVAL, "light_budget must be positive");
   if (refill_at < 1 || refill_at > 64) return p2c::set_error(P2C_EINVAL, "refill_at must be in 1..64");
+  if (handover_at < 0 || handover_at > 64) return p2c::set_error(P2C_EINVAL, "handover_at must be in 0..64");
   if (light_budget > max_itr) light_budget = max_itr;
   // counters: [0] next pair (u64), [1] replay count (u32) | heavy count (u32), [2] heavy cursor (u32)
   auto* c64 = static_cast<unsigned long long*>(counters);
@@ -916,9 +942,22 @@ int p2c_tuv_stream(const void* pairs, int64_t n, const uint64_t* totals, double 
   const auto* tp = static_cast<const p2c::TuvPair*>(pairs);
   hipStream_t s = p2c::as_stream(stream);
   hipLaunchKernelGGL(p2c::tuv_stream_kernel, dim3(grid_blocks), dim3(p2c::kTuvBlock), 0, s, tp, n, totals, tolerance,
-                     (uint64_t)light_budget, (uint32_t)refill_at, out, c64, st, heavy_list, c32 + 3, list_cap);
-  hipLaunchKernelGGL(p2c::tuv_heavy_kernel, dim3(grid_blocks), dim3(p2c::kTuvBlock), 0, s, tp, heavy_list, c32 + 3,
-                     list_cap, c32 + 4, st, tolerance, (uint64_t)max_itr, out, replay_list, c32 + 2);
+                     (uint64_t)light_budget, (uint32_t)refill_at, (uint32_t)handover_at, out, c64, st, heavy_list,
+                     c32 + 3, list_cap);
+  P2C_HIP_CHECK(hipGetLastError());
+  return P2C_OK;
+}
+
+int p2c_tuv_wave(const void* pairs, void* counters, const void* states, const uint32_t* heavy_list,
+                 uint32_t list_cap, double tolerance, int64_t max_itr, p2c_tuv_result* out, uint32_t* replay_list,
+                 int grid_blocks, void* stream) {
+  if (int rc = p2c::check_tuv(0.0, tolerance, 1.0, max_itr)) return rc;
+  if (grid_blocks <= 0) return p2c::set_error(P2C_EINVAL, "grid_blocks must be positive");
+  auto* c32 = static_cast<uint32_t*>(counters);
+  hipLaunchKernelGGL(p2c::tuv_heavy_kernel, dim3(grid_blocks), dim3(p2c::kTuvBlock), 0, p2c::as_stream(stream),
+                     static_cast<const p2c::TuvPair*>(pairs), heavy_list, c32 + 3, list_cap, c32 + 4,
+                     static_cast<const p2c::WalkState*>(states), tolerance, (uint64_t)max_itr, out, replay_list,
+                     c32 + 2);
   P2C_HIP_CHECK(hipGetLastError());
   return P2C_OK;
 }

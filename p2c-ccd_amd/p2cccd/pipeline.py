@@ -87,7 +87,7 @@ class CcdPipeline:
         self.last = counts
         return counts
 
-    # -- batched: device-side counts, persistent certificate, graph ----------
+    # -- batched: device-side counts, stream certificate, graph --------------
     def _enqueue_batched(self, events: tuple | None = None) -> None:
         bp, c = self.bp, self.cert
         bp.build(self.x0, self.x1, self.thickness)
@@ -111,7 +111,8 @@ class CcdPipeline:
 
     def step_batched_timed(self, events: tuple) -> StepCounts:
         """One batched step outside the graph, with `events` recorded around
-        the persistent certificate kernel (same kernels as the graph)."""
+        the certificate kernels (same kernels as the graph; see
+        TuvCertifier.certify)."""
         self._enqueue_batched(events)
         torch.cuda.synchronize(self.device)
         self.last = self._read_counts()

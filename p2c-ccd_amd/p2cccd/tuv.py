@@ -38,7 +38,8 @@ nat._sig("p2c_tuv_setup_indexed", [_i32, _vp, _vp, _vp, _i64, _f64, _f64, _vp, _
 nat._sig("p2c_tuv_setup_pairs", [_i32, _vp, _vp, _vp, _vp, _i64, _vp, _i64, _i64, _f64, _f64, _vp, _vp])
 nat._sig("p2c_tuv_plain", [_vp, _i64, _vp, _f64, _i64, _vp, _vp, _vp, _u32, _vp])
 nat._sig("p2c_tuv_persistent", [_vp, _i64, _vp, _f64, _i64, _i64, _vp, _vp, _vp, _vp, _u32, _i32, _vp])
-nat._sig("p2c_tuv_stream", [_vp, _i64, _vp, _f64, _i64, _i64, _i32, _vp, _vp, _vp, _vp, _vp, _u32, _i32, _vp])
+nat._sig("p2c_tuv_stream", [_vp, _i64, _vp, _f64, _i64, _i64, _i32, _i32, _vp, _vp, _vp, _vp, _u32, _i32, _vp])
+nat._sig("p2c_tuv_wave", [_vp, _vp, _vp, _vp, _u32, _f64, _i64, _vp, _vp, _i32, _vp])
 nat._sig("p2c_tuv_rounds", [_vp, _i64, _vp, _f64, _i64, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _u32, _i32, _vp])
 nat.lib.p2c_tuv_state_bytes.restype = nat.ctypes.c_size_t
 nat._sig("p2c_tuv_replay", [_vp, _vp, _vp, _u32, _f64, _i64, _vp, _i64, _i64, _vp, _vp])
@@ -56,6 +57,7 @@ class TuvConfig:
     blocks_per_cu: int = 3       # persistent grid: resident blocks of 256 per CU
     light_budget: int = 1024     # boxes a pair may take in the stream/persistent pass before the wave pass
     refill_at: int = 8           # stream: idle lanes of a wave that trigger a refill
+    handover_at: int = 8         # stream: busy lanes of a drained wave at or below which all go to the wave pass
     round_budgets: tuple = (8, 32, 128, 512)  # walk steps per pair per round before the wave pass
 
 
@@ -116,7 +118,9 @@ class TuvCertifier:
                 events: tuple | None = None) -> None:
         """Certifies the pairs set up last (n, or min(n, totals[0] + totals[1])
         when device totals are given); results land in self.results.
-        `events` (start, end) are recorded around the certificate kernel."""
+        `events` (start, end[, mid]) are recorded around the certificate
+        kernels (stream form: `mid` between the stream and the wave pass),
+        before the replay."""
         s = nat.stream_handle(stream)
         c = self.counters
         c.zero_()
@@ -134,9 +138,13 @@ class TuvCertifier:
                      self.replay_list.data_ptr(), self._cap, self.grid_blocks, s)
         elif mode == "stream":
             nat.call("p2c_tuv_stream", self.pairs.data_ptr(), n, tp, self.cfg.tolerance, self.cfg.max_itr,
-                     self.cfg.light_budget, self.cfg.refill_at, self.results.data_ptr(), c.data_ptr(),
-                     self.states.data_ptr(), self.heavy_list.data_ptr(), self.replay_list.data_ptr(), self._cap,
-                     self.grid_blocks, s)
+                     self.cfg.light_budget, self.cfg.refill_at, self.cfg.handover_at, self.results.data_ptr(),
+                     c.data_ptr(), self.states.data_ptr(), self.heavy_list.data_ptr(), self._cap, self.grid_blocks, s)
+            if events is not None and len(events) > 2:
+                events[2].record()
+            nat.call("p2c_tuv_wave", self.pairs.data_ptr(), c.data_ptr(), self.states.data_ptr(),
+                     self.heavy_list.data_ptr(), self._cap, self.cfg.tolerance, self.cfg.max_itr,
+                     self.results.data_ptr(), self.replay_list.data_ptr(), self.grid_blocks, s)
         elif mode == "persistent":
             nat.call("p2c_tuv_persistent", self.pairs.data_ptr(), n, tp, self.cfg.tolerance, self.cfg.max_itr,
                      self.cfg.light_budget, self.results.data_ptr(), c.data_ptr(), self.heavy_list.data_ptr(),

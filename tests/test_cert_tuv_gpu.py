@@ -81,19 +81,21 @@ def test_rounds_with_tiny_budgets_match_oracle(gpu, budgets):
         assert cert.heavy_count() > 0
 
 
-@pytest.mark.parametrize("refill_at", [1, 8, 64])
-@pytest.mark.parametrize("light_budget", [4, 1024])
-def test_stream_refill_and_handover_match_oracle(gpu, refill_at, light_budget):
+@pytest.mark.parametrize("refill_at,light_budget,handover_at",
+                         [(1, 4, 0), (8, 4, 8), (64, 4, 64), (1, 1024, 64), (8, 1024, 8), (64, 1024, 0)])
+def test_stream_refill_and_handover_match_oracle(gpu, refill_at, light_budget, handover_at):
     # refill_at 1 refills a lane as soon as it idles, 64 only when the whole
-    # wave is idle; light budget 4 sends most pairs, mid-walk, to the wave pass.
+    # wave is idle; light budget 4 sends most pairs, mid-walk, to the wave
+    # pass; handover_at 64 hands every walk still open when the pairs run
+    # out (some between two boxes) to the wave pass.
     qs = workloads.make_query_set(3000, 3000, seed=67)
-    cert = TuvCertifier(TuvConfig(light_budget=light_budget, refill_at=refill_at))
+    cert = TuvCertifier(TuvConfig(light_budget=light_budget, refill_at=refill_at, handover_at=handover_at))
     for is_ee, x in ((False, qs.vf), (True, qs.ee)):
         got = cert.certify_positions(x, is_ee, "stream")
         ref = oracle.tuv_cert(is_ee, x, threads=8)
-        _check(got, ref, (refill_at, light_budget, is_ee))
+        _check(got, ref, (refill_at, light_budget, handover_at, is_ee))
         assert cert.replay_count() == 0
-        if light_budget == 4:
+        if light_budget == 4 or handover_at == 64:
             assert cert.heavy_count() > 0
 
 
@@ -104,7 +106,7 @@ def test_single_lane_forms_walk_the_same_boxes(gpu):
     qs = workloads.make_query_set(4000, 4000, seed=73)
     for is_ee, x in ((False, qs.vf), (True, qs.ee)):
         plain = TuvCertifier().certify_positions(x, is_ee, "plain")
-        big = TuvConfig(light_budget=1_000_000, round_budgets=(1_000_000,))
+        big = TuvConfig(light_budget=1_000_000, round_budgets=(1_000_000,), handover_at=0)
         for mode in ("stream", "rounds"):
             c = TuvCertifier(big)
             got = c.certify_positions(x, is_ee, mode)
