@@ -79,6 +79,8 @@ int orc_tuv_eval(int is_ee, const double* v8x3 /* [8][3]: 4 at t0 then 4 at t1 *
 int orc_tuv_eval_batch(int is_ee, const double* v, int64_t n, double ms, double tolerance,
                        double t_max, int64_t max_itr, int64_t heap_cap, orc_tuv_result* out,
                        int nthreads);
+void orc_tuv_box(int is_ee, const double* v8x3, double tl, double th, double ul, double uh, double vl, double vh,
+                 double mn[3], double mx[3], double err[3]);
 void orc_tuv_params(int is_ee, const double* v8x3, double tolerance, double err_out[3],
                     double tol_out[3]);
 

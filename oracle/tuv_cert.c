@@ -53,7 +53,10 @@
 #define TUV_MAXE 52
 #define ONE63 (1ull << 63)
 
-int orc_tuv_mutation = 0; /* planted mistakes for tests: 1 drops err, 2 breadth-first cap */
+/* planted mistakes for tests/test_oracle_tuv.py: 1 drops the error bound,
+ * 2 drops the VF u + v <= 1 restriction, 3 builds the second EE edge from the
+ * wrong vertex */
+int orc_tuv_mutation = 0;
 
 typedef struct {
   double x0[4][3], d[4][3];
@@ -88,7 +91,7 @@ static void eval_box(const Tuv* q, double tl, double th, double ul, double uh, d
             f = X[0][k] - w;
           } else {
             const double pa = X[0][k] + us[iu] * (X[1][k] - X[0][k]);
-            const double pb = X[2][k] + vs[iv] * (X[3][k] - X[2][k]);
+            const double pb = X[2][k] + vs[iv] * (X[3][k] - X[orc_tuv_mutation == 3 ? 1 : 2][k]);
             f = pa - pb;
           }
           mn[k] = fmin(mn[k], f);
@@ -145,6 +148,16 @@ void orc_tuv_params(int is_ee, const double* v, double tolerance, double err_out
     err_out[k] = q.err[k];
     inv_tol_out[k] = q.inv_tol[k];
   }
+}
+
+/* corner range of F over one box, and the pair's error bound (for the
+ * enclosure test) */
+void orc_tuv_box(int is_ee, const double* v, double tl, double th, double ul, double uh, double vl, double vh,
+                 double mn[3], double mx[3], double err[3]) {
+  Tuv q;
+  setup(&q, is_ee, v, 0.0, 1e-6);
+  eval_box(&q, tl, th, ul, uh, vl, vh, mn, mx);
+  for (int k = 0; k < 3; ++k) err[k] = q.err[k];
 }
 
 /* split dimension for exponents e[3], or -1 when no dimension may split */
@@ -228,13 +241,13 @@ int orc_tuv_eval(int is_ee, const double* v, double ms, double tolerance, double
     if (b.kt >= best) break;
     if (b.kt != last_t) {
       last_t = b.kt;
-      if (orc_tuv_mutation != 2 && popped > max_itr) {
+      if (popped > max_itr) {
         capped = 1;
         cap_key = b.kt;
         break;
       }
     }
-    if (popped >= hard_cap || (orc_tuv_mutation == 2 && popped >= max_itr)) {
+    if (popped >= hard_cap) {
       capped = 1;
       cap_key = b.kt;
       break;
@@ -268,7 +281,7 @@ int orc_tuv_eval(int is_ee, const double* v, double ms, double tolerance, double
     else hi.kv += half;
     const Box ch[2] = {lo, hi};
     for (int c = 0; c < 2; ++c) {
-      if (!is_ee && ch[c].ku + ch[c].kv > ONE63) continue; /* u_lo + v_lo > 1 */
+      if (!is_ee && orc_tuv_mutation != 2 && ch[c].ku + ch[c].kv > ONE63) continue; /* u_lo + v_lo > 1 */
       if (hn >= hcap) {
         if (hcap >= heap_cap) {
           capped = 1;
