@@ -88,6 +88,29 @@ def cpu_baseline(scene, sample_pairs: int, seed: int = 7) -> dict:
     }
 
 
+def probe(pipe, steps: int) -> int:
+    """Batched steps outside the graph (one dispatch per kernel, so counters
+    attach to single launches), then a 1 GiB device-to-device copy whose
+    algorithmic bytes are known (1 GiB read + 1 GiB written)."""
+    import torch
+
+    pipe.step_batched(use_graph=False)
+    ms = []
+    for _ in range(steps):
+        ev = tuple(torch.cuda.Event(enable_timing=True) for _ in range(3))
+        pipe.step_batched_timed(ev)
+        ms.append(ev[0].elapsed_time(ev[2]))
+    a = torch.empty(1 << 28, dtype=torch.float32, device=pipe.device)
+    a.fill_(1.0)
+    c = torch.empty_like(a)
+    torch.cuda.synchronize(pipe.device)
+    c.copy_(a)
+    torch.cuda.synchronize(pipe.device)
+    print(json.dumps({"probe": "stream kernel ms per step, then a 1 GiB copy", "stream_ms": ms,
+                      "pairs": pipe.last.pairs, "copy_bytes": 2 * a.numel() * 4}), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,6 +120,9 @@ def main() -> int:
     ap.add_argument("--workload", choices=["cloth", "icospheres"], default="cloth")
     ap.add_argument("--cpu-sample", type=int, default=100_000, help="candidates certified for the CPU baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--probe", type=int, default=0,
+                    help="profiling probe: run this many batched steps outside the graph, then a 1 GiB device copy "
+                         "(the calibration kernel for HBM byte counters); prints the stream-kernel times")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -112,6 +138,9 @@ def main() -> int:
     log("building scene")
     scene = scenes.cloth_on_sphere() if args.workload == "cloth" else scenes.two_icospheres()
     pipe = CcdPipeline(scene, device=dev)
+
+    if args.probe > 0:
+        return probe(pipe, args.probe)
 
     log("warm-up")
     for _ in range(max(args.warmup, 1)):
