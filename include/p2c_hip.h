@@ -249,6 +249,44 @@ int p2c_bp_count(const p2c_bp_mesh* mesh, void* workspace, uint64_t* totals, voi
 int p2c_bp_emit(const p2c_bp_mesh* mesh, void* workspace, int32_t* vf_pairs, int64_t vf_cap, int32_t* ee_pairs,
                 int64_t ee_cap, void* stream);
 
+/* ---- scheduling scorers ---------------------------------------------------
+ * Scores only order certificate work; exact certificates decide every pair.
+ *
+ * TinySTPF candidate scores (FP64), replacing ScoreCandidate with its
+ * FeatureFromCandidateWithIndex / PredictTinyStpf and GroupKey
+ * (scene_object_envelope_native_ti_walltime.cpp:756-771, 893-1131).  Weights
+ * are packed as doubles: mean[32] std[32] w0[64][32] b0[64] w1[32][64]
+ * b1[32] w2[32] b2 (the order of LoadTinyStpfWeights, :995-1066). */
+#define P2C_TINY_STPF_WEIGHTS 4289
+#define P2C_SCORE_LEARNED 0
+#define P2C_SCORE_LEARNED_RESIDUAL 1
+#define P2C_SCORE_PROXIMITY 2
+#define P2C_SCORE_MOTION 3
+#define P2C_SCORE_RANDOM 4
+/* Per candidate c of a VF (family 1: pairs (vertex, face), topo = faces[nf][3],
+ * object_a = vertex objects, object_b = face objects) or EE list (family 2:
+ * pairs (edge, edge), topo = edges[ne][2], object_a = object_b = edge
+ * objects): scores[c] under `policy` with candidate index c, and
+ * group_keys[c] = (min object << 32) | max object.  `totals` (device [VF, EE,
+ * ...] counts, may be NULL) caps n.  scores or group_keys may be NULL. */
+int p2c_tiny_stpf_score(int family, const double* x0, const double* x1, const int32_t* topo, const int32_t* pairs,
+                        int64_t n, const uint64_t* totals, const int32_t* object_a, const int32_t* object_b,
+                        int policy, const double* weights, double* scores, uint64_t* group_keys, void* stream);
+/* The 32 features of n candidates from their rows (double[n][8][3], the 4
+ * vertices at t0 then at t1) and candidate indices; out double[n][32]. */
+int p2c_tiny_stpf_features(int family, const double* rows, const uint64_t* index, int64_t n, double* out,
+                           void* stream);
+/* PredictTinyStpf on n feature rows (double[n][32]). */
+int p2c_tiny_stpf_predict(const double* weights, const double* features, int64_t n, double* out, void* stream);
+
+/* STPF priority trunk (FP32 weights and accumulation): the trunk and priority
+ * head of STPFModel (src/python/p2cccd/proposal/stpf_model.py:45-93) at
+ * feature_dim 32, hidden 128, two GELU layers: sigmoid(wp . h2 + bp).  Weights
+ * packed as floats: w1[128][32] b1[128] w2[128][128] b2[128] wp[128] bp
+ * (torch nn.Linear's [out][in] layout).  features float[n][32], out float[n]. */
+#define P2C_STPF_WEIGHTS 20865
+int p2c_stpf_priority(const float* weights, const float* features, int64_t n, float* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -60,6 +60,14 @@ def lib() -> ctypes.CDLL:
         _lib.orc_pt_distance.restype = _f64
         _lib.orc_ee_distance.argtypes = [_vp, _f64]
         _lib.orc_ee_distance.restype = _f64
+        _lib.orc_tiny_features.argtypes = [_i32, _vp, ctypes.c_uint64, _vp]
+        _lib.orc_tiny_features.restype = None
+        _lib.orc_tiny_predict.argtypes = [_vp, _vp]
+        _lib.orc_tiny_predict.restype = _f64
+        _lib.orc_tiny_score_batch.argtypes = [_i32, _i32, _vp, _vp, _vp, _i64, _vp, _vp]
+        _lib.orc_tiny_score_batch.restype = None
+        _lib.orc_unit_hash.argtypes = [ctypes.c_uint64]
+        _lib.orc_unit_hash.restype = _f64
         if hasattr(_lib, "orc_tuv_eval_batch"):
             _lib.orc_tuv_eval_batch.argtypes = [_i32, _vp, _i64, _f64, _f64, _f64, _i64, _i64, _vp, _i32]
             _lib.orc_tuv_eval_batch.restype = _i32
@@ -228,3 +236,37 @@ def candidates(scene, method="sweep", cap=None):
         pairs = buf[:n]
         out[kind] = pairs[np.lexsort((pairs[:, 1], pairs[:, 0]))]
     return out
+
+
+# ---- TinySTPF scorer --------------------------------------------------------
+def tiny_features(is_vf: bool, rows: np.ndarray, index: np.ndarray) -> np.ndarray:
+    """32 features per candidate from rows (n, 8, 3) and candidate indices (oracle/mlp.c)."""
+    rows = np.ascontiguousarray(rows, np.float64)
+    out = np.zeros((rows.shape[0], 32))
+    for c in range(rows.shape[0]):
+        f = np.zeros(32)
+        lib().orc_tiny_features(int(is_vf), rows[c].ctypes.data, int(index[c]), f.ctypes.data)
+        out[c] = f
+    return out
+
+
+def tiny_predict(weights: np.ndarray, feats: np.ndarray) -> np.ndarray:
+    w = np.ascontiguousarray(weights, np.float64)
+    feats = np.ascontiguousarray(feats, np.float64)
+    return np.array([lib().orc_tiny_predict(w.ctypes.data, feats[c].ctypes.data) for c in range(feats.shape[0])])
+
+
+def tiny_scores(policy: int, is_vf: bool, rows: np.ndarray, index: np.ndarray, keys: np.ndarray,
+                weights: np.ndarray | None) -> np.ndarray:
+    rows = np.ascontiguousarray(rows, np.float64)
+    index = np.ascontiguousarray(index, np.uint64)
+    keys = np.ascontiguousarray(keys, np.uint64)
+    w = np.ascontiguousarray(weights if weights is not None else np.zeros(4289), np.float64)
+    out = np.zeros(rows.shape[0])
+    lib().orc_tiny_score_batch(policy, int(is_vf), rows.ctypes.data, index.ctypes.data, keys.ctypes.data,
+                               rows.shape[0], w.ctypes.data, out.ctypes.data)
+    return out
+
+
+def set_mlp_mutation(k: int) -> None:
+    ctypes.c_int.in_dll(lib(), "orc_mlp_mutation").value = k

@@ -99,12 +99,15 @@ int64_t orc_brute_vf(const double* vbox, const int32_t* vobj, int64_t nv, const 
 int64_t orc_brute_ee(const double* ebox, const int32_t* edges, const int32_t* eobj, int64_t ne,
                      const uint8_t* allowed, int32_t nobj, int32_t* out, int64_t cap);
 
-/* scorers */
-double orc_tiny_stpf(const double* mean, const double* std, const double* w0, const double* b0,
-                     const double* w1, const double* b1, const double* w2, double b2,
-                     const double* feature);
-void orc_mlp_score(const float* feat, int64_t n, const float* w0, const float* b0,
-                   const float* w1, const float* b1, const float* w2, float b2, float* out);
+/* scorers (mlp.c) */
+uint64_t orc_mix64(uint64_t x);
+double orc_unit_hash(uint64_t x);
+void orc_tiny_features(int is_vf, const double* rows24, uint64_t index, double f[32]);
+double orc_tiny_predict(const double* packed_weights, const double f[32]);
+double orc_tiny_score(int policy, int is_vf, const double* rows24, uint64_t index, uint64_t group_key,
+                      const double* packed_weights);
+void orc_tiny_score_batch(int policy, int is_vf, const double* rows, const uint64_t* index, const uint64_t* keys,
+                          int64_t n, const double* packed_weights, double* out);
 
 #ifdef __cplusplus
 }
