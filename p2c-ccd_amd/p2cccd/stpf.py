@@ -145,6 +145,9 @@ def save_stpf(path: str | Path, model: StpfPriority, meta: dict | None = None) -
     np.savez(path, packed=model.packed().cpu().numpy(), **{k: np.asarray(v) for k, v in (meta or {}).items()})
 
 
+DEFAULT_PRIORITY_WEIGHTS = Path(__file__).resolve().parent / "data" / "stpf_priority.npz"
+
+
 def load_stpf(path: str | Path, device) -> torch.Tensor:
     with np.load(path, allow_pickle=False) as z:
         packed = torch.from_numpy(np.ascontiguousarray(z["packed"], dtype=np.float32))
