@@ -120,6 +120,7 @@ def main() -> int:
     ap.add_argument("--workload", choices=["cloth", "icospheres"], default="cloth")
     ap.add_argument("--cpu-sample", type=int, default=100_000, help="candidates certified for the CPU baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default certificate parameters")
     ap.add_argument("--probe", type=int, default=0,
                     help="profiling probe: run this many batched steps outside the graph, then a 1 GiB device copy "
                          "(the calibration kernel for HBM byte counters); prints the stream-kernel times")
@@ -146,6 +147,13 @@ def main() -> int:
     for _ in range(max(args.warmup, 1)):
         pipe.step_batched()
     torch.cuda.synchronize(dev)
+    tuned = None
+    if not args.no_autotune:
+        log("autotune (stream-kernel launch and queue parameters, exact answers required)")
+        tuned = pipe.autotune()
+        log(f"autotune: {tuned}")
+        pipe.step_batched()  # recapture the graph with the tuned parameters
+        torch.cuda.synchronize(dev)
 
     # timed region: K graph-replayed batched steps, each synchronised
     lat = []
@@ -220,6 +228,7 @@ def main() -> int:
             "ee_pairs": pipe.last.n_ee,
             "certificate": "tuv inclusion, tolerance 1e-6, ms 0, t_max 1, max_itr 1e6",
             "execution": "batched: device counts + stream certificate + wave pass + HIP graph replay",
+            "autotuned": tuned,
         },
         "p50_ms": percentile(lat, 50),
         "p99_ms": percentile(lat, 99),

@@ -138,6 +138,24 @@ class CcdPipeline:
                 return counts
         raise RuntimeError("candidate counts kept outgrowing the pair buffers")
 
+    # -- tuning --------------------------------------------------------------
+    def autotune(self, space: dict | None = None, cache=None) -> dict:
+        """Picks the certificate's launch and queue parameters on this scene's
+        current pairs (p2cccd.autotune; every admitted configuration returns
+        bit-identical results) and drops the captured graph."""
+        from .autotune import CertificateTuner, summarize
+
+        if self.vf is None:
+            self.step_batched(use_graph=False)
+        self._enqueue_batched()
+        torch.cuda.synchronize(self.device)
+        self.last = self._read_counts()
+        tuner = CertificateTuner(self.cert, self.last.pairs, cache=cache)
+        cfg, trials = tuner.tune(space)
+        self.graph = None
+        return {"refill_at": cfg.refill_at, "light_budget": cfg.light_budget, "handover_at": cfg.handover_at,
+                "blocks_per_cu": cfg.blocks_per_cu, **summarize(trials)}
+
     # -- results ------------------------------------------------------------
     def results(self) -> dict:
         """Host copies of the last step: candidate pairs and per-pair results
