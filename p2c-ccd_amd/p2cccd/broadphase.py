@@ -57,6 +57,10 @@ class BroadPhase:
         self.workspace = torch.empty(self.ws_bytes, dtype=torch.uint8, device=dev)
         self.totals = torch.zeros(3, dtype=torch.int64, device=dev)
 
+    @staticmethod
+    def workspace_bytes(n_vertices: int, n_faces: int, n_edges: int, n_objects: int) -> int:
+        return int(nat.lib.p2c_bp_workspace_bytes(n_vertices, n_faces, n_edges, n_objects))
+
     @classmethod
     def for_scene(cls, scene: Scene, device="cuda") -> "BroadPhase":
         return cls(scene.topo, scene.n_vertices, device)

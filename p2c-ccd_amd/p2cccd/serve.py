@@ -1,0 +1,428 @@
+"""Serving: per-timestep CCD requests against many resident scenes.
+
+Each scene keeps its topology, LBVH workspace, positions and candidate
+buffers in HBM.  A request carries one timestep's positions (x0, x1) for one
+scene.  The server runs a continuous-batching loop over two lanes (a HIP
+stream and a certificate workspace each):
+
+  * admission: a queued request joins the next batch when its scene is
+    resident or can be made resident within the HBM budget (least recently
+    used idle scenes are evicted), its scene has no request in flight, and
+    the batch has request and pair slots left;
+  * a batch runs every request's broad phase on its lane, reads all candidate
+    counts with one wait on that lane, then sets up every request's pairs into
+    one record buffer and certifies them in one shared launch (stream kernel
+    + wave pass + replay), so requests of mixed size share launches;
+  * while one lane's batch is on the GPU, the host admits new arrivals into
+    the other lane's batch, whose broad phase overlaps the first batch's
+    certificate on the device.
+
+Per-pair answers do not depend on what else shares a batch: candidate order
+is fixed per scene and the certificate's result per pair is order-free, so a
+request's result is bit-identical alone and in the mix (tests check it).
+
+The workload of the serving benchmark: resident
+scenes, each a pair of jittered icospheres or grid meshes with log-uniform
+triangle counts; Poisson arrivals; each request one step of random rigid
+motion plus per-vertex N(0, 1e-3) noise.
+"""
+from __future__ import annotations
+
+import collections
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import scenes
+from .broadphase import BroadPhase
+from .tuv import TUV_RESULT_DTYPE, TuvCertifier, TuvConfig
+
+
+@dataclass
+class Request:
+    rid: int
+    scene_id: int
+    x0: np.ndarray
+    x1: np.ndarray
+    arrival: float = 0.0
+
+
+@dataclass
+class Completed:
+    rid: int
+    scene_id: int
+    arrival: float
+    done: float
+    vf: np.ndarray
+    ee: np.ndarray
+    hit: np.ndarray      # uint32 per pair, VF then EE
+    toi: np.ndarray      # float64 per pair
+
+    @property
+    def latency_ms(self) -> float:
+        return (self.done - self.arrival) * 1e3
+
+    @property
+    def pairs(self) -> int:
+        return len(self.hit)
+
+
+# ---- admission (host logic only) ------------------------------------------------
+class Residency:
+    """Which scenes hold HBM, under a byte budget, with LRU eviction of scenes
+    that have no request in flight."""
+
+    def __init__(self, budget: int):
+        self.budget = budget
+        self.bytes: dict[int, int] = {}
+        self.lru: collections.OrderedDict[int, None] = collections.OrderedDict()
+        self.busy: set[int] = set()
+        self.evictions = 0
+
+    @property
+    def used(self) -> int:
+        return sum(self.bytes.values())
+
+    def touch(self, sid: int) -> None:
+        self.lru.pop(sid, None)
+        self.lru[sid] = None
+
+    def plan(self, sid: int, need: int) -> list[int] | None:
+        """Scenes to evict so that `sid` (needing `need` bytes if not resident)
+        fits; None if it cannot fit now."""
+        if sid in self.bytes:
+            return []
+        if need > self.budget:
+            raise RuntimeError(f"scene {sid} needs {need} bytes, more than the HBM budget {self.budget}")
+        free = self.budget - self.used
+        out = []
+        for victim in self.lru:
+            if free >= need:
+                break
+            if victim in self.busy or victim == sid:
+                continue
+            out.append(victim)
+            free += self.bytes[victim]
+        return out if free >= need else None
+
+    def add(self, sid: int, nbytes: int) -> None:
+        self.bytes[sid] = nbytes
+        self.touch(sid)
+
+    def remove(self, sid: int) -> None:
+        self.bytes.pop(sid, None)
+        self.lru.pop(sid, None)
+        self.evictions += 1
+
+
+# ---- device state -------------------------------------------------------------
+class ResidentScene:
+    def __init__(self, scene: scenes.Scene, device, headroom: float = 1.25):
+        self.scene = scene
+        self.device = device
+        self.bp = BroadPhase.for_scene(scene, device)
+        self.x0 = torch.empty((scene.n_vertices, 3), dtype=torch.float64, device=device)
+        self.x1 = torch.empty_like(self.x0)
+        self.headroom = headroom
+        self.cap_vf = self.cap_ee = 0
+        self.vf = self.ee = None
+
+    def ensure(self, n_vf: int, n_ee: int) -> None:
+        if self.vf is not None and n_vf <= self.cap_vf and n_ee <= self.cap_ee:
+            return
+        self.cap_vf = max(int(n_vf * self.headroom) + 256, self.cap_vf)
+        self.cap_ee = max(int(n_ee * self.headroom) + 256, self.cap_ee)
+        self.vf = torch.empty((self.cap_vf, 2), dtype=torch.int32, device=self.device)
+        self.ee = torch.empty((self.cap_ee, 2), dtype=torch.int32, device=self.device)
+
+    @staticmethod
+    def estimate_bytes(scene: scenes.Scene, pairs_guess: int = 0) -> int:
+        nv, nf, ne = scene.n_vertices, len(scene.topo.faces), len(scene.topo.edges)
+        ws = BroadPhase.workspace_bytes(nv, nf, ne, scene.topo.object_count)
+        return ws + 2 * nv * 24 + (nf * 12 + ne * 8) + 8 * int(pairs_guess * 1.25)
+
+    def nbytes(self) -> int:
+        return self.bp.ws_bytes + 2 * self.x0.numel() * 8 + 8 * (self.cap_vf + self.cap_ee) + \
+            self.bp.faces.numel() * 4 + self.bp.edges.numel() * 4
+
+
+@dataclass
+class _Batch:
+    reqs: list
+    offsets: list = field(default_factory=list)
+    counts: list = field(default_factory=list)
+    total: int = 0
+    event: torch.cuda.Event | None = None
+    host_res: torch.Tensor | None = None
+    host_pairs: list = field(default_factory=list)
+
+
+class Lane:
+    def __init__(self, device, cfg: TuvConfig | None):
+        self.stream = torch.cuda.Stream(device)
+        self.cert = TuvCertifier(cfg, device)
+        self.batch: _Batch | None = None
+
+
+class Server:
+    def __init__(self, scene_table: dict[int, scenes.Scene], hbm_budget: int, device="cuda",
+                 max_batch_requests: int = 64, max_batch_pairs: int = 1 << 23, cfg: TuvConfig | None = None,
+                 lanes: int = 2):
+        self.table = scene_table
+        self.device = torch.device(device)
+        self.res = Residency(hbm_budget)
+        self.resident: dict[int, ResidentScene] = {}
+        self.last_pairs: dict[int, int] = {}
+        self.max_batch_requests = max_batch_requests
+        self.max_batch_pairs = max_batch_pairs
+        self.lanes = [Lane(self.device, cfg) for _ in range(lanes)]
+        self.batches = 0
+        self.loads = 0
+
+    # -- residency ---------------------------------------------------------
+    def _make_resident(self, sid: int) -> bool:
+        if sid in self.resident:
+            self.res.touch(sid)
+            return True
+        need = ResidentScene.estimate_bytes(self.table[sid], self.last_pairs.get(sid, 0))
+        plan = self.res.plan(sid, need)
+        if plan is None:
+            return False
+        for victim in plan:
+            del self.resident[victim]
+            self.res.remove(victim)
+        self.resident[sid] = ResidentScene(self.table[sid], self.device)
+        self.res.add(sid, need)
+        self.loads += 1
+        return True
+
+    # -- batches -----------------------------------------------------------
+    def _admit(self, queue: collections.deque) -> list:
+        batch, pairs, skipped = [], 0, collections.deque()
+        busy_now = set()
+        while queue and len(batch) < self.max_batch_requests:
+            r = queue.popleft()
+            # pairs of the scene's last step, or 16 per triangle before its first
+            est = self.last_pairs.get(r.scene_id, 16 * len(self.table[r.scene_id].topo.faces))
+            if r.scene_id in self.res.busy or r.scene_id in busy_now or (batch and pairs + est > self.max_batch_pairs):
+                skipped.append(r)
+                continue
+            if not self._make_resident(r.scene_id):
+                skipped.append(r)
+                continue
+            batch.append(r)
+            busy_now.add(r.scene_id)
+            self.res.busy.add(r.scene_id)
+            pairs += est
+        queue.extendleft(reversed(skipped))
+        return batch
+
+    def _launch(self, lane: Lane, reqs: list) -> None:
+        b = _Batch(reqs=reqs)
+        with torch.cuda.stream(lane.stream):
+            for r in reqs:
+                rs = self.resident[r.scene_id]
+                rs.x0.copy_(torch.from_numpy(r.x0), non_blocking=False)
+                rs.x1.copy_(torch.from_numpy(r.x1), non_blocking=False)
+                rs.bp.build(rs.x0, rs.x1, rs.scene.thickness, stream=lane.stream)
+                rs.bp.count(stream=lane.stream)
+            totals = torch.stack([self.resident[r.scene_id].bp.totals for r in reqs]).to("cpu", non_blocking=True)
+            lane.stream.synchronize()  # the counts size this batch; waits on this lane only
+            off = 0
+            for r, t in zip(reqs, totals.tolist()):
+                n_vf, n_ee, bad = t
+                if bad:
+                    raise RuntimeError(f"{bad} BVH traversals exceeded their step bound (scene {r.scene_id})")
+                rs = self.resident[r.scene_id]
+                rs.ensure(n_vf, n_ee)
+                self.res.bytes[r.scene_id] = rs.nbytes()  # the estimate, corrected by the real footprint
+                rs.bp.emit(rs.vf, rs.ee, stream=lane.stream)
+                b.offsets.append(off)
+                b.counts.append((n_vf, n_ee))
+                off += n_vf + n_ee
+                self.last_pairs[r.scene_id] = n_vf + n_ee
+            b.total = off
+            cert = lane.cert
+            cert.reserve(max(off, 1))
+            for r, o, (n_vf, n_ee) in zip(reqs, b.offsets, b.counts):
+                rs = self.resident[r.scene_id]
+                if n_vf:
+                    cert.setup_pairs(rs.x0, rs.x1, rs.bp.faces, rs.vf, False, n_vf, out_offset=o, stream=lane.stream)
+                if n_ee:
+                    cert.setup_pairs(rs.x0, rs.x1, rs.bp.edges, rs.ee, True, n_ee, out_offset=o + n_vf,
+                                     stream=lane.stream)
+            if off:
+                cert.certify(off, "stream", stream=lane.stream)
+            b.host_res = cert.result_tensor(max(off, 1))[: off * TUV_RESULT_DTYPE.itemsize].to("cpu", non_blocking=True)
+            for r, (n_vf, n_ee) in zip(reqs, b.counts):
+                rs = self.resident[r.scene_id]
+                b.host_pairs.append((rs.vf[:n_vf].to("cpu", non_blocking=True), rs.ee[:n_ee].to("cpu", non_blocking=True)))
+            b.event = torch.cuda.Event()
+            b.event.record(lane.stream)
+        lane.batch = b
+        self.batches += 1
+
+    def _complete(self, lane: Lane, out: list) -> None:
+        b = lane.batch
+        b.event.synchronize()
+        now = time.perf_counter()
+        res = b.host_res.numpy().view(TUV_RESULT_DTYPE)
+        for r, o, (n_vf, n_ee), (vf, ee) in zip(b.reqs, b.offsets, b.counts, b.host_pairs):
+            seg = res[o:o + n_vf + n_ee]
+            out.append(Completed(r.rid, r.scene_id, r.arrival, now, vf.numpy(), ee.numpy(), seg["hit"].copy(),
+                                 seg["toi"].copy()))
+            self.res.busy.discard(r.scene_id)
+        lane.batch = None
+
+    def run(self, requests: list, realtime: bool = True) -> list:
+        """Serves `requests` (arrival offsets in seconds from the start when
+        realtime, otherwise all queued at once); returns completions with
+        arrival/done on the same clock (perf_counter)."""
+        pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
+        queue: collections.deque = collections.deque()
+        out: list = []
+        t0 = time.perf_counter()
+        launched: list = []  # busy lanes, oldest first
+        while pending or queue or launched:
+            now = time.perf_counter() - t0
+            while pending and (not realtime or pending[0].arrival <= now):
+                r = pending.popleft()
+                if not realtime:
+                    r.arrival = now
+                queue.append(r)
+            for lane in list(launched):  # retire finished batches
+                if lane.batch.event.query():
+                    self._complete(lane, out)
+                    launched.remove(lane)
+            free = [lane for lane in self.lanes if lane.batch is None]
+            if free and queue:
+                reqs = self._admit(queue)
+                if reqs:
+                    self._launch(free[0], reqs)
+                    launched.append(free[0])
+                    continue
+            if launched and (queue or not pending):
+                # nothing admissible right now: wait for the oldest batch
+                self._complete(launched[0], out)
+                launched.pop(0)
+            elif pending:
+                time.sleep(max(0.0, min(pending[0].arrival - (time.perf_counter() - t0), 0.001)))
+            elif queue:
+                raise RuntimeError("a queued request can never be admitted (HBM budget too small?)")
+        for c in out:
+            c.arrival += t0
+        return out
+
+
+# ---- the serving workload -------------------------------------------------------
+def _grid_mesh(n: int, size: float):
+    """n x n vertex grid in the z = 0 plane, 2 (n-1)^2 triangles."""
+    u = np.linspace(-size / 2, size / 2, n)
+    X, Y = np.meshgrid(u, u, indexing="ij")
+    v = np.stack([X.ravel(), Y.ravel(), np.zeros(n * n)], 1)
+    i = np.arange(n - 1)
+    a = (i[:, None] * n + i[None, :]).ravel()
+    f = np.concatenate([np.stack([a, a + n, a + 1], 1), np.stack([a + 1, a + n, a + n + 1], 1)])
+    return v, f.astype(np.int32)
+
+
+def make_scene(sid: int, triangles: int, seed: int) -> scenes.Scene:
+    """Two jittered meshes of about `triangles` triangles in total, 0.02
+    apart: icospheres (for about half of the seeds, when a subdivision level
+    matches the size to within 2x) or square grids."""
+    rng = np.random.default_rng(seed)
+    per = max(triangles // 2, 20)
+    level = int(round(np.log(per / 20.0) / np.log(4.0)))
+    if rng.random() < 0.5 and level >= 1 and 0.5 <= 20 * 4 ** level / per <= 2.0:
+        v, f = scenes.icosphere(level)
+        v2 = v.copy() + np.array([2.02, 0.0, 0.0])
+        kind = "spheres"
+    else:
+        n = max(int(np.sqrt(per / 2.0)) + 1, 3)
+        v, f = _grid_mesh(n, 2.0)
+        v2 = v.copy() + np.array([0.0, 0.0, 0.02])
+        kind = "grids"
+    v = v + rng.normal(0, 1e-3, v.shape)
+    v2 = v2 + rng.normal(0, 1e-3, v2.shape)
+    verts = np.concatenate([v, v2])
+    faces = np.concatenate([f, f + len(v)]).astype(np.int32)
+    topo = scenes.prepare_topology(faces, len(verts))
+    return scenes.Scene(name=f"serve{sid}_{kind}_{len(faces)}", x0=verts, x1=verts.copy(), topo=topo,
+                        thickness=0.0, info={"kind": kind, "triangles": int(len(faces))})
+
+
+def step_request(rid: int, sid: int, scene: scenes.Scene, seed: int, arrival: float = 0.0) -> Request:
+    """One timestep: the second mesh moves rigidly toward the first (random
+    rotation up to 1 degree, translation 0.015 against a 0.02 gap), every
+    vertex adds N(0, 1e-3)."""
+    rng = np.random.default_rng(seed)
+    x0 = scene.x0.copy()
+    x1 = x0.copy()
+    half = len(x0) // 2
+    ang = np.deg2rad(rng.uniform(0, 1))
+    axis = rng.normal(size=3)
+    axis /= np.linalg.norm(axis)
+    K = np.array([[0, -axis[2], axis[1]], [axis[2], 0, -axis[0]], [-axis[1], axis[0], 0]])
+    R = np.eye(3) + np.sin(ang) * K + (1 - np.cos(ang)) * (K @ K)
+    c = x0[half:].mean(0)
+    direction = x0[:half].mean(0) - c
+    direction /= np.linalg.norm(direction) + 1e-300
+    x1[half:] = (x0[half:] - c) @ R.T + c + 0.015 * direction
+    x1 += rng.normal(0, 1e-3, x1.shape)
+    return Request(rid, sid, np.ascontiguousarray(x0), np.ascontiguousarray(x1), arrival)
+
+
+def poisson_workload(n_scenes: int, n_requests: int, rate_hz: float, min_tri: int = 1000, max_tri: int = 500_000,
+                     seed: int = 5):
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(min_tri), np.log(max_tri), n_scenes)).astype(int)
+    table = {i: make_scene(i, int(sizes[i]), seed + 7 * i) for i in range(n_scenes)}
+    arrivals = np.cumsum(rng.exponential(1.0 / rate_hz, n_requests))
+    sids = rng.integers(0, n_scenes, n_requests)
+    reqs = [step_request(k, int(s), table[int(s)], seed + 1000 + k, float(a)) for k, (s, a) in
+            enumerate(zip(sids, arrivals))]
+    return table, reqs
+
+
+def report(done: list, wall: float) -> dict:
+    lat = np.array([c.latency_ms for c in done])
+    pairs = int(sum(c.pairs for c in done))
+    return {"requests": len(done), "pairs": pairs, "pairs_per_s": pairs / wall, "p50_ms": float(np.percentile(lat, 50)),
+            "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean())}
+
+
+def main() -> int:
+    import argparse
+    import json
+
+    ap = argparse.ArgumentParser(description="serving benchmark: Poisson requests over resident scenes")
+    ap.add_argument("--scenes", type=int, default=256)
+    ap.add_argument("--requests", type=int, default=2000)
+    ap.add_argument("--rate", type=float, default=400.0, help="arrivals per second")
+    ap.add_argument("--max-triangles", type=int, default=500_000)
+    ap.add_argument("--budget-gb", type=float, default=64.0)
+    args = ap.parse_args()
+    t = time.perf_counter()
+    table, reqs = poisson_workload(args.scenes, args.requests, args.rate, max_tri=args.max_triangles)
+    print(f"[serve] built {len(table)} scenes in {time.perf_counter() - t:.1f} s", flush=True)
+    srv = Server(table, int(args.budget_gb * (1 << 30)))
+    # make every scene resident and size its buffers with one step each
+    warm = [step_request(-1 - i, sid, table[sid], 99 + i) for i, sid in enumerate(sorted(table))]
+    srv.run(warm, realtime=False)
+    loads = srv.loads
+    print(f"[serve] {loads} scenes resident ({srv.res.used / 2**30:.2f} GiB)", flush=True)
+    t = time.perf_counter()
+    done = srv.run(reqs, realtime=True)
+    wall = time.perf_counter() - t
+    print(f"[serve] served {len(done)} requests in {wall:.1f} s", flush=True)
+    rep = report(done, wall)
+    rep.update(scenes=args.scenes, rate_hz=args.rate, batches=srv.batches, scene_loads_during_run=srv.loads - loads,
+               evictions=srv.res.evictions, budget_gb=args.budget_gb, wall_s=wall)
+    print(json.dumps(rep), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())
