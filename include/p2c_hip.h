@@ -249,6 +249,43 @@ int p2c_bp_count(const p2c_bp_mesh* mesh, void* workspace, uint64_t* totals, voi
 int p2c_bp_emit(const p2c_bp_mesh* mesh, void* workspace, int32_t* vf_pairs, int64_t vf_cap, int32_t* ee_pairs,
                 int64_t ee_cap, void* stream);
 
+/* ---- proxy-scene raw candidates --------------------------------------------
+ * Replaces GenerateRawCandidatesCpu (src/cpp/rt_candidate/candidate_buffer.cpp:
+ * 144-181) and the OptiX trace it stands beside, with the CPU path's FP64
+ * semantics.  The set holds device arrays of n proxy primitives (proxy_id =
+ * index, ProxyPrimitive of src/cpp/rt_candidate/proxy_scene.h:29-41). */
+typedef struct {
+  const uint32_t* object_id;
+  const uint32_t* patch_id;
+  const uint32_t* slab_id;
+  const uint32_t* proxy_type; /* 1 swept AABB, 2 capsule */
+  const double* t0;
+  const double* t1;
+  const double* bounds; /* [n][6]: min xyz then max xyz */
+  const double* motion; /* [n][4]: translation_bound, rotation_angle, radial_motion_bound, conservative_radius */
+  int64_t n;
+  uint64_t query_id;
+} p2c_proxy_set;
+
+/* RawCandidateHit (candidate_buffer.h:19-31), 64 bytes. */
+#define P2C_RAW_VALID 1u
+#define P2C_RAW_AABB_OVERLAP 2u
+typedef struct {
+  uint64_t query_id;
+  uint64_t pair_key;
+  uint32_t proxy_a_index, proxy_b_index, slab_id, rt_hit_count, flags, reserved0;
+  float motion_bound[4];
+  uint32_t reserved1, reserved2;
+} p2c_raw_hit;
+
+/* Raw hits per primitive i (pairs i < j), into counts[n] (device). */
+int p2c_proxy_raw_count(const p2c_proxy_set* set, uint32_t* counts, void* stream);
+/* Raw hits in the reference's (i, j) order: primitive i's at offsets[i]
+ * (device int64, the exclusive scan of the counts); hits at or beyond `cap`
+ * are dropped. */
+int p2c_proxy_raw_fill(const p2c_proxy_set* set, const int64_t* offsets, p2c_raw_hit* out, int64_t cap,
+                       void* stream);
+
 /* ---- scheduling scorers ---------------------------------------------------
  * Scores only order certificate work; exact certificates decide every pair.
  *

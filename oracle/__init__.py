@@ -66,6 +66,7 @@ def lib() -> ctypes.CDLL:
         _lib.orc_tiny_predict.restype = _f64
         _lib.orc_tiny_score_batch.argtypes = [_i32, _i32, _vp, _vp, _vp, _i64, _vp, _vp]
         _lib.orc_tiny_score_batch.restype = None
+        _lib.orc_proxy_generate.restype = _i32
         _lib.orc_unit_hash.argtypes = [ctypes.c_uint64]
         _lib.orc_unit_hash.restype = _f64
         if hasattr(_lib, "orc_tuv_eval_batch"):
@@ -270,3 +271,93 @@ def tiny_scores(policy: int, is_vf: bool, rows: np.ndarray, index: np.ndarray, k
 
 def set_mlp_mutation(k: int) -> None:
     ctypes.c_int.in_dll(lib(), "orc_mlp_mutation").value = k
+
+
+# ---- proxy-scene candidates ---------------------------------------------------
+RAW_HIT = np.dtype([("query_id", "<u8"), ("pair_key", "<u8"), ("proxy_a_index", "<u4"), ("proxy_b_index", "<u4"),
+                    ("slab_id", "<u4"), ("rt_hit_count", "<u4"), ("flags", "<u4"), ("reserved0", "<u4"),
+                    ("motion_bound", "<f4", (4,)), ("reserved1", "<u4"), ("reserved2", "<u4")])
+CANDIDATE = np.dtype([("candidate_id", "<u8"), ("query_id", "<u8"), ("slab_id", "<u4"), ("object_a_id", "<u4"),
+                      ("patch_a_id", "<u4"), ("object_b_id", "<u4"), ("patch_b_id", "<u4"), ("proxy_type_a", "<u4"),
+                      ("proxy_type_b", "<u4"), ("rt_hit_count", "<u4"), ("motion_bound", "<f4", (4,)),
+                      ("flags", "<u4"), ("reserved", "<u4")])
+DENSITY = np.dtype([("proxy_count", "<u8"), ("object_count", "<u8"), ("slab_count", "<u8"),
+                    ("cross_object_same_slab_pair_count", "<u8"), ("raw_hit_count", "<u8"),
+                    ("compact_candidate_count", "<u8"), ("raw_hits_per_proxy", "<f8"), ("candidates_per_proxy", "<f8"),
+                    ("candidates_per_slab", "<f8"), ("aabb_overlap_ratio", "<f8"),
+                    ("avg_rt_hits_per_candidate", "<f8")])
+assert RAW_HIT.itemsize == 64 and CANDIDATE.itemsize == 72
+
+
+class _ProxySet(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_void_p) for name in ("object_id", "patch_id", "slab_id", "proxy_type", "t0", "t1",
+                                                      "bounds", "motion")] + [("n", ctypes.c_int64)]
+
+
+PROXY_FIELDS = ("object_id", "patch_id", "slab_id", "proxy_type", "t0", "t1", "bounds", "motion")
+
+
+def _proxy_arrays(scene: dict):
+    arr = {k: np.ascontiguousarray(scene[k], dtype=np.uint32) for k in PROXY_FIELDS[:4]}
+    arr.update({k: np.ascontiguousarray(scene[k], dtype=np.float64) for k in PROXY_FIELDS[4:]})
+    s = _ProxySet(*[arr[k].ctypes.data for k in PROXY_FIELDS], len(arr["t0"]))
+    return s, arr
+
+
+def _proxy_run(fn, query_id: int, scene: dict, with_err: bool):
+    s, keep = _proxy_arrays(scene)
+    n = len(keep["t0"])
+    cap = max(n * (n - 1) // 2, 1)
+    raw = np.zeros(cap, RAW_HIT)
+    cand = np.zeros(cap, CANDIDATE)
+    dens = np.zeros(1, DENSITY)
+    nr, nc = ctypes.c_int64(), ctypes.c_int64()
+    vp = ctypes.c_void_p
+    args = [ctypes.c_uint64(query_id), ctypes.byref(s), ctypes.c_int64(cap), vp(raw.ctypes.data), ctypes.byref(nr),
+            ctypes.c_int64(cap), vp(cand.ctypes.data), ctypes.byref(nc), vp(dens.ctypes.data)]
+    if with_err:
+        err = ctypes.create_string_buffer(256)
+        rc = fn(*args, err, 256)
+        if rc != 0:
+            raise ValueError(err.value.decode())
+    else:
+        rc = fn(*args)
+        if rc != 0:
+            raise ValueError(f"oracle rejected the proxy scene (code {rc})")
+    return raw[: nr.value], cand[: nc.value], dens[0]
+
+
+def proxy_generate(query_id: int, scene: dict):
+    """Raw hits, compact candidates and density of a proxy scene given as arrays (oracle/proxy.c)."""
+    return _proxy_run(lib().orc_proxy_generate, query_id, scene, False)
+
+
+def ref_proxy_generate(query_id: int, scene: dict):
+    """The same through the reference's compiled candidate_buffer.cpp (oracle/_ref)."""
+    return _proxy_run(ref_lib().ref_proxy_generate, query_id, scene, True)
+
+
+def ref_proxy_kat_scene() -> tuple[int, dict]:
+    L = ref_lib()
+    cap = 16
+    out = {k: np.zeros(cap, np.uint32) for k in PROXY_FIELDS[:4]}
+    out.update(t0=np.zeros(cap), t1=np.zeros(cap), bounds=np.zeros((cap, 6)), motion=np.zeros((cap, 4)))
+    n, q = ctypes.c_int64(), ctypes.c_uint64()
+    rc = L.ref_proxy_kat_scene(ctypes.c_int64(cap), *[ctypes.c_void_p(out[k].ctypes.data) for k in PROXY_FIELDS],
+                               ctypes.byref(n), ctypes.byref(q))
+    if rc != 0:
+        raise ValueError("reference BuildProxyScene failed")
+    return int(q.value), {k: v[: n.value] for k, v in out.items()}
+
+
+def random_proxy_scene(rng, n: int, n_objects: int, n_slabs: int, extent: float = 1.0, size: float = 0.15) -> dict:
+    """A seeded proxy scene: boxes in [-extent, extent]^3, objects, patches, and
+    slabs that partition [0, 1] (each slab one global interval)."""
+    lo = rng.uniform(-extent, extent, (n, 3))
+    hi = lo + rng.uniform(0.0, size, (n, 3))
+    slab = rng.integers(0, n_slabs, n).astype(np.uint32)
+    edges = np.linspace(0.0, 1.0, n_slabs + 1)
+    return dict(object_id=rng.integers(1, n_objects + 1, n).astype(np.uint32),
+                patch_id=rng.integers(0, 6, n).astype(np.uint32), slab_id=slab,
+                proxy_type=rng.integers(1, 3, n).astype(np.uint32), t0=edges[slab], t1=edges[slab + 1],
+                bounds=np.concatenate([lo, hi], 1), motion=rng.uniform(0.0, 0.1, (n, 4)))

@@ -99,6 +99,44 @@ int64_t orc_brute_vf(const double* vbox, const int32_t* vobj, int64_t nv, const 
 int64_t orc_brute_ee(const double* ebox, const int32_t* edges, const int32_t* eobj, int64_t ne,
                      const uint8_t* allowed, int32_t nobj, int32_t* out, int64_t cap);
 
+/* proxy-scene candidates (proxy.c; reference shim ref_proxy_shim.cpp) */
+typedef struct {
+  const uint32_t* object_id;
+  const uint32_t* patch_id;
+  const uint32_t* slab_id;
+  const uint32_t* proxy_type; /* 1 swept AABB, 2 capsule */
+  const double* t0;
+  const double* t1;
+  const double* bounds; /* [n][6]: min xyz, max xyz */
+  const double* motion; /* [n][4]: translation, rotation angle, radial motion, conservative radius */
+  int64_t n;
+} orc_proxy_set;
+
+typedef struct { /* RawCandidateHit, 64 bytes */
+  uint64_t query_id, pair_key;
+  uint32_t proxy_a_index, proxy_b_index, slab_id, rt_hit_count, flags, reserved0;
+  float motion_bound[4];
+  uint32_t reserved1, reserved2;
+} orc_raw_hit;
+
+typedef struct { /* CandidateRecord fields */
+  uint64_t candidate_id, query_id;
+  uint32_t slab_id, object_a_id, patch_a_id, object_b_id, patch_b_id, proxy_type_a, proxy_type_b, rt_hit_count;
+  float motion_bound[4];
+  uint32_t flags, reserved;
+} orc_candidate;
+
+typedef struct {
+  uint64_t proxy_count, object_count, slab_count, cross_object_same_slab_pair_count, raw_hit_count,
+      compact_candidate_count;
+  double raw_hits_per_proxy, candidates_per_proxy, candidates_per_slab, aabb_overlap_ratio,
+      avg_rt_hits_per_candidate;
+} orc_density;
+
+int orc_proxy_generate(uint64_t query_id, const orc_proxy_set* set, int64_t raw_cap, orc_raw_hit* raw,
+                       int64_t* n_raw, int64_t cand_cap, orc_candidate* cand, int64_t* n_cand,
+                       orc_density* density);
+
 /* scorers (mlp.c) */
 uint64_t orc_mix64(uint64_t x);
 double orc_unit_hash(uint64_t x);
