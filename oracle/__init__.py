@@ -361,3 +361,65 @@ def random_proxy_scene(rng, n: int, n_objects: int, n_slabs: int, extent: float 
                 patch_id=rng.integers(0, 6, n).astype(np.uint32), slab_id=slab,
                 proxy_type=rng.integers(1, 3, n).astype(np.uint32), t0=edges[slab], t1=edges[slab + 1],
                 bounds=np.concatenate([lo, hi], 1), motion=rng.uniform(0.0, 0.1, (n, 4)))
+
+
+# ---- proposal policy (compiled reference only) --------------------------------
+def ref_schedule_runtime(cands, rows, outs, masks: dict, cfg: dict):
+    """ScheduleRuntimeExactWorkItemsFromProposals of the compiled reference.
+    Returns (u64[n, 8], f64[n, 2], f32[n], stats u64[10]) or raises ValueError."""
+    L = ref_lib()
+    vp = ctypes.c_void_p
+    n = len(cands)
+    c = lambda a, t: np.ascontiguousarray(a, dtype=t)  # noqa: E731
+    keep = [c(cands["candidate_id"], np.uint64), c(cands["query_id"], np.uint64), c(cands["slab_id"], np.uint32),
+            c(cands["object_a_id"], np.uint32), c(cands["patch_a_id"], np.uint32), c(cands["object_b_id"], np.uint32),
+            c(cands["patch_b_id"], np.uint32), c(cands["proxy_type_a"], np.uint32), c(cands["proxy_type_b"], np.uint32),
+            c(cands["rt_hit_count"], np.uint32), c(cands["motion_bound"], np.float32)]
+    rk = [c(rows["candidate_id"], np.uint64), c(rows["features"], np.float32)]
+    ok = [c(outs["candidate_id"], np.uint64), c(outs["interval_scores"], np.float32),
+          c(outs["family_scores"], np.float32), c(outs["priority_score"], np.float32),
+          c(outs["cost_score"], np.float32), c(outs["uncertainty_score"], np.float32)]
+    mq = c(list(masks.keys()), np.uint64)
+    mv = c(list(masks.values()), np.uint32)
+    cd = c([cfg["fallback_interval_t0"], cfg["fallback_interval_t1"]], np.float64)
+    cf = c([cfg["family_score_threshold"], cfg["uncertainty_fallback_threshold"], cfg["ood_abs_feature_threshold"]],
+           np.float32)
+    cu = c([cfg["first_work_item_id"], cfg["conservative_feature_family_mask"], int(cfg["preserve_candidate_order"])],
+           np.uint64)
+    wu = np.zeros((max(n, 1), 8), np.uint64)
+    wi = np.zeros((max(n, 1), 2))
+    wp = np.zeros(max(n, 1), np.float32)
+    st = np.zeros(10, np.uint64)
+    err = ctypes.create_string_buffer(256)
+    rc = L.ref_schedule_runtime(ctypes.c_int64(n), *[vp(a.ctypes.data) for a in keep], ctypes.c_int64(len(rows)),
+                                *[vp(a.ctypes.data) for a in rk], ctypes.c_int64(len(outs)),
+                                *[vp(a.ctypes.data) for a in ok], ctypes.c_int64(len(mq)), vp(mq.ctypes.data),
+                                vp(mv.ctypes.data), vp(cd.ctypes.data), vp(cf.ctypes.data), vp(cu.ctypes.data),
+                                vp(wu.ctypes.data), vp(wi.ctypes.data), vp(wp.ctypes.data), vp(st.ctypes.data), err,
+                                256)
+    if rc != 0:
+        raise ValueError(err.value.decode())
+    return wu[:n], wi[:n], wp[:n], st
+
+
+def ref_dense_group(inputs, cfg: dict):
+    L = ref_lib()
+    vp = ctypes.c_void_p
+    n = len(inputs)
+    c = lambda a, t: np.ascontiguousarray(a, dtype=t)  # noqa: E731
+    d6 = c(np.stack([inputs[f] for f in ("full_exact_cost", "narrow_exact_cost", "interval_t0", "interval_t1",
+                                         "contact_t0", "contact_t1")], 1), np.float64)
+    f3 = c(np.stack([inputs[f] for f in ("priority_score", "cost_score", "uncertainty_score")], 1), np.float32)
+    q, ci, col = c(inputs["query_id"], np.uint64), c(inputs["candidate_id"], np.uint64), \
+        c(inputs["candidate_collides"], np.uint8)
+    cf = c([cfg["uncertainty_fallback_threshold"]], np.float32)
+    cu = c([cfg["representative_attempt_limit"], int(cfg["preserve_input_order"])], np.uint64)
+    su, sd = np.zeros(14, np.uint64), np.zeros(4)
+    err = ctypes.create_string_buffer(256)
+    rc = L.ref_dense_group(ctypes.c_int64(n), vp(q.ctypes.data), vp(ci.ctypes.data), vp(d6.ctypes.data),
+                           vp(f3.ctypes.data), vp(col.ctypes.data), vp(cf.ctypes.data), vp(cu.ctypes.data),
+                           ctypes.c_double(cfg["interval_miss_penalty_scale"]), vp(su.ctypes.data), vp(sd.ctypes.data),
+                           err, 256)
+    if rc != 0:
+        raise ValueError(err.value.decode())
+    return su, sd
