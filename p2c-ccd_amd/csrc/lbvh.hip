@@ -83,7 +83,7 @@ __device__ inline bool overlap(const Box6& a, const double* mn, const double* mx
 // ---- workspace ----------------------------------------------------------
 struct Layout {
   size_t vbox, fbox, ebox, objkey, allowed, keys0, vals0, keys1, vals1, hist, fnodes, fparent, fflag, frcs,
-      fhi, enodes, eparent, eflag, ercs, ehi, qorder, eorder, counts, offsets, hist_off, scan_part, err, total;
+      fhi, flo, enodes, eparent, eflag, ercs, ehi, elo, qorder, eorder, counts, offsets, hist_off, scan_part, err, total;
   int64_t sort_cap, hist_blocks;
 };
 
@@ -115,11 +115,13 @@ Layout make_layout(int64_t nv, int64_t nf, int64_t ne, int32_t nobj) {
   L.fflag = take(4 * std::max<int64_t>(nf, 1));
   L.frcs = take(4 * std::max<int64_t>(nf, 1));
   L.fhi = take(4 * std::max<int64_t>(nf, 1));
+  L.flo = take(4 * std::max<int64_t>(nf, 1));
   L.enodes = take(sizeof(Node) * std::max<int64_t>(2 * ne - 1, 1));
   L.eparent = take(4 * std::max<int64_t>(2 * ne - 1, 1));
   L.eflag = take(4 * std::max<int64_t>(ne, 1));
   L.ercs = take(4 * std::max<int64_t>(ne, 1));
   L.ehi = take(4 * std::max<int64_t>(ne, 1));
+  L.elo = take(4 * std::max<int64_t>(ne, 1));
   L.qorder = take(4 * std::max<int64_t>(nv, 1));
   L.eorder = take(4 * std::max<int64_t>(ne, 1));
   L.counts = take(4 * (nv + ne + 1));
@@ -434,7 +436,8 @@ __device__ inline int delta(const uint32_t* keys, int64_t n, int64_t i, int64_t 
 
 __global__ __launch_bounds__(kBlock) void karras_kernel(const uint32_t* __restrict__ keys, int64_t n,
                                                         Node* __restrict__ nodes, int32_t* __restrict__ parent,
-                                                        int32_t* __restrict__ rcs, int32_t* __restrict__ hi_of) {
+                                                        int32_t* __restrict__ rcs, int32_t* __restrict__ hi_of,
+                                                        int32_t* __restrict__ lo_of) {
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n - 1; i += int64_t(gridDim.x) * blockDim.x) {
     const int d = delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1) >= 0 ? 1 : -1;
     const int dmin = delta(keys, n, i, i - d);
@@ -461,6 +464,7 @@ __global__ __launch_bounds__(kBlock) void karras_kernel(const uint32_t* __restri
     parent[right] = int32_t(i);
     rcs[gamma] = right;
     hi_of[i] = int32_t(hi);
+    lo_of[i] = int32_t(lo);
   }
 }
 
@@ -493,27 +497,40 @@ __global__ __launch_bounds__(kBlock) void leaves_kernel(const uint32_t* __restri
   }
 }
 
+// Bottom-up refit, one leaf per lane.  An internal node's leaf range holds
+// its own index (Karras), so a node whose range lies inside the block's 256
+// leaves is counted in an LDS flag with workgroup scope; only nodes spanning
+// blocks use the global flag with agent scope, whose release also publishes
+// the boxes the climbing lane built below it.  The first child to arrive
+// stops; the second sees both children's boxes and builds the parent's.
 __global__ __launch_bounds__(kBlock) void refit_kernel(int64_t n, Node* __restrict__ nodes,
                                                        const int32_t* __restrict__ parent,
+                                                       const int32_t* __restrict__ lo_of,
+                                                       const int32_t* __restrict__ hi_of,
                                                        uint32_t* __restrict__ flag) {
-  for (int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; q < n; q += int64_t(gridDim.x) * blockDim.x) {
-    int32_t node = parent[n - 1 + q];
-    while (node >= 0) {
-      // The first child to arrive stops; the second sees both children's
-      // boxes (release by the first, acquire here) and builds the parent's.
-      const uint32_t prev =
-          __hip_atomic_fetch_add(&flag[node], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-      if (prev == 0) break;
-      const Node& a = nodes[nodes[node].left];
-      const Node& b = nodes[nodes[node].right];
-      for (int k = 0; k < 3; ++k) {
-        const double amn = a.mn[k], bmn = b.mn[k];
-        const double amx = a.mx[k], bmx = b.mx[k];
-        nodes[node].mn[k] = bmn < amn ? bmn : amn;
-        nodes[node].mx[k] = amx < bmx ? bmx : amx;
-      }
-      node = parent[node];
+  __shared__ uint32_t local_flag[kBlock];
+  local_flag[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t b0 = int64_t(blockIdx.x) * kBlock;
+  const int64_t b1 = (b0 + kBlock < n ? b0 + kBlock : n) - 1;  // last leaf of the block
+  const int64_t q = b0 + threadIdx.x;
+  if (q >= n) return;
+  int32_t node = parent[n - 1 + q];
+  while (node >= 0) {
+    const bool local = lo_of[node] >= b0 && hi_of[node] <= b1;
+    const uint32_t prev =
+        local ? __hip_atomic_fetch_add(&local_flag[node - b0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP)
+              : __hip_atomic_fetch_add(&flag[node], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (prev == 0) break;
+    const Node& a = nodes[nodes[node].left];
+    const Node& b = nodes[nodes[node].right];
+    for (int k = 0; k < 3; ++k) {
+      const double amn = a.mn[k], bmn = b.mn[k];
+      const double amx = a.mx[k], bmx = b.mx[k];
+      nodes[node].mn[k] = bmn < amn ? bmn : amn;
+      nodes[node].mx[k] = amx < bmx ? bmx : amx;
     }
+    node = parent[node];
   }
 }
 
@@ -732,9 +749,9 @@ int p2c_bp_build(const p2c_bp_mesh* m, const double* x0, const double* x1, doubl
     int kind;
     const int32_t* idx;
     int64_t n;
-    size_t box, nodes, parent, flag, rcs, hi;
-  } trees[2] = {{0, m->faces, nf, L.fbox, L.fnodes, L.fparent, L.fflag, L.frcs, L.fhi},
-                {1, m->edges, ne, L.ebox, L.enodes, L.eparent, L.eflag, L.ercs, L.ehi}};
+    size_t box, nodes, parent, flag, rcs, hi, lo;
+  } trees[2] = {{0, m->faces, nf, L.fbox, L.fnodes, L.fparent, L.fflag, L.frcs, L.fhi, L.flo},
+                {1, m->edges, ne, L.ebox, L.enodes, L.eparent, L.eflag, L.ercs, L.ehi, L.elo}};
   for (const Tree& t : trees) {
     if (t.n == 0) continue;
     hipLaunchKernelGGL(prim_box_kernel, dim3(grid_for(t.n, kBlock)), dim3(kBlock), 0, s, t.kind, x0, x1, t.idx, t.n,
@@ -745,11 +762,15 @@ int p2c_bp_build(const p2c_bp_mesh* m, const double* x0, const double* x1, doubl
     uint32_t* flag = at<uint32_t>(ws, t.flag);
     if (t.n > 1)
       hipLaunchKernelGGL(karras_kernel, dim3(grid_for(t.n - 1, kBlock)), dim3(kBlock), 0, s, k0, t.n, nodes, parent,
-                         at<int32_t>(ws, t.rcs), at<int32_t>(ws, t.hi));
+                         at<int32_t>(ws, t.rcs), at<int32_t>(ws, t.hi), at<int32_t>(ws, t.lo));
     hipLaunchKernelGGL(leaves_kernel, dim3(grid_for(2 * t.n - 1, kBlock)), dim3(kBlock), 0, s, v0, t.n,
                        at<Box6>(ws, t.box), nodes, at<int32_t>(ws, t.rcs), at<int32_t>(ws, t.hi), parent, flag);
-    if (t.n > 1)
-      hipLaunchKernelGGL(refit_kernel, dim3(grid_for(t.n, kBlock)), dim3(kBlock), 0, s, t.n, nodes, parent, flag);
+    if (t.n > 1) {
+      // one block per 256 leaves, exactly (the LDS flags cover the block's leaves)
+      if ((t.n + kBlock - 1) / kBlock > (int64_t(1) << 20)) return set_error(P2C_ECAPACITY, "too many primitives");
+      hipLaunchKernelGGL(refit_kernel, dim3(unsigned((t.n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, t.n, nodes,
+                         parent, at<int32_t>(ws, t.lo), at<int32_t>(ws, t.hi), flag);
+    }
     // the edge leaf order (sorted edge ids) is also the EE query order
     if (t.kind == 1)
       P2C_HIP_CHECK(hipMemcpyAsync(at<uint32_t>(ws, L.eorder), v0, sizeof(uint32_t) * t.n, hipMemcpyDeviceToDevice, s));

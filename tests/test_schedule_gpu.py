@@ -13,7 +13,7 @@ import torch
 import oracle
 from p2cccd import scenes, schedule, stpf
 from p2cccd.broadphase import BroadPhase
-from p2cccd.tuv import TuvCertifier
+from p2cccd.tuv import TUV_RESULT_DTYPE, TuvCertifier
 
 pytestmark = pytest.mark.gpu
 
@@ -107,3 +107,32 @@ def test_spatial_cells_and_group_priority(gpu):
     c = cell.cpu().numpy()
     for i, k in enumerate(g.keys.cpu().numpy()):
         assert (r.first_hit[i] >= 0) == bool(hits[c == k].any())
+
+
+def test_dense_group_early_stop_on_gpu_certificate_outcomes(gpu):
+    # RunDenseGroupExactEarlyStop driven by the GPU certificate's own outcomes
+    # (contact flag, TOI, boxes as cost) instead of cost traces; with the
+    # proposed interval the whole step, every contact lies inside it, so the
+    # groups holding a contact are exactly the detected ones.
+    from p2cccd import policy
+
+    s, x0, x1, vf, ee = _scene_candidates()
+    t = s.topo
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+    cert = TuvCertifier()
+    pairs, topo = ee, dev(t.edges)
+    n = pairs.shape[0]
+    cert.reserve(n)
+    cert.setup_pairs(x0, x1, topo, pairs, True, n)
+    cert.certify(n, "stream")
+    torch.cuda.synchronize()
+    res = cert.result_tensor(n).cpu().numpy().view(TUV_RESULT_DTYPE)
+    scores, keys = stpf.tiny_scores("ee", x0, x1, topo, pairs, n, dev(t.edge_object), dev(t.edge_object), "proximity")
+    cell = schedule.spatial_cells(schedule.candidate_rows(x0, x1, topo, pairs, "ee"))
+    d = policy.dense_group_from_certificates(cell.cpu().numpy() + 1, res["hit"], res["toi"], res["boxes"],
+                                             scores.cpu().numpy().astype(np.float32))
+    st = policy.run_dense_group_exact_early_stop(d)
+    positive = len(np.unique(d["query_id"][d["candidate_collides"]]))
+    assert st["positive_group_count"] == positive > 0
+    assert st["tp"] == positive and st["fn"] == 0 and st["fp"] == 0
+    assert st["learned_exact_calls"] < st["no_proposal_exact_calls"]
