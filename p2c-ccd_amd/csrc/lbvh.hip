@@ -20,8 +20,12 @@
 //   4. Karras build (distinct 64-bit keys: code << 32 | position) with
 //      escape indices for stackless traversal; bottom-up FP64 refit with
 //      acquire/release counters
-//   5. stackless traversal, one lane per query in Morton order: count,
-//      exclusive scan, fill (deterministic output order)
+//   5. stackless traversal, one lane per query in Morton order: count
+//      (keeping each query's first kHitCache partners), exclusive scan,
+//      fill (deterministic output order): queries with at most kHitCache
+//      hits are copied from the cache, only the others traverse again.
+//      An edge query visits only the leaves after its own, so each
+//      unordered EE pair is found once.
 #include <cmath>
 
 #include "p2c_hip.h"
@@ -34,6 +38,7 @@ constexpr int kBlock = 256;
 constexpr int kMaxObjects = 64;
 constexpr int kSortItems = 16;  // keys per lane per radix tile
 constexpr int kSortTile = kBlock * kSortItems;
+constexpr int kHitCache = 64;  // partners kept per query by the count pass
 
 struct Box6 {
   double mn[3];
@@ -83,7 +88,7 @@ __device__ inline bool overlap(const Box6& a, const double* mn, const double* mx
 // ---- workspace ----------------------------------------------------------
 struct Layout {
   size_t vbox, fbox, ebox, objkey, allowed, keys0, vals0, keys1, vals1, hist, fnodes, fparent, fflag, frcs,
-      fhi, flo, enodes, eparent, eflag, ercs, ehi, elo, qorder, eorder, counts, offsets, hist_off, scan_part, err, total;
+      fhi, flo, enodes, eparent, eflag, ercs, ehi, elo, qorder, eorder, counts, cache, offsets, hist_off, scan_part, err, total;
   int64_t sort_cap, hist_blocks;
 };
 
@@ -125,6 +130,7 @@ Layout make_layout(int64_t nv, int64_t nf, int64_t ne, int32_t nobj) {
   L.qorder = take(4 * std::max<int64_t>(nv, 1));
   L.eorder = take(4 * std::max<int64_t>(ne, 1));
   L.counts = take(4 * (nv + ne + 1));
+  L.cache = take(4 * size_t(kHitCache) * size_t(nv + ne));
   L.offsets = take(8 * (nv + ne + 1));
   L.hist_off = take(8 * 256 * L.hist_blocks);
   const int64_t scan_items = std::max<int64_t>(nv + ne + 1, 256 * L.hist_blocks);
@@ -560,10 +566,10 @@ __device__ inline bool pair_allowed(const uint8_t* allowed, int32_t nobj, int32_
 // A pre-order walk visits each of the 2n-1 nodes at most once; `max_steps`
 // bounds the loop regardless, and an exhausted bound is counted in *err.
 template <typename Accept, typename Emit>
-__device__ inline uint32_t traverse(const Node* __restrict__ nodes, const Box6& q, int64_t max_steps,
+__device__ inline uint32_t traverse(const Node* __restrict__ nodes, int32_t start, const Box6& q, int64_t max_steps,
                                     uint32_t* err, Accept accept, Emit emit) {
   uint32_t count = 0;
-  int32_t node = 0;
+  int32_t node = start;
   int64_t steps = 0;
   while (node >= 0) {
     if (++steps > max_steps) {
@@ -596,9 +602,11 @@ __global__ __launch_bounds__(kBlock) void vf_query_kernel(const Node* __restrict
                                                           const Box6* __restrict__ vbox,
                                                           const uint32_t* __restrict__ qorder, int64_t nv, VfRule r,
                                                           uint32_t* __restrict__ err, uint32_t* __restrict__ counts,
+                                                          int32_t* __restrict__ cache,
                                                           const uint64_t* __restrict__ offsets,
                                                           int32_t* __restrict__ out, int64_t cap) {
   for (int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; s < nv; s += int64_t(gridDim.x) * blockDim.x) {
+    if (kFill && offsets[s + 1] - offsets[s] <= uint64_t(kHitCache)) continue;  // copied from the cache
     const int32_t v = int32_t(qorder[s]);
     const int32_t ov = r.vobj[v];
     uint32_t c = 0;
@@ -606,7 +614,7 @@ __global__ __launch_bounds__(kBlock) void vf_query_kernel(const Node* __restrict
       const Box6 q = vbox[v];
       const uint64_t base = kFill ? offsets[s] : 0;
       c = traverse(
-          fnodes, q, 2 * nf, err,
+          fnodes, 0, q, 2 * nf, err,
           [&](int32_t f) {
             const int32_t of = r.fobj[f];
             if (!pair_allowed(r.allowed, r.nobj, ov, of)) return false;
@@ -617,6 +625,7 @@ __global__ __launch_bounds__(kBlock) void vf_query_kernel(const Node* __restrict
             return true;
           },
           [&](int32_t f, uint32_t k) {
+            if (!kFill && k < uint32_t(kHitCache)) cache[s * kHitCache + k] = f;
             if (kFill && base + k < uint64_t(cap)) {
               out[(base + k) * 2] = v;
               out[(base + k) * 2 + 1] = f;
@@ -631,19 +640,22 @@ template <bool kFill>
 __global__ __launch_bounds__(kBlock) void ee_query_kernel(const Node* __restrict__ enodes, int64_t ne,
                                                           const Box6* __restrict__ ebox, const uint32_t* __restrict__ eorder,
                                                           EeRule r, uint32_t* __restrict__ err,
-                                                          uint32_t* __restrict__ counts,
+                                                          uint32_t* __restrict__ counts, int32_t* __restrict__ cache,
                                                           const uint64_t* __restrict__ offsets, int64_t nv,
                                                           int32_t* __restrict__ out, int64_t cap) {
   for (int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; s < ne; s += int64_t(gridDim.x) * blockDim.x) {
+    if (kFill && offsets[nv + s + 1] - offsets[nv + s] <= uint64_t(kHitCache)) continue;  // copied from the cache
     const int32_t i = int32_t(eorder[s]);
     const Box6 q = ebox[i];
     const int32_t oi = r.eobj[i];
     const int32_t a0 = r.edges[int64_t(i) * 2], a1 = r.edges[int64_t(i) * 2 + 1];
     const uint64_t base = kFill ? offsets[nv + s] - offsets[nv] : 0;
+    // Each unordered pair once: only the leaves after this edge's own leaf s
+    // (in tree order) are visited -- in the escape-linked pre-order those are
+    // exactly the nodes after leaf s -- and the pair is emitted as (min, max).
     const uint32_t c = traverse(
-        enodes, q, 2 * ne, err,
+        enodes, enodes[ne - 1 + s].escape, q, 2 * ne, err,
         [&](int32_t j) {
-          if (i >= j) return false;
           const int32_t oj = r.eobj[j];
           if (!pair_allowed(r.allowed, r.nobj, oi, oj)) return false;
           if (oi == oj) {
@@ -653,12 +665,44 @@ __global__ __launch_bounds__(kBlock) void ee_query_kernel(const Node* __restrict
           return true;
         },
         [&](int32_t j, uint32_t k) {
+          if (!kFill && k < uint32_t(kHitCache)) cache[s * kHitCache + k] = j;
           if (kFill && base + k < uint64_t(cap)) {
-            out[(base + k) * 2] = i;
-            out[(base + k) * 2 + 1] = j;
+            out[(base + k) * 2] = i < j ? i : j;
+            out[(base + k) * 2 + 1] = i < j ? j : i;
           }
         });
     if (!kFill) counts[s] = c;
+  }
+}
+
+// Fill for queries with at most kHitCache hits: one lane per cache slot
+// (query-major, so a wave reads whole cache lines), the pair written at the
+// query's scanned offset.  Vertex queries are s < nv, edge queries follow.
+__global__ __launch_bounds__(kBlock) void cache_copy_kernel(const int32_t* __restrict__ cache,
+                                                            const uint64_t* __restrict__ offsets,
+                                                            const uint32_t* __restrict__ qorder,
+                                                            const uint32_t* __restrict__ eorder, int64_t nv, int64_t ne,
+                                                            int32_t* __restrict__ vf, int64_t vf_cap,
+                                                            int32_t* __restrict__ ee, int64_t ee_cap) {
+  const int64_t slots = (nv + ne) * kHitCache;
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < slots; t += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t s = t / kHitCache;
+    const uint32_t k = uint32_t(t % kHitCache);
+    const uint64_t c = offsets[s + 1] - offsets[s];
+    if (c > uint64_t(kHitCache) || k >= c) continue;
+    const int32_t p = cache[t];
+    if (s < nv) {
+      const uint64_t o = offsets[s] + k;
+      if (vf == nullptr || o >= uint64_t(vf_cap)) continue;
+      vf[o * 2] = int32_t(qorder[s]);
+      vf[o * 2 + 1] = p;
+    } else {
+      const uint64_t o = offsets[s] - offsets[nv] + k;
+      if (ee == nullptr || o >= uint64_t(ee_cap)) continue;
+      const int32_t i = int32_t(eorder[s - nv]);
+      ee[o * 2] = i < p ? i : p;
+      ee[o * 2 + 1] = i < p ? p : i;
+    }
   }
 }
 
@@ -799,11 +843,12 @@ int p2c_bp_count(const p2c_bp_mesh* m, void* ws, uint64_t* totals, void* stream)
   if (nv > 0)
     hipLaunchKernelGGL(vf_query_kernel<false>, dim3(grid_for(nv, kBlock)), dim3(kBlock), 0, s,
                        at<Node>(ws, L.fnodes), nf, at<Box6>(ws, L.vbox), at<uint32_t>(ws, L.qorder), nv, vr,
-                       at<uint32_t>(ws, L.err), counts, nullptr, nullptr, 0);
+                       at<uint32_t>(ws, L.err), counts, at<int32_t>(ws, L.cache), nullptr, nullptr, 0);
   if (ne > 0)
     hipLaunchKernelGGL(ee_query_kernel<false>, dim3(grid_for(ne, kBlock)), dim3(kBlock), 0, s,
                        at<Node>(ws, L.enodes), ne, at<Box6>(ws, L.ebox), at<uint32_t>(ws, L.eorder), er,
-                       at<uint32_t>(ws, L.err), counts + nv, nullptr, nv, nullptr, 0);
+                       at<uint32_t>(ws, L.err), counts + nv, at<int32_t>(ws, L.cache) + nv * kHitCache, nullptr, nv,
+                       nullptr, 0);
   const int64_t n = nv + ne;
   const int64_t sb = (n + 1023) / 1024;
   uint64_t* part = at<uint64_t>(ws, L.scan_part);
@@ -830,11 +875,15 @@ int p2c_bp_emit(const p2c_bp_mesh* m, void* ws, int32_t* vf_pairs, int64_t vf_ca
   if (nv > 0 && vf_pairs != nullptr)
     hipLaunchKernelGGL(vf_query_kernel<true>, dim3(grid_for(nv, kBlock)), dim3(kBlock), 0, s,
                        at<Node>(ws, L.fnodes), nf, at<Box6>(ws, L.vbox), at<uint32_t>(ws, L.qorder), nv, vr,
-                       at<uint32_t>(ws, L.err), nullptr, offsets, vf_pairs, vf_cap);
+                       at<uint32_t>(ws, L.err), nullptr, nullptr, offsets, vf_pairs, vf_cap);
   if (ne > 0 && ee_pairs != nullptr)
     hipLaunchKernelGGL(ee_query_kernel<true>, dim3(grid_for(ne, kBlock)), dim3(kBlock), 0, s,
                        at<Node>(ws, L.enodes), ne, at<Box6>(ws, L.ebox), at<uint32_t>(ws, L.eorder), er,
-                       at<uint32_t>(ws, L.err), nullptr, offsets, nv, ee_pairs, ee_cap);
+                       at<uint32_t>(ws, L.err), nullptr, nullptr, offsets, nv, ee_pairs, ee_cap);
+  if (nv + ne > 0 && (vf_pairs != nullptr || ee_pairs != nullptr))
+    hipLaunchKernelGGL(cache_copy_kernel, dim3(grid_for((nv + ne) * kHitCache, kBlock, 1 << 16)), dim3(kBlock), 0, s,
+                       at<int32_t>(ws, L.cache), offsets, at<uint32_t>(ws, L.qorder), at<uint32_t>(ws, L.eorder), nv, ne,
+                       vf_pairs, vf_cap, ee_pairs, ee_cap);
   P2C_HIP_CHECK(hipGetLastError());
   return P2C_OK;
 }
