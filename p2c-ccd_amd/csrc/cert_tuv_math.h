@@ -40,11 +40,20 @@ struct TuvPair {
   uint32_t is_ee;
 };
 
-P2C_TUV_HD double key_to_f64(uint64_t k) { return (double)k * 0x1p-63; }
+// k * 2^-63 for a box corner key: a multiple of 2^11 below 2^63 (depth <= 52
+// per dimension), so k >> 11 < 2^52 sits exactly in the mantissa of 2^52
+// and the conversion is exact (the same value as (double)k * 0x1p-63).
+P2C_TUV_HD double key_to_f64(uint64_t k) {
+  return (__builtin_bit_cast(double, 0x4330000000000000ull | (k >> 11)) - 0x1p52) * 0x1p-52;
+}
 
 // F at the 8 corners of box [tl,th]x[ul,uh]x[vl,vh]; per-coordinate min/max.
-P2C_TUV_HD void tuv_corner_range(const TuvPair& q, double tl, double th, double ul, double uh, double vl,
-                                 double vh, double mn[3], double mx[3]) {
+// The pair kind is a template parameter so the six (coordinate, time)
+// blocks are straight-line code the compiler can interleave; a run-time
+// branch inside each block would serialise them.
+template <bool kEE>
+P2C_TUV_HD void tuv_corner_range_of(const TuvPair& q, double tl, double th, double ul, double uh, double vl,
+                                    double vh, double mn[3], double mx[3]) {
   const double ts[2] = {tl, th};
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
@@ -56,7 +65,7 @@ P2C_TUV_HD void tuv_corner_range(const TuvPair& q, double tl, double th, double 
       const double X2 = q.x0[2][k] + ts[it] * q.d[2][k];
       const double X3 = q.x0[3][k] + ts[it] * q.d[3][k];
       double f[4];
-      if (!q.is_ee) {
+      if (!kEE) {
         const double e1 = X2 - X1, e2 = X3 - X1;
         const double w1l = X1 + ul * e1, w1h = X1 + uh * e1;
         const double s2l = vl * e2, s2h = vh * e2;
@@ -82,6 +91,14 @@ P2C_TUV_HD void tuv_corner_range(const TuvPair& q, double tl, double th, double 
     mn[k] = lo;
     mx[k] = hi;
   }
+}
+
+P2C_TUV_HD void tuv_corner_range(const TuvPair& q, double tl, double th, double ul, double uh, double vl,
+                                 double vh, double mn[3], double mx[3]) {
+  if (q.is_ee)
+    tuv_corner_range_of<true>(q, tl, th, ul, uh, vl, vh, mn, mx);
+  else
+    tuv_corner_range_of<false>(q, tl, th, ul, uh, vl, vh, mn, mx);
 }
 
 // Which dimension a box with exponents e splits along, or 3 when none may.
@@ -116,11 +133,75 @@ P2C_TUV_HD void put3(T v[3], int d, T x) {
 // Split dimension of `level`.  The word is chosen with constant indices so
 // the pair record stays in registers (a dynamic index would move it to
 // scratch memory).
+// Levels below 64 (the first two words) take one select; deeper levels,
+// which few walks reach, a branch the wave skips when no lane needs it.
 P2C_TUV_HD int tuv_level_dim(const TuvPair& q, uint32_t level) {
   const uint32_t w = level >> 5;
-  const uint64_t word = w == 0 ? q.split[0] : w == 1 ? q.split[1] : w == 2 ? q.split[2] : w == 3 ? q.split[3]
-                                                                                               : q.split[4];
+  uint64_t word = w == 0 ? q.split[0] : q.split[1];
+  if (w >= 2) word = w == 2 ? q.split[2] : (w == 3 ? q.split[3] : q.split[4]);
   return (int)((word >> ((level & 31u) * 2u)) & 3ull);
+}
+
+// The split sequence of tuv_split_dim, 2 bits per level (3 = no split),
+// and its length.  Each level splits the dimension with the largest
+// r_k = 2^-e_k * inv_tol_k above 1 (e_k < kTuvMaxExp; the first such k on
+// ties).  r_k is carried as a running value halved at each split of k: the
+// same double as the ldexp product while it is normal, and once below 1 it
+// never splits again, so the sequence equals the level-by-level
+// tuv_split_dim replay.  Levels are packed into one register word and a
+// full word is placed once per 32 levels.
+P2C_TUV_HD void tuv_split_sequence(const double inv_tol[3], uint64_t split[5], uint32_t* done_level) {
+  uint64_t w0 = ~0ull, w1 = ~0ull, w2 = ~0ull, w3 = ~0ull, w4 = ~0ull;
+  double r0 = inv_tol[0], r1 = inv_tol[1], r2 = inv_tol[2];
+  int e0 = 0, e1 = 0, e2 = 0;
+  uint64_t cur = ~0ull;
+  uint32_t level = 0;
+  for (;;) {
+    int d = 3;
+    double br = 1.0;
+    if (e0 < kTuvMaxExp && r0 > br) {
+      br = r0;
+      d = 0;
+    }
+    if (e1 < kTuvMaxExp && r1 > br) {
+      br = r1;
+      d = 1;
+    }
+    if (e2 < kTuvMaxExp && r2 > br) d = 2;
+    if (d == 3) break;
+    r0 = d == 0 ? r0 * 0.5 : r0;
+    r1 = d == 1 ? r1 * 0.5 : r1;
+    r2 = d == 2 ? r2 * 0.5 : r2;
+    e0 += d == 0;
+    e1 += d == 1;
+    e2 += d == 2;
+    const uint32_t sh = (level & 31u) * 2u;
+    cur = (cur & ~(3ull << sh)) | ((uint64_t)d << sh);
+    ++level;
+    if ((level & 31u) == 0) {
+      const uint32_t wi = (level >> 5) - 1;
+      w0 = wi == 0 ? cur : w0;
+      w1 = wi == 1 ? cur : w1;
+      w2 = wi == 2 ? cur : w2;
+      w3 = wi == 3 ? cur : w3;
+      w4 = wi == 4 ? cur : w4;
+      cur = ~0ull;
+    }
+  }
+  if ((level & 31u) != 0) {
+    const uint32_t wi = level >> 5;
+    w0 = wi == 0 ? cur : w0;
+    w1 = wi == 1 ? cur : w1;
+    w2 = wi == 2 ? cur : w2;
+    w3 = wi == 3 ? cur : w3;
+    w4 = wi == 4 ? cur : w4;
+  }
+  split[0] = w0;
+  split[1] = w1;
+  split[2] = w2;
+  split[3] = w3;
+  split[4] = w4;
+  *done_level = level;
 }
 
 // Per-pair constants from the 8 positions (rows 0-3 at t=0, 4-7 at t=1).
@@ -172,29 +253,7 @@ P2C_TUV_HD void tuv_setup(TuvPair* q, const double v[8][3], int is_ee, double ms
         L[2] = dv > L[2] ? dv : L[2];
       }
   for (int k = 0; k < 3; ++k) q->inv_tol[k] = (3.0 * L[k]) / tolerance;
-  // split sequence, 2 bits per level (3 = no split), built in registers
-  uint64_t w0 = ~0ull, w1 = ~0ull, w2 = ~0ull, w3 = ~0ull, w4 = ~0ull;
-  int e[3] = {0, 0, 0};
-  uint32_t level = 0;
-  for (;;) {
-    const int d = tuv_split_dim(*q, e);
-    if (d == 3) break;
-    const uint32_t wi = level >> 5, sh = (level & 31u) * 2u;
-    const uint64_t clear = ~(3ull << sh), set = (uint64_t)d << sh;
-    w0 = wi == 0 ? ((w0 & clear) | set) : w0;
-    w1 = wi == 1 ? ((w1 & clear) | set) : w1;
-    w2 = wi == 2 ? ((w2 & clear) | set) : w2;
-    w3 = wi == 3 ? ((w3 & clear) | set) : w3;
-    w4 = wi == 4 ? ((w4 & clear) | set) : w4;
-    put3(e, d, get3(e, d) + 1);
-    ++level;
-  }
-  q->split[0] = w0;
-  q->split[1] = w1;
-  q->split[2] = w2;
-  q->split[3] = w3;
-  q->split[4] = w4;
-  q->done_level = level;
+  tuv_split_sequence(q->inv_tol, q->split, &q->done_level);
 }
 
 // Box outcome: 0 = no zero inside, 1 = done (certifies a hit at its t_lo),
