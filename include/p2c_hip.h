@@ -286,6 +286,52 @@ int p2c_proxy_raw_count(const p2c_proxy_set* set, uint32_t* counts, void* stream
 int p2c_proxy_raw_fill(const p2c_proxy_set* set, const int64_t* offsets, p2c_raw_hit* out, int64_t cap,
                        void* stream);
 
+/* CandidateRecord (src/cpp/common/runtime_contracts.h:41-56), 80 bytes with
+ * the C++ struct's natural alignment (reserved fields are its padding). */
+typedef struct {
+  uint32_t schema_version, reserved0;
+  uint64_t candidate_id;
+  uint64_t query_id;
+  uint32_t slab_id, object_a_id, patch_a_id, object_b_id, patch_b_id;
+  uint8_t proxy_type_a, proxy_type_b;
+  uint16_t reserved1;
+  uint32_t rt_hit_count;
+  float motion_bound[4];
+  uint32_t proxy_features_offset;
+  uint32_t flags;
+  uint32_t reserved2;
+} p2c_candidate;
+
+/* Exclusive scan of n u32 values into out[n + 1] (u64; out[n] = total);
+ * part: device u64[n / 1024 + 2].  Used for the raw-hit offsets. */
+int p2c_exclusive_scan_u32(const uint32_t* in, int64_t n, uint64_t* out, uint64_t* part, void* stream);
+/* Workspace bytes of p2c_proxy_compact for n_raw hits. */
+size_t p2c_proxy_compact_workspace_bytes(int64_t n_raw);
+/* CompactRawCandidates (src/cpp/rt_candidate/candidate_buffer.cpp:183-228)
+ * on the device: validates every raw hit as ValidateRawHit (:66-102), merges
+ * hits of equal MakeCandidateKey (:104-124) in buffer order (rt_hit_count
+ * sum, per-field std::max of the motion bounds, OR of the flags) and writes
+ * the candidates in key order with candidate_id 1, 2, ...  Proxy types must
+ * be below 256 (the reference's ProxyType is a byte).  status (device
+ * u64[2]): [0] the candidate count, [1] 0 on success, else 1 + (index of the
+ * first failing hit in buffer order << 4 | code); p2c_proxy_status_message
+ * gives the reference's message for it.  Candidates at or beyond `cap` are
+ * dropped. */
+int p2c_proxy_compact(const p2c_proxy_set* set, const p2c_raw_hit* raw, int64_t n_raw, void* workspace,
+                      size_t workspace_bytes, p2c_candidate* out, int64_t cap, uint64_t* status, void* stream);
+const char* p2c_proxy_status_message(uint64_t error_word);
+/* Host-owned forms for the reference's host-vector API
+ * (GenerateCandidatesForProxyScene, py_module.cpp:103-115, and
+ * CompactRawCandidates): the set's arrays are host pointers; the library
+ * copies them in, runs count, scan, fill and compaction on the device and
+ * returns malloc'ed raw hits and candidates (free with p2c_host_free).
+ * stage_ms (optional, [2]): device time of the trace and of the compaction. */
+int p2c_proxy_generate_host(const p2c_proxy_set* host_set, p2c_raw_hit** raw, int64_t* n_raw, p2c_candidate** cand,
+                            int64_t* n_cand, double* stage_ms);
+int p2c_proxy_compact_host(const p2c_proxy_set* host_set, const p2c_raw_hit* raw, int64_t n_raw,
+                           p2c_candidate** cand, int64_t* n_cand);
+void p2c_host_free(void* p);
+
 /* ---- scheduling scorers ---------------------------------------------------
  * Scores only order certificate work; exact certificates decide every pair.
  *

@@ -7,6 +7,7 @@
 
 #include "certificate_hip.h"
 #include "p2c_hip.h"
+#include "proxy_hip.h"
 
 namespace py = pybind11;
 using namespace p2cccd;
@@ -30,6 +31,16 @@ std::vector<PrimitiveIntervalResult> BatchOrThrow(
   }
   ThrowIfError(s);
   return out;
+}
+
+// Backend names of ParseCandidateBackend (py_module.cpp:87-95) plus this
+// build's own; every accepted name runs the gfx950 path.
+void CheckCandidateBackend(const std::string& name) {
+  static const char* const kNames[] = {"cpu_reference", "cpu_reference_rt", "optix", "optix_compatible",
+                                       "optix_rt",      "hip_exact",        "hip_lbvh"};
+  for (const char* n : kNames)
+    if (name == n) return;
+  throw py::value_error("unsupported candidate backend: " + name);
 }
 
 py::dict BindingStatus() {
@@ -308,6 +319,157 @@ PYBIND11_MODULE(p2cccd_cpp, m) {
           return true;
         },
         py::arg("rows"));
+  // ---- proxy-scene candidates (py_module.cpp:1577-1660, 1870-1882) ----
+  py::class_<Aabb>(m, "Aabb")
+      .def(py::init<>())
+      .def_readwrite("min", &Aabb::min)
+      .def_readwrite("max", &Aabb::max);
+  py::class_<PatchMotionBound>(m, "PatchMotionBound")
+      .def(py::init<>())
+      .def_readwrite("patch_id", &PatchMotionBound::patch_id)
+      .def_readwrite("t0", &PatchMotionBound::t0)
+      .def_readwrite("t1", &PatchMotionBound::t1)
+      .def_readwrite("center_t0", &PatchMotionBound::center_t0)
+      .def_readwrite("center_t1", &PatchMotionBound::center_t1)
+      .def_readwrite("translation_bound", &PatchMotionBound::translation_bound)
+      .def_readwrite("rotation_angle", &PatchMotionBound::rotation_angle)
+      .def_readwrite("center_rotation_bound", &PatchMotionBound::center_rotation_bound)
+      .def_readwrite("surface_rotation_bound", &PatchMotionBound::surface_rotation_bound)
+      .def_readwrite("radial_motion_bound", &PatchMotionBound::radial_motion_bound)
+      .def_readwrite("conservative_radius", &PatchMotionBound::conservative_radius);
+  py::class_<CapsuleProxy>(m, "CapsuleProxy")
+      .def(py::init<>())
+      .def_readwrite("patch_id", &CapsuleProxy::patch_id)
+      .def_readwrite("endpoint0", &CapsuleProxy::endpoint0)
+      .def_readwrite("endpoint1", &CapsuleProxy::endpoint1)
+      .def_readwrite("radius", &CapsuleProxy::radius)
+      .def_readwrite("motion_bound", &CapsuleProxy::motion_bound);
+  py::class_<ProxyPrimitive>(m, "ProxyPrimitive")
+      .def(py::init<>())
+      .def_readwrite("proxy_id", &ProxyPrimitive::proxy_id)
+      .def_readwrite("object_id", &ProxyPrimitive::object_id)
+      .def_readwrite("patch_id", &ProxyPrimitive::patch_id)
+      .def_readwrite("slab_id", &ProxyPrimitive::slab_id)
+      .def_readwrite("motion_segment_id", &ProxyPrimitive::motion_segment_id)
+      .def_readwrite("proxy_type", &ProxyPrimitive::proxy_type)
+      .def_readwrite("t0", &ProxyPrimitive::t0)
+      .def_readwrite("t1", &ProxyPrimitive::t1)
+      .def_readwrite("bounds", &ProxyPrimitive::bounds)
+      .def_readwrite("capsule", &ProxyPrimitive::capsule)
+      .def_readwrite("motion_bound", &ProxyPrimitive::motion_bound);
+  py::class_<ProxyScene>(m, "ProxyScene")
+      .def(py::init<>())
+      .def_readwrite("query_id", &ProxyScene::query_id)
+      .def_readwrite("primitives", &ProxyScene::primitives);
+  py::class_<RawCandidateHit>(m, "RawCandidateHit")
+      .def(py::init<>())
+      .def_readwrite("query_id", &RawCandidateHit::query_id)
+      .def_readwrite("pair_key", &RawCandidateHit::pair_key)
+      .def_readwrite("proxy_a_index", &RawCandidateHit::proxy_a_index)
+      .def_readwrite("proxy_b_index", &RawCandidateHit::proxy_b_index)
+      .def_readwrite("slab_id", &RawCandidateHit::slab_id)
+      .def_readwrite("rt_hit_count", &RawCandidateHit::rt_hit_count)
+      .def_readwrite("flags", &RawCandidateHit::flags)
+      .def_readwrite("motion_bound", &RawCandidateHit::motion_bound);
+  py::class_<RawCandidateBuffer>(m, "RawCandidateBuffer")
+      .def(py::init<>())
+      .def_readwrite("hits", &RawCandidateBuffer::hits);
+  py::class_<CandidateRecord>(m, "CandidateRecord")
+      .def(py::init<>())
+      .def_readwrite("schema_version", &CandidateRecord::schema_version)
+      .def_readwrite("candidate_id", &CandidateRecord::candidate_id)
+      .def_readwrite("query_id", &CandidateRecord::query_id)
+      .def_readwrite("slab_id", &CandidateRecord::slab_id)
+      .def_readwrite("object_a_id", &CandidateRecord::object_a_id)
+      .def_readwrite("patch_a_id", &CandidateRecord::patch_a_id)
+      .def_readwrite("object_b_id", &CandidateRecord::object_b_id)
+      .def_readwrite("patch_b_id", &CandidateRecord::patch_b_id)
+      .def_readwrite("proxy_type_a", &CandidateRecord::proxy_type_a)
+      .def_readwrite("proxy_type_b", &CandidateRecord::proxy_type_b)
+      .def_readwrite("rt_hit_count", &CandidateRecord::rt_hit_count)
+      .def_readwrite("motion_bound", &CandidateRecord::motion_bound)
+      .def_readwrite("proxy_features_offset", &CandidateRecord::proxy_features_offset)
+      .def_readwrite("flags", &CandidateRecord::flags);
+  py::class_<RtCandidateTiming>(m, "RtCandidateTiming")
+      .def(py::init<>())
+      .def_readwrite("build_ms", &RtCandidateTiming::build_ms)
+      .def_readwrite("update_ms", &RtCandidateTiming::update_ms)
+      .def_readwrite("trace_ms", &RtCandidateTiming::trace_ms)
+      .def_readwrite("compact_ms", &RtCandidateTiming::compact_ms)
+      .def_readwrite("stats_ms", &RtCandidateTiming::stats_ms)
+      .def_readwrite("total_ms", &RtCandidateTiming::total_ms);
+  py::class_<CandidateDensityStats>(m, "CandidateDensityStats")
+      .def(py::init<>())
+      .def_readwrite("schema_version", &CandidateDensityStats::schema_version)
+      .def_readwrite("query_id", &CandidateDensityStats::query_id)
+      .def_readwrite("proxy_count", &CandidateDensityStats::proxy_count)
+      .def_readwrite("object_count", &CandidateDensityStats::object_count)
+      .def_readwrite("slab_count", &CandidateDensityStats::slab_count)
+      .def_readwrite("cross_object_same_slab_pair_count", &CandidateDensityStats::cross_object_same_slab_pair_count)
+      .def_readwrite("raw_hit_count", &CandidateDensityStats::raw_hit_count)
+      .def_readwrite("compact_candidate_count", &CandidateDensityStats::compact_candidate_count)
+      .def_readwrite("raw_hits_per_proxy", &CandidateDensityStats::raw_hits_per_proxy)
+      .def_readwrite("candidates_per_proxy", &CandidateDensityStats::candidates_per_proxy)
+      .def_readwrite("candidates_per_slab", &CandidateDensityStats::candidates_per_slab)
+      .def_readwrite("aabb_overlap_ratio", &CandidateDensityStats::aabb_overlap_ratio)
+      .def_readwrite("avg_rt_hits_per_candidate", &CandidateDensityStats::avg_rt_hits_per_candidate)
+      .def_readwrite("timing", &CandidateDensityStats::timing)
+      .def_readwrite("backend_name", &CandidateDensityStats::backend_name);
+  py::class_<CandidateGenerationResult>(m, "CandidateGenerationResult")
+      .def(py::init<>())
+      .def_readwrite("backend_name", &CandidateGenerationResult::backend_name)
+      .def_readwrite("raw_buffer", &CandidateGenerationResult::raw_buffer)
+      .def_readwrite("candidates", &CandidateGenerationResult::candidates)
+      .def_readwrite("timing", &CandidateGenerationResult::timing)
+      .def_readwrite("density", &CandidateGenerationResult::density);
+  m.def("validate_proxy_scene",
+        [](const ProxyScene& scene) {
+          ThrowIfError(ValidateProxyScene(scene));
+          return true;
+        },
+        py::arg("scene"));
+  m.def("generate_candidates_for_proxy_scene",
+        [](const ProxyScene& scene, std::uint64_t query_id, const std::string& backend_name,
+           bool allow_optix_cpu_fallback) {
+          (void)allow_optix_cpu_fallback;  // no fallback exists: every backend name runs the device path
+          CheckCandidateBackend(backend_name);
+          CandidateGenerationResult result;
+          Status s;
+          {
+            py::gil_scoped_release release;
+            s = GenerateCandidatesHip(scene, query_id == 0 ? scene.query_id : query_id, &result);
+          }
+          ThrowIfError(s);
+          return result;
+        },
+        py::arg("scene"), py::arg("query_id") = 0, py::arg("backend_name") = "cpu_reference",
+        py::arg("allow_optix_cpu_fallback") = false);
+  m.def("generate_raw_candidates_cpu",
+        [](const ProxyScene& scene, std::uint64_t query_id) {
+          // The reference's name; the hits come from the device path, identical
+          // to the CPU loop's in content and order.
+          CandidateGenerationResult result;
+          Status s;
+          {
+            py::gil_scoped_release release;
+            s = GenerateCandidatesHip(scene, query_id == 0 ? scene.query_id : query_id, &result);
+          }
+          ThrowIfError(s);
+          return result.raw_buffer;
+        },
+        py::arg("scene"), py::arg("query_id") = 0);
+  m.def("compact_raw_candidates",
+        [](const ProxyScene& scene, const RawCandidateBuffer& raw_buffer) {
+          std::vector<CandidateRecord> out;
+          Status s;
+          {
+            py::gil_scoped_release release;
+            s = CompactRawCandidatesHip(scene, raw_buffer, &out);
+          }
+          ThrowIfError(s);
+          return out;
+        },
+        py::arg("scene"), py::arg("raw_buffer"));
   m.def("audit_log_rows_for_query",
         [](const std::vector<AuditLogRow>& rows, std::uint64_t query_id) {
           std::vector<AuditLogRow> out;

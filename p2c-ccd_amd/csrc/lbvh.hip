@@ -30,14 +30,13 @@
 
 #include "p2c_hip.h"
 #include "runtime.h"
+#include "sort_scan.h"
 
 namespace p2c {
 namespace {
 
 constexpr int kBlock = 256;
 constexpr int kMaxObjects = 64;
-constexpr int kSortItems = 16;  // keys per lane per radix tile
-constexpr int kSortTile = kBlock * kSortItems;
 constexpr int kHitCache = 64;  // partners kept per query by the count pass
 
 struct Box6 {
@@ -274,162 +273,6 @@ __global__ __launch_bounds__(kBlock) void prim_box_kernel(int kind, const double
     keys[i] = morton_of(b, lo, inv);
     vals[i] = (uint32_t)i;
   }
-}
-
-// ---- 2. radix sort -------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void radix_count_kernel(const uint32_t* __restrict__ keys, int64_t n, int shift,
-                                                             uint32_t* __restrict__ hist, int64_t nblk) {
-  __shared__ uint32_t cnt[256];
-  cnt[threadIdx.x] = 0;
-  __syncthreads();
-  const int64_t base = int64_t(blockIdx.x) * kSortTile;
-  for (int j = 0; j < kSortItems; ++j) {
-    const int64_t e = base + int64_t(j) * kBlock + threadIdx.x;
-    if (e < n) atomicAdd(&cnt[(keys[e] >> shift) & 0xffu], 1u);
-  }
-  __syncthreads();
-  hist[int64_t(threadIdx.x) * nblk + blockIdx.x] = cnt[threadIdx.x];
-}
-
-// Stable scatter.  Wave w of a tile owns the contiguous chunk
-// [base + w*64*kSortItems, +64*kSortItems) and walks it 64 keys per round;
-// a key's rank among equal digits in its round comes from 8 ballots, its
-// offset in the wave from the wave's LDS digit counters, and the wave's
-// offset in the tile from a prefix over the waves' counters.
-__global__ __launch_bounds__(kBlock) void radix_scatter_kernel(const uint32_t* __restrict__ kin,
-                                                               const uint32_t* __restrict__ vin, int64_t n, int shift,
-                                                               const uint64_t* __restrict__ hist_off, int64_t nblk,
-                                                               uint32_t* __restrict__ kout,
-                                                               uint32_t* __restrict__ vout) {
-  constexpr int kWaves = kBlock / 64;
-  __shared__ uint32_t wcnt[kWaves][256];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  for (int i = threadIdx.x; i < kWaves * 256; i += kBlock) (&wcnt[0][0])[i] = 0;
-  __syncthreads();
-  const int64_t chunk = int64_t(blockIdx.x) * kSortTile + int64_t(w) * 64 * kSortItems;
-  uint32_t key[kSortItems], val[kSortItems], pos[kSortItems];
-  const uint64_t lt = (1ull << lane) - 1ull;
-#pragma unroll
-  for (int j = 0; j < kSortItems; ++j) {
-    const int64_t e = chunk + int64_t(j) * 64 + lane;
-    const bool valid = e < n;
-    key[j] = valid ? kin[e] : 0u;
-    val[j] = valid ? vin[e] : 0u;
-    const uint32_t d = (key[j] >> shift) & 0xffu;
-    uint64_t same = __ballot(valid);
-#pragma unroll
-    for (int b = 0; b < 8; ++b) {
-      const uint64_t bal = __ballot(valid && ((d >> b) & 1u));
-      same &= ((d >> b) & 1u) ? bal : ~bal;
-    }
-    const uint32_t rank = (uint32_t)__popcll(same & lt);
-    const uint32_t base = valid ? wcnt[w][d] : 0u;
-    pos[j] = base + rank;
-    __builtin_amdgcn_wave_barrier();
-    if (valid && rank == 0) wcnt[w][d] = base + (uint32_t)__popcll(same);
-    __builtin_amdgcn_wave_barrier();
-  }
-  __syncthreads();
-  // exclusive prefix over waves, per digit (one thread per digit)
-  {
-    const int d = threadIdx.x;
-    uint32_t run = 0;
-#pragma unroll
-    for (int ww = 0; ww < kWaves; ++ww) {
-      const uint32_t c = wcnt[ww][d];
-      wcnt[ww][d] = run;
-      run += c;
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int j = 0; j < kSortItems; ++j) {
-    const int64_t e = chunk + int64_t(j) * 64 + lane;
-    if (e < n) {
-      const uint32_t d = (key[j] >> shift) & 0xffu;
-      const uint64_t dst = hist_off[int64_t(d) * nblk + blockIdx.x] + wcnt[w][d] + pos[j];
-      kout[dst] = key[j];
-      vout[dst] = val[j];
-    }
-  }
-}
-
-// ---- exclusive scan (u32 -> u64), three phases --------------------------
-__device__ inline uint64_t wave_inclusive(uint64_t x) {
-  const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const uint64_t y = __shfl_up(x, o);
-    if (lane >= o) x += y;
-  }
-  return x;
-}
-
-// Block-wide exclusive scan of 4 items per thread (1024 per block).
-__device__ inline void block_scan4(uint64_t v[4], uint64_t* block_total) {
-  __shared__ uint64_t wsum[kBlock / 64];
-  uint64_t s = 0;
-  uint64_t t[4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    t[i] = s;
-    s += v[i];
-  }
-  const uint64_t inc = wave_inclusive(s);
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  if (lane == 63) wsum[w] = inc;
-  __syncthreads();
-  uint64_t wbase = 0, tot = 0;
-  for (int i = 0; i < kBlock / 64; ++i) {
-    if (i < w) wbase += wsum[i];
-    tot += wsum[i];
-  }
-  const uint64_t excl = wbase + inc - s;
-#pragma unroll
-  for (int i = 0; i < 4; ++i) v[i] = excl + t[i];
-  *block_total = tot;
-  __syncthreads();
-}
-
-__global__ __launch_bounds__(kBlock) void scan_local_kernel(const uint32_t* __restrict__ in, int64_t n,
-                                                            uint64_t* __restrict__ out, uint64_t* __restrict__ part) {
-  const int64_t base = int64_t(blockIdx.x) * 1024 + threadIdx.x * 4;
-  uint64_t v[4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i) v[i] = base + i < n ? in[base + i] : 0;
-  uint64_t tot;
-  block_scan4(v, &tot);
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-    if (base + i < n) out[base + i] = v[i];
-  if (threadIdx.x == 0) part[blockIdx.x] = tot;
-}
-
-__global__ __launch_bounds__(kBlock) void scan_part_kernel(uint64_t* __restrict__ part, int64_t m,
-                                                           uint64_t* __restrict__ grand) {
-  uint64_t carry = 0;
-  for (int64_t c = 0; c < m; c += 1024) {
-    const int64_t base = c + threadIdx.x * 4;
-    uint64_t v[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) v[i] = base + i < m ? part[base + i] : 0;
-    uint64_t tot;
-    block_scan4(v, &tot);
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-      if (base + i < m) part[base + i] = carry + v[i];
-    carry += tot;
-  }
-  if (threadIdx.x == 0) *grand = carry;
-}
-
-__global__ __launch_bounds__(kBlock) void scan_add_kernel(uint64_t* __restrict__ out, int64_t n,
-                                                          const uint64_t* __restrict__ part) {
-  const int64_t base = int64_t(blockIdx.x) * 1024 + threadIdx.x * 4;
-  const uint64_t add = part[blockIdx.x];
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-    if (base + i < n) out[base + i] += add;
 }
 
 // ---- 3. Karras build, escapes, refit ------------------------------------
@@ -732,27 +575,6 @@ int check_mesh(const p2c_bp_mesh* m) {
   return P2C_OK;
 }
 
-int radix_sort(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, int64_t n, uint32_t* hist, uint64_t* hist_off,
-               uint64_t* part, hipStream_t s) {
-  if (n <= 1) return P2C_OK;
-  const int64_t nblk = (n + kSortTile - 1) / kSortTile;
-  for (int pass = 0; pass < 4; ++pass) {
-    const int shift = pass * 8;
-    hipLaunchKernelGGL(radix_count_kernel, dim3(unsigned(nblk)), dim3(kBlock), 0, s, k0, n, shift, hist, nblk);
-    const int64_t m = 256 * nblk;
-    const int64_t sb = (m + 1023) / 1024;
-    hipLaunchKernelGGL(scan_local_kernel, dim3(unsigned(sb)), dim3(kBlock), 0, s, hist, m, hist_off, part);
-    hipLaunchKernelGGL(scan_part_kernel, dim3(1), dim3(kBlock), 0, s, part, sb, part + sb);
-    hipLaunchKernelGGL(scan_add_kernel, dim3(unsigned(sb)), dim3(kBlock), 0, s, hist_off, m, part);
-    hipLaunchKernelGGL(radix_scatter_kernel, dim3(unsigned(nblk)), dim3(kBlock), 0, s, k0, v0, n, shift, hist_off,
-                       nblk, k1, v1);
-    std::swap(k0, k1);
-    std::swap(v0, v1);
-  }
-  P2C_HIP_CHECK(hipGetLastError());
-  return P2C_OK;  // 4 passes: result back in the first buffers
-}
-
 }  // namespace
 }  // namespace p2c
 
@@ -849,14 +671,7 @@ int p2c_bp_count(const p2c_bp_mesh* m, void* ws, uint64_t* totals, void* stream)
                        at<Node>(ws, L.enodes), ne, at<Box6>(ws, L.ebox), at<uint32_t>(ws, L.eorder), er,
                        at<uint32_t>(ws, L.err), counts + nv, at<int32_t>(ws, L.cache) + nv * kHitCache, nullptr, nv,
                        nullptr, 0);
-  const int64_t n = nv + ne;
-  const int64_t sb = (n + 1023) / 1024;
-  uint64_t* part = at<uint64_t>(ws, L.scan_part);
-  if (n > 0) {
-    hipLaunchKernelGGL(scan_local_kernel, dim3(unsigned(sb)), dim3(kBlock), 0, s, counts, n, offsets, part);
-    hipLaunchKernelGGL(scan_part_kernel, dim3(1), dim3(kBlock), 0, s, part, sb, offsets + n);
-    hipLaunchKernelGGL(scan_add_kernel, dim3(unsigned(sb)), dim3(kBlock), 0, s, offsets, n, part);
-  }
+  if (int rc = exclusive_scan_u32(counts, nv + ne, offsets, at<uint64_t>(ws, L.scan_part), s)) return rc;
   hipLaunchKernelGGL(totals_kernel, dim3(1), dim3(64), 0, s, offsets, nv, ne, at<uint32_t>(ws, L.err), totals);
   P2C_HIP_CHECK(hipGetLastError());
   return P2C_OK;

@@ -1,0 +1,225 @@
+// Device radix sort and exclusive scan shared by the broad phase and the
+// proxy-scene compaction (gfx950).
+//   radix sort: LSD over 8-bit digits of u32 keys carrying u32 values; per
+//     4096-key tile a digit histogram, an exclusive scan of the histograms
+//     (digit-major), then a stable scatter ranked with wave ballots and
+//     per-wave LDS digit counters.
+//   scan: u32 -> u64 exclusive scan in three phases (per-1024 block scans,
+//     a scan of the block totals, the add-back).
+#pragma once
+
+#include <utility>
+
+#include "p2c_hip.h"
+#include "runtime.h"
+
+namespace p2c {
+namespace {
+
+constexpr int kScanBlock = 256;
+constexpr int kSortItems = 16;  // keys per lane per radix tile
+constexpr int kSortTile = kScanBlock * kSortItems;
+
+// ---- 2. radix sort -------------------------------------------------------
+__global__ __launch_bounds__(kScanBlock) void radix_count_kernel(const uint32_t* __restrict__ keys, int64_t n, int shift,
+                                                             uint32_t* __restrict__ hist, int64_t nblk) {
+  __shared__ uint32_t cnt[256];
+  cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t base = int64_t(blockIdx.x) * kSortTile;
+  for (int j = 0; j < kSortItems; ++j) {
+    const int64_t e = base + int64_t(j) * kScanBlock + threadIdx.x;
+    if (e < n) atomicAdd(&cnt[(keys[e] >> shift) & 0xffu], 1u);
+  }
+  __syncthreads();
+  hist[int64_t(threadIdx.x) * nblk + blockIdx.x] = cnt[threadIdx.x];
+}
+
+// Stable scatter.  Wave w of a tile owns the contiguous chunk
+// [base + w*64*kSortItems, +64*kSortItems) and walks it 64 keys per round;
+// a key's rank among equal digits in its round comes from 8 ballots, its
+// offset in the wave from the wave's LDS digit counters, and the wave's
+// offset in the tile from a prefix over the waves' counters.
+__global__ __launch_bounds__(kScanBlock) void radix_scatter_kernel(const uint32_t* __restrict__ kin,
+                                                               const uint32_t* __restrict__ vin, int64_t n, int shift,
+                                                               const uint64_t* __restrict__ hist_off, int64_t nblk,
+                                                               uint32_t* __restrict__ kout,
+                                                               uint32_t* __restrict__ vout) {
+  constexpr int kWaves = kScanBlock / 64;
+  __shared__ uint32_t wcnt[kWaves][256];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int i = threadIdx.x; i < kWaves * 256; i += kScanBlock) (&wcnt[0][0])[i] = 0;
+  __syncthreads();
+  const int64_t chunk = int64_t(blockIdx.x) * kSortTile + int64_t(w) * 64 * kSortItems;
+  uint32_t key[kSortItems], val[kSortItems], pos[kSortItems];
+  const uint64_t lt = (1ull << lane) - 1ull;
+#pragma unroll
+  for (int j = 0; j < kSortItems; ++j) {
+    const int64_t e = chunk + int64_t(j) * 64 + lane;
+    const bool valid = e < n;
+    key[j] = valid ? kin[e] : 0u;
+    val[j] = valid ? vin[e] : 0u;
+    const uint32_t d = (key[j] >> shift) & 0xffu;
+    uint64_t same = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const uint64_t bal = __ballot(valid && ((d >> b) & 1u));
+      same &= ((d >> b) & 1u) ? bal : ~bal;
+    }
+    const uint32_t rank = (uint32_t)__popcll(same & lt);
+    const uint32_t base = valid ? wcnt[w][d] : 0u;
+    pos[j] = base + rank;
+    __builtin_amdgcn_wave_barrier();
+    if (valid && rank == 0) wcnt[w][d] = base + (uint32_t)__popcll(same);
+    __builtin_amdgcn_wave_barrier();
+  }
+  __syncthreads();
+  // exclusive prefix over waves, per digit (one thread per digit)
+  {
+    const int d = threadIdx.x;
+    uint32_t run = 0;
+#pragma unroll
+    for (int ww = 0; ww < kWaves; ++ww) {
+      const uint32_t c = wcnt[ww][d];
+      wcnt[ww][d] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kSortItems; ++j) {
+    const int64_t e = chunk + int64_t(j) * 64 + lane;
+    if (e < n) {
+      const uint32_t d = (key[j] >> shift) & 0xffu;
+      const uint64_t dst = hist_off[int64_t(d) * nblk + blockIdx.x] + wcnt[w][d] + pos[j];
+      kout[dst] = key[j];
+      vout[dst] = val[j];
+    }
+  }
+}
+
+// ---- exclusive scan (u32 -> u64), three phases --------------------------
+__device__ inline uint64_t wave_inclusive(uint64_t x) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint64_t y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  return x;
+}
+
+// Block-wide exclusive scan of 4 items per thread (1024 per block).
+__device__ inline void block_scan4(uint64_t v[4], uint64_t* block_total) {
+  __shared__ uint64_t wsum[kScanBlock / 64];
+  uint64_t s = 0;
+  uint64_t t[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    t[i] = s;
+    s += v[i];
+  }
+  const uint64_t inc = wave_inclusive(s);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 63) wsum[w] = inc;
+  __syncthreads();
+  uint64_t wbase = 0, tot = 0;
+  for (int i = 0; i < kScanBlock / 64; ++i) {
+    if (i < w) wbase += wsum[i];
+    tot += wsum[i];
+  }
+  const uint64_t excl = wbase + inc - s;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) v[i] = excl + t[i];
+  *block_total = tot;
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(kScanBlock) void scan_local_kernel(const uint32_t* __restrict__ in, int64_t n,
+                                                            uint64_t* __restrict__ out, uint64_t* __restrict__ part) {
+  const int64_t base = int64_t(blockIdx.x) * 1024 + threadIdx.x * 4;
+  uint64_t v[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) v[i] = base + i < n ? in[base + i] : 0;
+  uint64_t tot;
+  block_scan4(v, &tot);
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    if (base + i < n) out[base + i] = v[i];
+  if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kScanBlock) void scan_part_kernel(uint64_t* __restrict__ part, int64_t m,
+                                                           uint64_t* __restrict__ grand) {
+  uint64_t carry = 0;
+  for (int64_t c = 0; c < m; c += 1024) {
+    const int64_t base = c + threadIdx.x * 4;
+    uint64_t v[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) v[i] = base + i < m ? part[base + i] : 0;
+    uint64_t tot;
+    block_scan4(v, &tot);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (base + i < m) part[base + i] = carry + v[i];
+    carry += tot;
+  }
+  if (threadIdx.x == 0) *grand = carry;
+}
+
+__global__ __launch_bounds__(kScanBlock) void scan_add_kernel(uint64_t* __restrict__ out, int64_t n,
+                                                          const uint64_t* __restrict__ part) {
+  const int64_t base = int64_t(blockIdx.x) * 1024 + threadIdx.x * 4;
+  const uint64_t add = part[blockIdx.x];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    if (base + i < n) out[base + i] += add;
+}
+
+// Stable LSD sort of (key, value) u32 pairs on the low 8 * passes bits of the
+// keys, ping-ponging between (k0, v0) and (k1, v1); an even number of passes
+// leaves the result in (k0, v0), an odd number in (k1, v1).  Workspace: hist
+// u32[256 * tiles], hist_off u64[256 * tiles], part u64[256 * tiles / 1024 + 2]
+// with tiles = ceil(n / kSortTile).
+inline int radix_sort_passes(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, int64_t n, int passes,
+                             uint32_t* hist, uint64_t* hist_off, uint64_t* part, hipStream_t s) {
+  if (n <= 1) return P2C_OK;
+  const int64_t nblk = (n + kSortTile - 1) / kSortTile;
+  for (int pass = 0; pass < passes; ++pass) {
+    const int shift = pass * 8;
+    hipLaunchKernelGGL(radix_count_kernel, dim3(unsigned(nblk)), dim3(kScanBlock), 0, s, k0, n, shift, hist, nblk);
+    const int64_t m = 256 * nblk;
+    const int64_t sb = (m + 1023) / 1024;
+    hipLaunchKernelGGL(scan_local_kernel, dim3(unsigned(sb)), dim3(kScanBlock), 0, s, hist, m, hist_off, part);
+    hipLaunchKernelGGL(scan_part_kernel, dim3(1), dim3(kScanBlock), 0, s, part, sb, part + sb);
+    hipLaunchKernelGGL(scan_add_kernel, dim3(unsigned(sb)), dim3(kScanBlock), 0, s, hist_off, m, part);
+    hipLaunchKernelGGL(radix_scatter_kernel, dim3(unsigned(nblk)), dim3(kScanBlock), 0, s, k0, v0, n, shift, hist_off,
+                       nblk, k1, v1);
+    std::swap(k0, k1);
+    std::swap(v0, v1);
+  }
+  P2C_HIP_CHECK(hipGetLastError());
+  return P2C_OK;
+}
+
+// Full 32-bit sort (4 passes): the result is back in (k0, v0).
+inline int radix_sort(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, int64_t n, uint32_t* hist,
+                      uint64_t* hist_off, uint64_t* part, hipStream_t s) {
+  return radix_sort_passes(k0, v0, k1, v1, n, 4, hist, hist_off, part, s);
+}
+
+// Exclusive scan of n u32 values into u64 (out[n] = total); part holds
+// n / 1024 + 2 u64.
+inline int exclusive_scan_u32(const uint32_t* in, int64_t n, uint64_t* out, uint64_t* part, hipStream_t s) {
+  if (n <= 0) return P2C_OK;
+  const int64_t sb = (n + 1023) / 1024;
+  hipLaunchKernelGGL(scan_local_kernel, dim3(unsigned(sb)), dim3(kScanBlock), 0, s, in, n, out, part);
+  hipLaunchKernelGGL(scan_part_kernel, dim3(1), dim3(kScanBlock), 0, s, part, sb, out + n);
+  hipLaunchKernelGGL(scan_add_kernel, dim3(unsigned(sb)), dim3(kScanBlock), 0, s, out, n, part);
+  P2C_HIP_CHECK(hipGetLastError());
+  return P2C_OK;
+}
+
+
+}  // namespace
+}  // namespace p2c

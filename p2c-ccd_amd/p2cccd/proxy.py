@@ -4,17 +4,19 @@ The reference path (src/cpp/rt_candidate/candidate_buffer.cpp:144-228,
 candidate_stats.cpp:128-170, proxy_scene.cpp:18-154):
   raw hits   every pair i < j of proxy primitives with different objects, the
              same slab, strictly overlapping times and inclusively overlapping
-             AABBs, in (i, j) order (p2c_proxy_raw_count / _fill);
+             AABBs, in (i, j) order (p2c_proxy_raw_count, an exclusive scan,
+             p2c_proxy_raw_fill);
   compaction one candidate per key (query, slab, first object, first patch,
              second object, second patch, first type, second type) with
              first = the smaller (object, patch, proxy id); rt_hit_count
              summed, motion bound maxed, flags OR'ed; candidate_id = 1, 2, ...
-             in key order (std::map order);
+             in key order (std::map order) -- p2c_proxy_compact, a stable
+             radix sort over the key fields and a merge per key on the device;
   density    proxy / object / slab counts, cross-object same-slab pairs and
-             the ratios of ComputeCandidateDensityStats.
-Compaction is a segmented reduction after a stable lexicographic sort, done
-with device tensors.  The scene is given as arrays (ProxyPrimitive fields,
-proxy_id = index); validation raises ValueError with the reference's messages.
+             the ratios of ComputeCandidateDensityStats (host bookkeeping).
+The scene is given as arrays (ProxyPrimitive fields, proxy_id = index);
+validation raises ValueError with the reference's messages.  The same device
+entry points serve p2cccd_cpp.generate_candidates_for_proxy_scene.
 """
 from __future__ import annotations
 
@@ -44,9 +46,24 @@ class ProxySet(ctypes.Structure):
     _fields_ = [(name, ctypes.c_void_p) for name in FIELDS] + [("n", ctypes.c_int64), ("query_id", ctypes.c_uint64)]
 
 
+# p2c_candidate: the C++ CandidateRecord with its natural alignment (80 bytes)
+CANDIDATE_C_DTYPE = np.dtype({
+    "names": ["schema_version", "candidate_id", "query_id", "slab_id", "object_a_id", "patch_a_id", "object_b_id",
+              "patch_b_id", "proxy_type_a", "proxy_type_b", "rt_hit_count", "motion_bound", "proxy_features_offset",
+              "flags"],
+    "formats": ["<u4", "<u8", "<u8", "<u4", "<u4", "<u4", "<u4", "<u4", "u1", "u1", "<u4", ("<f4", (4,)), "<u4", "<u4"],
+    "offsets": [0, 8, 16, 24, 28, 32, 36, 40, 44, 45, 48, 52, 68, 72],
+    "itemsize": 80})
+
 _vp, _i64 = nat._vp, nat._i64
 nat._sig("p2c_proxy_raw_count", [_vp, _vp, _vp])
 nat._sig("p2c_proxy_raw_fill", [_vp, _vp, _vp, _i64, _vp])
+nat._sig("p2c_exclusive_scan_u32", [_vp, _i64, _vp, _vp, _vp])
+nat._sig("p2c_proxy_compact", [_vp, _vp, _i64, _vp, ctypes.c_size_t, _vp, _i64, _vp, _vp])
+nat.lib.p2c_proxy_compact_workspace_bytes.argtypes = [_i64]
+nat.lib.p2c_proxy_compact_workspace_bytes.restype = ctypes.c_size_t
+nat.lib.p2c_proxy_status_message.argtypes = [ctypes.c_uint64]
+nat.lib.p2c_proxy_status_message.restype = ctypes.c_char_p
 
 
 def _time_tol(t0, t1):
@@ -92,72 +109,51 @@ def _device_arrays(scene: dict, device):
     return out
 
 
+def _set(query_id: int, arr: dict) -> ProxySet:
+    return ProxySet(*[arr[k].data_ptr() for k in FIELDS], arr["t0"].numel(), query_id)
+
+
 def raw_hits(query_id: int, arr: dict, stream=None) -> torch.Tensor:
     """Raw hits as a device uint8 tensor of n_raw * 64 bytes (RawCandidateHit layout)."""
     n = arr["t0"].numel()
-    s = ProxySet(*[arr[k].data_ptr() for k in FIELDS], n, query_id)
+    s = _set(query_id, arr)
     dev = arr["t0"].device
     counts = torch.zeros(max(n, 1), dtype=torch.int32, device=dev)
-    nat.call("p2c_proxy_raw_count", ctypes.addressof(s), counts.data_ptr(), nat.stream_handle(stream))
-    c64 = counts[:n].long()
-    offsets = torch.cumsum(c64, 0) - c64
-    total = int(c64.sum()) if n else 0
+    offsets = torch.zeros(n + 1, dtype=torch.int64, device=dev)
+    part = torch.empty(n // 1024 + 2, dtype=torch.int64, device=dev)
+    h = nat.stream_handle(stream)
+    if n:
+        nat.call("p2c_proxy_raw_count", ctypes.addressof(s), counts.data_ptr(), h)
+        nat.call("p2c_exclusive_scan_u32", counts.data_ptr(), n, offsets.data_ptr(), part.data_ptr(), h)
+    total = int(offsets[n]) if n else 0
     out = torch.empty(max(total, 1) * 64, dtype=torch.uint8, device=dev)
     if total:
-        nat.call("p2c_proxy_raw_fill", ctypes.addressof(s), offsets.data_ptr(), out.data_ptr(), total,
-                 nat.stream_handle(stream))
+        nat.call("p2c_proxy_raw_fill", ctypes.addressof(s), offsets.data_ptr(), out.data_ptr(), total, h)
     return out[: total * 64]
 
 
-def compact(query_id: int, arr: dict, raw: torch.Tensor):
-    """CompactRawCandidates on device tensors; returns a CANDIDATE_DTYPE array."""
+def compact(query_id: int, arr: dict, raw: torch.Tensor, stream=None) -> np.ndarray:
+    """CompactRawCandidates on the device; returns a CANDIDATE_DTYPE array.
+    Raises ValueError with the reference's message for an invalid raw hit or
+    an rt_hit_count overflow."""
     m = raw.numel() // 64
-    if m == 0:
-        return np.zeros(0, CANDIDATE_DTYPE)
-    words = raw.view(torch.int32).view(m, 16)
-    a = words[:, 4].long() & 0xFFFFFFFF
-    b = words[:, 5].long() & 0xFFFFFFFF
-    slab = words[:, 6].long() & 0xFFFFFFFF
-    rt = words[:, 7].long() & 0xFFFFFFFF
-    flags = words[:, 8].long() & 0xFFFFFFFF
-    motion = raw.view(torch.float32).view(m, 16)[:, 10:14]
-    obj = arr["object_id"].long() & 0xFFFFFFFF
-    patch = arr["patch_id"].long() & 0xFFFFFFFF
-    ptype = arr["proxy_type"].long() & 0xFFFFFFFF
-    oa, ob, pa, pb = obj[a], obj[b], patch[a], patch[b]
-    swap = (oa > ob) | ((oa == ob) & ((pa > pb) | ((pa == pb) & (a > b))))
-    first = torch.where(swap, b, a)
-    second = torch.where(swap, a, b)
-    fields = [slab, obj[first], patch[first], obj[second], patch[second], ptype[first], ptype[second]]
-    order = torch.arange(m, device=raw.device)
-    for f in reversed(fields):  # stable LSD passes: lexicographic by the key
-        order = order[torch.argsort(f[order], stable=True)]
-    keys = torch.stack([f[order] for f in fields], 1)
-    new = torch.ones(m, dtype=torch.bool, device=raw.device)
-    new[1:] = (keys[1:] != keys[:-1]).any(1)
-    seg = torch.cumsum(new.long(), 0) - 1
-    C = int(seg[-1]) + 1
-    rt_sum = torch.zeros(C, dtype=torch.int64, device=raw.device).index_add_(0, seg, rt[order])
-    mb = torch.full((C, 4), float("-inf"), dtype=torch.float32, device=raw.device)
-    mb.scatter_reduce_(0, seg[:, None].expand(-1, 4), motion[order], reduce="amax")
-    fl = torch.zeros(C, dtype=torch.int64, device=raw.device)
-    for bit in range(int(flags.max()).bit_length()):  # OR as a per-bit maximum
-        hit = (flags[order] >> bit) & 1
-        fl |= torch.zeros(C, dtype=torch.int64, device=raw.device).scatter_reduce_(0, seg, hit, reduce="amax") << bit
-    k = keys[new].cpu().numpy()
-    out = np.zeros(C, CANDIDATE_DTYPE)
-    out["schema_version"] = 1
-    out["candidate_id"] = np.arange(1, C + 1, dtype=np.uint64)
-    out["query_id"] = query_id
-    out["slab_id"], out["object_a_id"], out["patch_a_id"] = k[:, 0], k[:, 1], k[:, 2]
-    out["object_b_id"], out["patch_b_id"] = k[:, 3], k[:, 4]
-    out["proxy_type_a"], out["proxy_type_b"] = k[:, 5], k[:, 6]
-    if int(rt_sum.max()) > 0xFFFFFFFF:
-        raise ValueError("candidate rt_hit_count overflow during compaction")
-    out["rt_hit_count"] = rt_sum.cpu().numpy()
-    out["motion_bound"] = mb.cpu().numpy()
-    out["flags"] = fl.cpu().numpy()
-    return out
+    dev = arr["t0"].device
+    s = _set(query_id, arr)
+    ws_bytes = int(nat.lib.p2c_proxy_compact_workspace_bytes(m))
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+    out = torch.empty(max(m, 1) * CANDIDATE_C_DTYPE.itemsize, dtype=torch.uint8, device=dev)
+    status = torch.zeros(2, dtype=torch.int64, device=dev)
+    raw_ptr = raw.data_ptr() if m else out.data_ptr()
+    nat.call("p2c_proxy_compact", ctypes.addressof(s), raw_ptr, m, ws.data_ptr(), ws_bytes, out.data_ptr(), m,
+             status.data_ptr(), nat.stream_handle(stream))
+    count, err = (int(v) & 0xFFFFFFFFFFFFFFFF for v in status.cpu())
+    if err:
+        raise ValueError(nat.lib.p2c_proxy_status_message(err).decode())
+    c = out[: count * CANDIDATE_C_DTYPE.itemsize].cpu().numpy().view(CANDIDATE_C_DTYPE)
+    res = np.zeros(count, CANDIDATE_DTYPE)
+    for name in CANDIDATE_DTYPE.names:
+        res[name] = c[name]
+    return res
 
 
 def density(scene: dict, n_raw: int, cand: np.ndarray) -> dict:

@@ -51,3 +51,57 @@ def test_density_arithmetic_matches_oracle():
         d = proxy.density(scene, len(raw), c)
         for f in oracle.DENSITY.names:
             assert d[f] == dens[f], f
+
+
+def cpp_scene(q: int, scene: dict):
+    """The scene as the reference's pybind objects (ProxyScene of
+    ProxyPrimitive, proxy_id = index)."""
+    import p2cccd_cpp as cpp
+    s = cpp.ProxyScene()
+    s.query_id = q
+    prims = []
+    mb_fields = ("translation_bound", "rotation_angle", "radial_motion_bound", "conservative_radius")
+    for i in range(len(scene["t0"])):
+        p = cpp.ProxyPrimitive()
+        p.proxy_id = i
+        p.object_id = int(scene["object_id"][i])
+        p.patch_id = int(scene["patch_id"][i])
+        p.slab_id = int(scene["slab_id"][i])
+        p.proxy_type = cpp.ProxyType(int(scene["proxy_type"][i]))
+        p.t0, p.t1 = float(scene["t0"][i]), float(scene["t1"][i])
+        b = cpp.Aabb()
+        b.min = [float(v) for v in scene["bounds"][i][:3]]
+        b.max = [float(v) for v in scene["bounds"][i][3:]]
+        p.bounds = b
+        m = cpp.PatchMotionBound()
+        for f, v in zip(mb_fields, scene["motion"][i]):
+            setattr(m, f, float(v))
+        p.motion_bound = m
+        prims.append(p)
+    s.primitives = prims
+    return s
+
+
+def test_pybind_validation_and_backend_names():
+    """Host-side checks of the pybind surface: ValidateProxyScene messages
+    (including proxy_id = index), backend-name parsing, and the messages the
+    device compaction reports, before any kernel runs."""
+    import p2cccd_cpp as cpp
+    q, scene, *_ = next(golden_scenes())
+    s = cpp_scene(q, scene)
+    assert cpp.validate_proxy_scene(s)
+    prims = list(s.primitives)
+    prims[1].proxy_id = 7
+    s.primitives = prims
+    with pytest.raises(ValueError, match="proxy_id must match primitive array index"):
+        cpp.validate_proxy_scene(s)
+    s = cpp_scene(q, scene)
+    s.query_id = 0
+    with pytest.raises(ValueError, match="ProxyScene.query_id is required"):
+        cpp.validate_proxy_scene(s)
+    with pytest.raises(ValueError, match="unsupported candidate backend: bogus"):
+        cpp.generate_candidates_for_proxy_scene(cpp_scene(q, scene), backend_name="bogus")
+    msg = proxy.nat.lib.p2c_proxy_status_message
+    assert msg(0) == b""
+    assert msg(1 + ((5 << 4) | 9)) == b"raw candidate motion_bound must be finite and non-negative"
+    assert msg(1 + ((0 << 4) | 10)) == b"candidate rt_hit_count overflow during compaction"

@@ -47,3 +47,92 @@ def test_large_scene_against_oracle(gpu):
     raw, cand, dens = oracle.proxy_generate(77, s)
     assert len(raw) > 1000 and len(cand) < len(raw)
     _check(res, raw, cand, dens)
+
+
+def _records(objs, fields):
+    return {f: np.array([np.array(getattr(o, f)) if f == "motion_bound" else
+                         (int(getattr(o, f)) if not isinstance(getattr(o, f), float) else getattr(o, f))
+                         for o in objs]) for f in fields}
+
+
+def test_pybind_generate_matches_golden(gpu):
+    """p2cccd_cpp.generate_candidates_for_proxy_scene (the reference's pybind
+    entry) on the device path: identical to the compiled reference's output."""
+    import p2cccd_cpp as cpp
+    from test_proxy import cpp_scene
+    for q, scene, raw, cand, dens in golden_scenes():
+        for backend in ("hip_exact", "cpu_reference", "optix"):
+            res = cpp.generate_candidates_for_proxy_scene(cpp_scene(q, scene), backend_name=backend)
+            assert res.backend_name == "hip_exact"
+            r = res.raw_buffer.hits
+            c = res.candidates
+            assert len(r) == len(raw) and len(c) == len(cand)
+            if len(r):
+                got = _records(r, RAW_FIELDS)
+                for f in RAW_FIELDS:
+                    assert np.array_equal(got[f].astype(raw[f].dtype), raw[f]), f
+            if len(c):
+                got = _records(c, CAND_FIELDS)
+                for f in CAND_FIELDS:
+                    assert np.array_equal(got[f].astype(cand[f].dtype), cand[f]), f
+            for f in oracle.DENSITY.names:
+                if f in ("timing", "backend_name", "query_id", "schema_version"):
+                    continue
+                assert getattr(res.density, f) == dens[f], f
+
+
+def test_pybind_compact_reports_first_error_in_buffer_order(gpu):
+    import p2cccd_cpp as cpp
+    from test_proxy import cpp_scene
+    q, scene, raw, cand, dens = list(golden_scenes())[5]
+    s = cpp_scene(q, scene)
+    buf = cpp.generate_raw_candidates_cpu(s)
+    assert len(buf.hits) == len(raw) > 10
+    out = cpp.compact_raw_candidates(s, buf)
+    assert [int(c.candidate_id) for c in out] == list(range(1, len(cand) + 1))
+
+    def broken(edits):
+        b = cpp.RawCandidateBuffer()
+        hits = list(cpp.generate_raw_candidates_cpu(s).hits)
+        for i, fn in edits:
+            fn(hits[i])
+        b.hits = hits
+        return b
+
+    def setattr_(f, v):
+        return lambda h: setattr(h, f, v)
+
+    cases = [
+        ([(3, setattr_("flags", 0))], "not marked valid"),
+        ([(3, setattr_("query_id", 0))], "query_id is required"),
+        ([(3, setattr_("query_id", q + 1))], "must match ProxyScene.query_id"),
+        ([(3, setattr_("proxy_b_index", 10 ** 6))], "proxy index is out of range"),
+        ([(3, setattr_("pair_key", 5))], "pair_key does not match"),
+        ([(3, setattr_("motion_bound", [0.0, -1.0, 0.0, 0.0]))], "finite and non-negative"),
+        # two broken hits: the earlier one in buffer order is reported
+        ([(7, setattr_("flags", 0)), (2, setattr_("query_id", 0))], "query_id is required"),
+        ([(2, setattr_("flags", 0)), (7, setattr_("query_id", 0))], "not marked valid"),
+    ]
+    for edits, msg in cases:
+        with pytest.raises(ValueError, match=msg):
+            cpp.compact_raw_candidates(s, broken(edits))
+    # rt_hit_count overflow: two hits of one key summing past 2^32 - 1
+    hits = list(cpp.generate_raw_candidates_cpu(s).hits)
+    keys = {}
+    pair = None
+    for i, h in enumerate(hits):
+        a, b = int(h.proxy_a_index), int(h.proxy_b_index)
+        k = (int(scene["slab_id"][a]),) + tuple(sorted([(int(scene["object_id"][a]), int(scene["patch_id"][a]),
+                                                         int(scene["proxy_type"][a])),
+                                                        (int(scene["object_id"][b]), int(scene["patch_id"][b]),
+                                                         int(scene["proxy_type"][b]))]))
+        if k in keys:
+            pair = (keys[k], i)
+            break
+        keys[k] = i
+    assert pair is not None
+    hits[pair[0]].rt_hit_count = 0xFFFFFFFF
+    b = cpp.RawCandidateBuffer()
+    b.hits = hits
+    with pytest.raises(ValueError, match="rt_hit_count overflow during compaction"):
+        cpp.compact_raw_candidates(s, b)
