@@ -111,6 +111,223 @@ def probe(pipe, steps: int) -> int:
     return 0
 
 
+FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 (dense)
+STPF_FLOPS_ROW = 2 * (32 * 128 + 128 * 128 + 128)      # three affine layers, multiply-add counted as 2
+TINY_FLOPS_ROW = 2 * (32 * 64 + 64 * 32 + 32) + 2 * 32  # TinySTPF MLP + z-score
+
+
+def _gpu_ms(fn, repeats: int = 5) -> float:
+    """Best-of-`repeats` device time of fn() on the current stream (HIP events)."""
+    import torch
+
+    best = float("inf")
+    for _ in range(repeats):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        fn()
+        e1.record()
+        torch.cuda.synchronize()
+        best = min(best, e0.elapsed_time(e1))
+    return best
+
+
+def _cpu_s(fn) -> float:
+    t = time.perf_counter()
+    fn()
+    return time.perf_counter() - t
+
+
+def _row(name, workload, gpu, cpu, roofline=None, parity=None, extra=None):
+    line = {"row": name, "workload": workload, "gpu": gpu, "roofline": roofline, "cpu_baseline": cpu,
+            "parity": parity}
+    line.update(extra or {})
+    print(json.dumps(line), flush=True)
+
+
+def rows_suite() -> int:
+    """One JSON line per hot-path row: the GPU entry point on its workload,
+    its roofline where the work has a closed-form count, and the reference's
+    CPU path (compiled in place where it builds, else the oracle restatement)
+    timed on this host, one thread, with a parity check of the two outputs."""
+    import numpy as np
+    import torch
+
+    import oracle
+    from p2cccd import _native as nat
+    from p2cccd import proxy, scenes, serve, stpf, workloads
+    from p2cccd.pipeline import CcdPipeline
+    from p2cccd.tuv import TuvCertifier
+
+    dev = torch.device("cuda:0")
+    qs = workloads.make_query_set(10_000, 10_000)
+
+    # (t,u,v) certificate on the plain query set (10k VF + 10k EE)
+    log("rows: (t,u,v) certificate, 10k VF + 10k EE queries")
+    cert = TuvCertifier(device=dev)
+    for kind, x in (("vf", qs.vf), ("ee", qs.ee)):
+        xd = torch.from_numpy(np.ascontiguousarray(x)).to(dev)
+        res = cert.certify_positions(xd, kind == "ee")
+        n = xd.shape[0]
+        ms = {m: _gpu_ms(lambda m=m: (cert.setup_positions(xd, kind == "ee"), cert.certify(n, m)))
+              for m in ("stream", "plain")}
+        boxes = int(res["boxes"].sum())
+        flops = boxes * (FLOP_BOX_VF if kind == "vf" else FLOP_BOX_EE)
+        ref = [None]
+        t_cpu = _cpu_s(lambda: ref.__setitem__(0, oracle.tuv_cert(kind == "ee", x, threads=1)))
+        hit = res["hit"] != 0
+        same = bool(np.array_equal(hit, ref[0]["hit"] != 0) and np.array_equal(res["toi"][hit], ref[0]["toi"][hit]))
+        ach = flops / (ms["stream"] * 1e-3) / 1e12
+        _row(f"tuv_certificate_{kind}", f"plain query set, {n} {kind.upper()} queries (tolerance 1e-6, ms 0)",
+             {"value": n / (ms["stream"] * 1e-3), "unit": "queries/s", "ms": ms["stream"],
+              "plain_form_ms": ms["plain"], "boxes": boxes},
+             {"value": n / t_cpu, "unit": "queries/s", "cores": 1, "kind": "port",
+              "sample": f"all {n} queries through the oracle's (t,u,v) restatement"},
+             {"bound": "fp64-valu", "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+              "frac": ach / FP64_PEAK_TFLOPS, "traffic": None},
+             {"identical_hits_and_toi": same})
+
+    # reference interval certificate (CertificateEngine semantics)
+    log("rows: reference interval certificate")
+    cfg = nat.cert_config()
+    for fam, kind, x in ((1, "point_triangle", qs.vf), (2, "edge_edge", qs.ee)):
+        prims = nat.prims_from_positions(x)
+        d_in = torch.from_numpy(prims.view(np.uint8).copy()).to(dev)
+        d_out = torch.zeros(len(prims) * nat.PRIM_RESULT_DTYPE.itemsize, dtype=torch.uint8, device=dev)
+
+        def run():
+            nat.call("p2c_cert_ref_batch", fam, d_in.data_ptr(), len(prims), 0.0, 1.0, nat.ptr(cfg),
+                     d_out.data_ptr(), None, nat.stream_handle())
+        run()
+        torch.cuda.synchronize()
+        ms = _gpu_ms(run)
+        got = d_out.cpu().numpy().view(nat.PRIM_RESULT_DTYPE)
+        compiled = oracle.ref_available()
+        f = oracle.ref_cert_compiled if compiled else oracle.ref_cert
+        ref = [None]
+        t_cpu = _cpu_s(lambda: ref.__setitem__(0, f(fam, oracle.prims_from_positions(x))))
+        # the compiled reference does not count subdivision nodes; every other common field must match
+        fields = [k for k in nat.PRIM_RESULT_DTYPE.names if k in ref[0].dtype.names and not (compiled and k == "nodes")]
+        same = all(np.array_equal(got[k].astype(np.float64), ref[0][k].astype(np.float64)) for k in fields)
+        _row(f"reference_certificate_{kind}", f"plain query set, {len(prims)} primitives, eps_time 1e-4, depth 32",
+             {"value": len(prims) / (ms * 1e-3), "unit": "primitives/s", "ms": ms,
+              "nodes": int(got["nodes"].sum())},
+             {"value": len(prims) / t_cpu, "unit": "primitives/s", "cores": 1,
+              "kind": "reference" if compiled else "port",
+              "sample": "all primitives through " + ("the compiled certificate_engine.cpp" if compiled else
+                                                     "the oracle restatement")},
+             None, {"bit_exact": same})
+
+    # full plain pipeline on two level-6 icospheres
+    log("rows: two level-6 icospheres, full step")
+    sc = scenes.two_icospheres()
+    pipe = CcdPipeline(sc, device=dev)
+    pipe.step_batched()
+    lat = pipe.timed(pipe.step_batched, 10)
+    pipe.step_plain()
+    plain = pipe.timed(pipe.step_plain, 3)
+    npairs = pipe.last.pairs
+    cpu = cpu_baseline(sc, 100_000)
+    _row("icospheres_step", f"{sc.name}: {npairs} candidates per step",
+         {"value": npairs / (np.mean(lat) * 1e-3), "unit": "pairs/s", "ms_per_step": float(np.mean(lat)),
+          "p99_ms": percentile(lat, 99), "plain_ms_per_step": float(np.mean(plain))}, cpu, None,
+         "covered by tests/test_pipeline_gpu.py (identical decisions and TOI against the oracle)")
+
+    # learned group scheduling: the two scorer kernels
+    log("rows: scorer kernels")
+    packed = stpf.load_stpf(stpf.DEFAULT_PRIORITY_WEIGHTS, dev)
+    rng = np.random.default_rng(3)
+    feats = torch.from_numpy(rng.normal(size=(65536, 32)).astype(np.float32)).to(dev)
+    ms = _gpu_ms(lambda: stpf.stpf_priority(packed, feats))
+    model = stpf.StpfPriority()
+    p = packed.cpu()
+    o = 0
+    for prm in (model.l1.weight, model.l1.bias, model.l2.weight, model.l2.bias, model.head.weight, model.head.bias):
+        prm.data.copy_(p[o:o + prm.numel()].view_as(prm))
+        o += prm.numel()
+    torch.set_num_threads(1)
+    fc = feats.cpu()
+    with torch.no_grad():
+        want = model(fc)
+        t_cpu = _cpu_s(lambda: model(fc))
+    got = stpf.stpf_priority(packed, feats).cpu()
+    ach = 65536 * STPF_FLOPS_ROW / (ms * 1e-3) / 1e12
+    _row("stpf_priority_scorer", "65,536 groups x 32 FP32 features, 32-128-128-1 GELU, sigmoid",
+         {"value": 65536 / (ms * 1e-3), "unit": "groups/s", "ms": ms},
+         {"value": 65536 / t_cpu, "unit": "groups/s", "cores": 1, "kind": "port",
+          "sample": "the same model in torch on the CPU, FP32, one thread"},
+         {"bound": "fp32", "achieved": ach, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+          "frac": ach / FP32_PEAK_TFLOPS, "traffic": None},
+         {"max_abs_diff_vs_torch_fp32": float((got - want).abs().max())})
+    bp = pipe.bp
+    w = torch.from_numpy(rng.normal(0, 0.2, stpf.TINY_STPF_WEIGHTS)).to(dev)
+    w[32:64] = 1.0  # std slots
+    t = sc.topo
+    x0, x1 = pipe.x0, pipe.x1
+    vf = pipe.vf[: pipe.last.n_vf]
+    n = int(vf.shape[0])
+    topo = torch.from_numpy(np.ascontiguousarray(t.faces)).to(dev)
+    oa = torch.from_numpy(np.ascontiguousarray(t.vertex_object)).to(dev)
+    ob = torch.from_numpy(np.ascontiguousarray(t.face_object)).to(dev)
+    ms = _gpu_ms(lambda: stpf.tiny_scores("vf", x0, x1, topo, vf, n, oa, ob, "learned", w))
+    sample = rng.choice(n, size=min(n, 200_000), replace=False)
+    pv = vf.cpu().numpy()[sample]
+    vid = np.concatenate([pv[:, :1], t.faces[pv[:, 1]]], axis=1)
+    rows = np.concatenate([sc.x0[vid], sc.x1[vid]], axis=1)
+    keys = np.zeros(len(sample), np.uint64)
+    t_cpu = _cpu_s(lambda: oracle.tiny_scores(0, True, rows, sample.astype(np.uint64), keys, w.cpu().numpy()))
+    ach = n * TINY_FLOPS_ROW / (ms * 1e-3) / 1e12
+    _row("tiny_stpf_scores", f"{n} VF candidates of the icosphere step, learned policy (FP64)",
+         {"value": n / (ms * 1e-3), "unit": "candidates/s", "ms": ms},
+         {"value": len(sample) / t_cpu, "unit": "candidates/s", "cores": 1, "kind": "port",
+          "sample": f"{len(sample)} sampled candidates through the oracle's restatement"},
+         {"bound": "fp64-valu", "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+          "frac": ach / FP64_PEAK_TFLOPS, "traffic": None},
+         "covered by tests/test_scorer_gpu.py")
+    del pipe
+
+    # proxy-scene raw candidates + compaction
+    log("rows: proxy scene")
+    s = oracle.random_proxy_scene(np.random.default_rng(31), 20_000, 8, 4, extent=1.0, size=0.08)
+    arr = proxy._device_arrays(s, dev)
+    holder = {}
+
+    def gen():
+        raw = proxy.raw_hits(91, arr)
+        holder["raw"] = raw
+        holder["cand"] = proxy.compact(91, arr, raw)
+    gen()
+    ms = _gpu_ms(gen, 3)
+    compiled = oracle.ref_available()
+    f = oracle.ref_proxy_generate if compiled else oracle.proxy_generate
+    ref = [None]
+    t_cpu = _cpu_s(lambda: ref.__setitem__(0, f(91, s)))
+    r_raw, r_cand = ref[0][0], ref[0][1]
+    same = len(r_raw) == holder["raw"].numel() // 64 and len(r_cand) == len(holder["cand"]) and bool(
+        np.array_equal(holder["cand"]["candidate_id"], r_cand["candidate_id"].astype(np.uint64)))
+    npair = 20_000 * 19_999 // 2
+    _row("proxy_scene_candidates", f"20,000 proxies, 8 objects, 4 slabs: {len(r_raw)} raw hits, "
+         f"{len(r_cand)} candidates",
+         {"value": npair / (ms * 1e-3), "unit": "pair tests/s", "ms": ms},
+         {"value": npair / t_cpu, "unit": "pair tests/s", "cores": 1, "kind": "reference" if compiled else "port",
+          "sample": "the whole scene through " + ("the compiled candidate_buffer.cpp" if compiled else "the oracle")},
+         None, {"identical_counts_and_ids": same})
+
+    # serving: Poisson requests over 256 resident scenes
+    log("rows: serving")
+    table, reqs = serve.poisson_workload(256, 600, 150.0)
+    srv = serve.Server(table, int(64 * (1 << 30)))
+    warm = [serve.step_request(-1 - i, sid, table[sid], 99 + i) for i, sid in enumerate(sorted(table))]
+    srv.run(warm, realtime=False)
+    t0 = time.perf_counter()
+    done = srv.run(reqs, realtime=True)
+    rep = serve.report(done, time.perf_counter() - t0)
+    _row("serving", "256 resident scenes (1k-500k triangles), 600 Poisson requests at 150 Hz",
+         {"value": rep["pairs_per_s"], "unit": "pairs/s", "p50_ms": rep["p50_ms"], "p99_ms": rep["p99_ms"],
+          "requests": rep["requests"]}, None, None,
+         "covered by tests/test_serve_gpu.py (each result identical alone and in the mix)")
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -124,6 +341,9 @@ def main() -> int:
     ap.add_argument("--probe", type=int, default=0,
                     help="profiling probe: run this many batched steps outside the graph, then a 1 GiB device copy "
                          "(the calibration kernel for HBM byte counters); prints the stream-kernel times")
+    ap.add_argument("--rows", action="store_true",
+                    help="per-row suite: each hot-path entry point on its workload beside the reference CPU path "
+                         "(one JSON line per row)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -134,6 +354,9 @@ def main() -> int:
 
     from p2cccd import scenes
     from p2cccd.pipeline import CcdPipeline
+
+    if args.rows:
+        return rows_suite()
 
     dev = torch.device("cuda:0")
     log("building scene")
