@@ -231,6 +231,13 @@ typedef struct p2c_bp_mesh {
   int64_t n_vertices, n_faces, n_edges;
   int32_t n_objects;
   int32_t reserved;
+  /* Optional scratch kept between p2c_bp_count and p2c_bp_emit: int32
+   * [(V + E) * hit_cache_slots].  The count pass keeps each query's first
+   * hit_cache_slots partners there and the fill copies them, traversing
+   * again only for queries with more; NULL (or 0 slots): the fill traverses
+   * every query.  Output is the same either way. */
+  int32_t* hit_cache;
+  int64_t hit_cache_slots;
 } p2c_bp_mesh;
 
 size_t p2c_bp_workspace_bytes(int64_t n_vertices, int64_t n_faces, int64_t n_edges, int32_t n_objects);

@@ -21,9 +21,10 @@
 //      escape indices for stackless traversal; bottom-up FP64 refit with
 //      acquire/release counters
 //   5. stackless traversal, one lane per query in Morton order: count
-//      (keeping each query's first kHitCache partners), exclusive scan,
-//      fill (deterministic output order): queries with at most kHitCache
-//      hits are copied from the cache, only the others traverse again.
+//      (keeping each query's first hit_cache_slots partners in the caller's
+//      cache), exclusive scan, fill (deterministic output order): queries
+//      with at most that many hits are copied from the cache, only the
+//      others traverse again.
 //      An edge query visits only the leaves after its own, so each
 //      unordered EE pair is found once.
 #include <cmath>
@@ -37,7 +38,6 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kMaxObjects = 64;
-constexpr int kHitCache = 64;  // partners kept per query by the count pass
 
 struct Box6 {
   double mn[3];
@@ -87,7 +87,7 @@ __device__ inline bool overlap(const Box6& a, const double* mn, const double* mx
 // ---- workspace ----------------------------------------------------------
 struct Layout {
   size_t vbox, fbox, ebox, objkey, allowed, keys0, vals0, keys1, vals1, hist, fnodes, fparent, fflag, frcs,
-      fhi, flo, enodes, eparent, eflag, ercs, ehi, elo, qorder, eorder, counts, cache, offsets, hist_off, scan_part, err, total;
+      fhi, flo, enodes, eparent, eflag, ercs, ehi, elo, qorder, eorder, counts, offsets, hist_off, scan_part, err, total;
   int64_t sort_cap, hist_blocks;
 };
 
@@ -129,7 +129,6 @@ Layout make_layout(int64_t nv, int64_t nf, int64_t ne, int32_t nobj) {
   L.qorder = take(4 * std::max<int64_t>(nv, 1));
   L.eorder = take(4 * std::max<int64_t>(ne, 1));
   L.counts = take(4 * (nv + ne + 1));
-  L.cache = take(4 * size_t(kHitCache) * size_t(nv + ne));
   L.offsets = take(8 * (nv + ne + 1));
   L.hist_off = take(8 * 256 * L.hist_blocks);
   const int64_t scan_items = std::max<int64_t>(nv + ne + 1, 256 * L.hist_blocks);
@@ -445,11 +444,11 @@ __global__ __launch_bounds__(kBlock) void vf_query_kernel(const Node* __restrict
                                                           const Box6* __restrict__ vbox,
                                                           const uint32_t* __restrict__ qorder, int64_t nv, VfRule r,
                                                           uint32_t* __restrict__ err, uint32_t* __restrict__ counts,
-                                                          int32_t* __restrict__ cache,
+                                                          int32_t* __restrict__ cache, int64_t slots,
                                                           const uint64_t* __restrict__ offsets,
                                                           int32_t* __restrict__ out, int64_t cap) {
   for (int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; s < nv; s += int64_t(gridDim.x) * blockDim.x) {
-    if (kFill && offsets[s + 1] - offsets[s] <= uint64_t(kHitCache)) continue;  // copied from the cache
+    if (kFill && offsets[s + 1] - offsets[s] <= uint64_t(slots)) continue;  // copied from the cache
     const int32_t v = int32_t(qorder[s]);
     const int32_t ov = r.vobj[v];
     uint32_t c = 0;
@@ -468,7 +467,7 @@ __global__ __launch_bounds__(kBlock) void vf_query_kernel(const Node* __restrict
             return true;
           },
           [&](int32_t f, uint32_t k) {
-            if (!kFill && k < uint32_t(kHitCache)) cache[s * kHitCache + k] = f;
+            if (!kFill && int64_t(k) < slots) cache[s * slots + k] = f;
             if (kFill && base + k < uint64_t(cap)) {
               out[(base + k) * 2] = v;
               out[(base + k) * 2 + 1] = f;
@@ -484,10 +483,10 @@ __global__ __launch_bounds__(kBlock) void ee_query_kernel(const Node* __restrict
                                                           const Box6* __restrict__ ebox, const uint32_t* __restrict__ eorder,
                                                           EeRule r, uint32_t* __restrict__ err,
                                                           uint32_t* __restrict__ counts, int32_t* __restrict__ cache,
-                                                          const uint64_t* __restrict__ offsets, int64_t nv,
-                                                          int32_t* __restrict__ out, int64_t cap) {
+                                                          int64_t slots, const uint64_t* __restrict__ offsets,
+                                                          int64_t nv, int32_t* __restrict__ out, int64_t cap) {
   for (int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; s < ne; s += int64_t(gridDim.x) * blockDim.x) {
-    if (kFill && offsets[nv + s + 1] - offsets[nv + s] <= uint64_t(kHitCache)) continue;  // copied from the cache
+    if (kFill && offsets[nv + s + 1] - offsets[nv + s] <= uint64_t(slots)) continue;  // copied from the cache
     const int32_t i = int32_t(eorder[s]);
     const Box6 q = ebox[i];
     const int32_t oi = r.eobj[i];
@@ -508,7 +507,7 @@ __global__ __launch_bounds__(kBlock) void ee_query_kernel(const Node* __restrict
           return true;
         },
         [&](int32_t j, uint32_t k) {
-          if (!kFill && k < uint32_t(kHitCache)) cache[s * kHitCache + k] = j;
+          if (!kFill && int64_t(k) < slots) cache[s * slots + k] = j;
           if (kFill && base + k < uint64_t(cap)) {
             out[(base + k) * 2] = i < j ? i : j;
             out[(base + k) * 2 + 1] = i < j ? j : i;
@@ -518,21 +517,21 @@ __global__ __launch_bounds__(kBlock) void ee_query_kernel(const Node* __restrict
   }
 }
 
-// Fill for queries with at most kHitCache hits: one lane per cache slot
+// Fill for queries with at most `slots` hits: one lane per cache slot
 // (query-major, so a wave reads whole cache lines), the pair written at the
 // query's scanned offset.  Vertex queries are s < nv, edge queries follow.
 __global__ __launch_bounds__(kBlock) void cache_copy_kernel(const int32_t* __restrict__ cache,
                                                             const uint64_t* __restrict__ offsets,
                                                             const uint32_t* __restrict__ qorder,
                                                             const uint32_t* __restrict__ eorder, int64_t nv, int64_t ne,
-                                                            int32_t* __restrict__ vf, int64_t vf_cap,
+                                                            int64_t slots, int32_t* __restrict__ vf, int64_t vf_cap,
                                                             int32_t* __restrict__ ee, int64_t ee_cap) {
-  const int64_t slots = (nv + ne) * kHitCache;
-  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < slots; t += int64_t(gridDim.x) * blockDim.x) {
-    const int64_t s = t / kHitCache;
-    const uint32_t k = uint32_t(t % kHitCache);
+  const int64_t total = (nv + ne) * slots;
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < total; t += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t s = t / slots;
+    const uint32_t k = uint32_t(t % slots);
     const uint64_t c = offsets[s + 1] - offsets[s];
-    if (c > uint64_t(kHitCache) || k >= c) continue;
+    if (c > uint64_t(slots) || k >= c) continue;
     const int32_t p = cache[t];
     if (s < nv) {
       const uint64_t o = offsets[s] + k;
@@ -572,6 +571,8 @@ int check_mesh(const p2c_bp_mesh* m) {
     return set_error(P2C_EUNSUPPORTED, "object count must be in [1, 64] per scene");
   if (m->n_faces > (int64_t(1) << 30) || m->n_edges > (int64_t(1) << 30) || m->n_vertices > (int64_t(1) << 30))
     return set_error(P2C_EUNSUPPORTED, "meshes above 2^30 primitives are not supported");
+  if (m->hit_cache_slots < 0 || m->hit_cache_slots > 4096)
+    return set_error(P2C_EINVAL, "hit_cache_slots must be in [0, 4096]");
   return P2C_OK;
 }
 
@@ -660,16 +661,17 @@ int p2c_bp_count(const p2c_bp_mesh* m, void* ws, uint64_t* totals, void* stream)
   const int64_t nv = m->n_vertices, nf = m->n_faces, ne = m->n_edges;
   uint32_t* counts = at<uint32_t>(ws, L.counts);
   uint64_t* offsets = at<uint64_t>(ws, L.offsets);
+  const int64_t slots = m->hit_cache != nullptr ? m->hit_cache_slots : 0;
   VfRule vr{m->faces, m->face_object, m->vertex_object, at<uint8_t>(ws, L.allowed), m->n_objects};
   EeRule er{m->edges, m->edge_object, at<uint8_t>(ws, L.allowed), m->n_objects};
   if (nv > 0)
     hipLaunchKernelGGL(vf_query_kernel<false>, dim3(grid_for(nv, kBlock)), dim3(kBlock), 0, s,
                        at<Node>(ws, L.fnodes), nf, at<Box6>(ws, L.vbox), at<uint32_t>(ws, L.qorder), nv, vr,
-                       at<uint32_t>(ws, L.err), counts, at<int32_t>(ws, L.cache), nullptr, nullptr, 0);
+                       at<uint32_t>(ws, L.err), counts, m->hit_cache, slots, nullptr, nullptr, 0);
   if (ne > 0)
     hipLaunchKernelGGL(ee_query_kernel<false>, dim3(grid_for(ne, kBlock)), dim3(kBlock), 0, s,
                        at<Node>(ws, L.enodes), ne, at<Box6>(ws, L.ebox), at<uint32_t>(ws, L.eorder), er,
-                       at<uint32_t>(ws, L.err), counts + nv, at<int32_t>(ws, L.cache) + nv * kHitCache, nullptr, nv,
+                       at<uint32_t>(ws, L.err), counts + nv, m->hit_cache + nv * slots, slots, nullptr, nv,
                        nullptr, 0);
   if (int rc = exclusive_scan_u32(counts, nv + ne, offsets, at<uint64_t>(ws, L.scan_part), s)) return rc;
   hipLaunchKernelGGL(totals_kernel, dim3(1), dim3(64), 0, s, offsets, nv, ne, at<uint32_t>(ws, L.err), totals);
@@ -685,19 +687,20 @@ int p2c_bp_emit(const p2c_bp_mesh* m, void* ws, int32_t* vf_pairs, int64_t vf_ca
   hipStream_t s = as_stream(stream);
   const int64_t nv = m->n_vertices, nf = m->n_faces, ne = m->n_edges;
   const uint64_t* offsets = at<uint64_t>(ws, L.offsets);
+  const int64_t slots = m->hit_cache != nullptr ? m->hit_cache_slots : 0;
   VfRule vr{m->faces, m->face_object, m->vertex_object, at<uint8_t>(ws, L.allowed), m->n_objects};
   EeRule er{m->edges, m->edge_object, at<uint8_t>(ws, L.allowed), m->n_objects};
   if (nv > 0 && vf_pairs != nullptr)
     hipLaunchKernelGGL(vf_query_kernel<true>, dim3(grid_for(nv, kBlock)), dim3(kBlock), 0, s,
                        at<Node>(ws, L.fnodes), nf, at<Box6>(ws, L.vbox), at<uint32_t>(ws, L.qorder), nv, vr,
-                       at<uint32_t>(ws, L.err), nullptr, nullptr, offsets, vf_pairs, vf_cap);
+                       at<uint32_t>(ws, L.err), nullptr, nullptr, slots, offsets, vf_pairs, vf_cap);
   if (ne > 0 && ee_pairs != nullptr)
     hipLaunchKernelGGL(ee_query_kernel<true>, dim3(grid_for(ne, kBlock)), dim3(kBlock), 0, s,
                        at<Node>(ws, L.enodes), ne, at<Box6>(ws, L.ebox), at<uint32_t>(ws, L.eorder), er,
-                       at<uint32_t>(ws, L.err), nullptr, nullptr, offsets, nv, ee_pairs, ee_cap);
-  if (nv + ne > 0 && (vf_pairs != nullptr || ee_pairs != nullptr))
-    hipLaunchKernelGGL(cache_copy_kernel, dim3(grid_for((nv + ne) * kHitCache, kBlock, 1 << 16)), dim3(kBlock), 0, s,
-                       at<int32_t>(ws, L.cache), offsets, at<uint32_t>(ws, L.qorder), at<uint32_t>(ws, L.eorder), nv, ne,
+                       at<uint32_t>(ws, L.err), nullptr, nullptr, slots, offsets, nv, ee_pairs, ee_cap);
+  if (slots > 0 && nv + ne > 0 && (vf_pairs != nullptr || ee_pairs != nullptr))
+    hipLaunchKernelGGL(cache_copy_kernel, dim3(grid_for((nv + ne) * slots, kBlock, 1 << 16)), dim3(kBlock), 0, s,
+                       m->hit_cache, offsets, at<uint32_t>(ws, L.qorder), at<uint32_t>(ws, L.eorder), nv, ne, slots,
                        vf_pairs, vf_cap, ee_pairs, ee_cap);
   P2C_HIP_CHECK(hipGetLastError());
   return P2C_OK;

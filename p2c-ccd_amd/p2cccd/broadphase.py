@@ -28,6 +28,8 @@ class _Mesh(ctypes.Structure):
         ("n_edges", ctypes.c_int64),
         ("n_objects", ctypes.c_int32),
         ("reserved", ctypes.c_int32),
+        ("hit_cache", ctypes.c_void_p),
+        ("hit_cache_slots", ctypes.c_int64),
     ]
 
 
@@ -39,8 +41,12 @@ nat._sig("p2c_bp_count", [_vp, _vp, _vp, _vp])
 nat._sig("p2c_bp_emit", [_vp, _vp, _vp, _i64, _vp, _i64, _vp])
 
 
+DEFAULT_CACHE_SLOTS = 256  # partners per query kept by the count pass (covers nearly all queries of dense scenes)
+
+
 class BroadPhase:
-    def __init__(self, topo: Topology, n_vertices: int, device: str | torch.device = "cuda"):
+    def __init__(self, topo: Topology, n_vertices: int, device: str | torch.device = "cuda",
+                 cache_slots: int = DEFAULT_CACHE_SLOTS):
         self.device = torch.device(device)
         dev = self.device
 
@@ -50,20 +56,27 @@ class BroadPhase:
         self.faces, self.edges = up(topo.faces), up(topo.edges)
         self.vobj, self.fobj, self.eobj = up(topo.vertex_object), up(topo.face_object), up(topo.edge_object)
         self.n_vertices, self.n_faces, self.n_edges = n_vertices, topo.faces.shape[0], topo.edges.shape[0]
+        self.cache_slots = int(cache_slots)
+        self.cache = torch.empty(max((self.n_vertices + self.n_edges) * self.cache_slots, 1), dtype=torch.int32,
+                                 device=dev)
         self.mesh = _Mesh(self.faces.data_ptr(), self.edges.data_ptr(), self.vobj.data_ptr(), self.fobj.data_ptr(),
-                          self.eobj.data_ptr(), self.n_vertices, self.n_faces, self.n_edges, topo.object_count, 0)
+                          self.eobj.data_ptr(), self.n_vertices, self.n_faces, self.n_edges, topo.object_count, 0,
+                          self.cache.data_ptr() if self.cache_slots else None, self.cache_slots)
         self.ws_bytes = int(nat.lib.p2c_bp_workspace_bytes(self.n_vertices, self.n_faces, self.n_edges,
                                                            topo.object_count))
         self.workspace = torch.empty(self.ws_bytes, dtype=torch.uint8, device=dev)
         self.totals = torch.zeros(3, dtype=torch.int64, device=dev)
 
     @staticmethod
-    def workspace_bytes(n_vertices: int, n_faces: int, n_edges: int, n_objects: int) -> int:
-        return int(nat.lib.p2c_bp_workspace_bytes(n_vertices, n_faces, n_edges, n_objects))
+    def workspace_bytes(n_vertices: int, n_faces: int, n_edges: int, n_objects: int,
+                        cache_slots: int = DEFAULT_CACHE_SLOTS) -> int:
+        """Workspace plus the hit cache."""
+        return int(nat.lib.p2c_bp_workspace_bytes(n_vertices, n_faces, n_edges, n_objects)) + \
+            4 * (n_vertices + n_edges) * cache_slots
 
     @classmethod
-    def for_scene(cls, scene: Scene, device="cuda") -> "BroadPhase":
-        return cls(scene.topo, scene.n_vertices, device)
+    def for_scene(cls, scene: Scene, device="cuda", cache_slots: int = DEFAULT_CACHE_SLOTS) -> "BroadPhase":
+        return cls(scene.topo, scene.n_vertices, device, cache_slots)
 
     def build(self, x0: torch.Tensor, x1: torch.Tensor, thickness: float, stream=None) -> None:
         assert x0.dtype == torch.float64 and x0.is_cuda and x0.shape == (self.n_vertices, 3) and x0.is_contiguous()

@@ -118,11 +118,14 @@ class Residency:
 
 
 # ---- device state -------------------------------------------------------------
+CACHE_SLOTS = 64  # hit-cache partners per query for resident scenes (kept small: one cache per scene)
+
+
 class ResidentScene:
     def __init__(self, scene: scenes.Scene, device, headroom: float = 1.25):
         self.scene = scene
         self.device = device
-        self.bp = BroadPhase.for_scene(scene, device)
+        self.bp = BroadPhase.for_scene(scene, device, CACHE_SLOTS)
         self.x0 = torch.empty((scene.n_vertices, 3), dtype=torch.float64, device=device)
         self.x1 = torch.empty_like(self.x0)
         self.headroom = headroom
@@ -140,12 +143,12 @@ class ResidentScene:
     @staticmethod
     def estimate_bytes(scene: scenes.Scene, pairs_guess: int = 0) -> int:
         nv, nf, ne = scene.n_vertices, len(scene.topo.faces), len(scene.topo.edges)
-        ws = BroadPhase.workspace_bytes(nv, nf, ne, scene.topo.object_count)
+        ws = BroadPhase.workspace_bytes(nv, nf, ne, scene.topo.object_count, CACHE_SLOTS)
         return ws + 2 * nv * 24 + (nf * 12 + ne * 8) + 8 * int(pairs_guess * 1.25)
 
     def nbytes(self) -> int:
-        return self.bp.ws_bytes + 2 * self.x0.numel() * 8 + 8 * (self.cap_vf + self.cap_ee) + \
-            self.bp.faces.numel() * 4 + self.bp.edges.numel() * 4
+        return self.bp.ws_bytes + self.bp.cache.numel() * 4 + 2 * self.x0.numel() * 8 + \
+            8 * (self.cap_vf + self.cap_ee) + self.bp.faces.numel() * 4 + self.bp.edges.numel() * 4
 
 
 @dataclass

@@ -61,3 +61,16 @@ def test_folded_cloth_matches_oracle_and_is_deterministic(gpu):
     assert vf.shape[0] + ee.shape[0] > 10_000_000
     _, (vf2, ee2) = _gpu_sets(s)
     assert torch.equal(vf, vf2) and torch.equal(ee, ee2)
+
+
+@pytest.mark.parametrize("slots", [0, 1, 16])
+def test_hit_cache_size_does_not_change_the_output(gpu, slots):
+    """The count pass's hit cache only saves the fill its traversal: any
+    cache size (none, one partner, some) gives the same pairs in the same
+    order as the default."""
+    s = scenes.two_icospheres(level=4, gap=-0.01)
+    x0, x1 = torch.from_numpy(s.x0).cuda(), torch.from_numpy(s.x1).cuda()
+    ref = BroadPhase.for_scene(s, "cuda").candidates(x0, x1, s.thickness)
+    got = BroadPhase.for_scene(s, "cuda", cache_slots=slots).candidates(x0, x1, s.thickness)
+    assert ref[0].shape[0] > 1000 and ref[1].shape[0] > 1000
+    assert torch.equal(ref[0], got[0]) and torch.equal(ref[1], got[1])
