@@ -129,7 +129,10 @@ int p2c_cert_ref_eval_cpu(int family, const p2c_prim* prim, double interval_t0,
  * library's (FP64 corner enclosure of the multilinear F(t,u,v), forward error
  * bound, toi = least t_lo over done boxes); see the design notes.  Pairs are
  * first expanded into an opaque per-pair workspace of P2C_TUV_PAIR_BYTES
- * each; certification writes one p2c_tuv_result per pair.  A pair whose
+ * each, stored in tiles of 64 pairs: a buffer for n pairs takes
+ * p2c_tuv_pairs_bytes(n) = ceil(n / 64) * 64 * P2C_TUV_PAIR_BYTES bytes (and
+ * a capacity passed to a setup call counts whole tiles); certification
+ * writes one p2c_tuv_result per pair.  A pair whose
  * search exceeds max_itr boxes is appended to the replay list and must be
  * finished with p2c_tuv_replay (the fallback-replay queue). */
 #define P2C_TUV_PAIR_BYTES 288
@@ -146,6 +149,7 @@ typedef struct p2c_tuv_result {
 } p2c_tuv_result;
 
 size_t p2c_tuv_pair_bytes(void);
+size_t p2c_tuv_pairs_bytes(int64_t n);
 size_t p2c_tuv_heap_bytes(int64_t heap_cap);
 
 /* v: device (n, 8, 3) doubles, rows 0-3 the vertices at t=0 (VF: p, f0, f1,

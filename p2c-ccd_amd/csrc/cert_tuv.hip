@@ -30,6 +30,42 @@ static_assert(sizeof(p2c_tuv_result) == 32, "p2c_tuv_result layout");
 
 constexpr int kTuvBlock = 256;
 
+// Pair records are stored in tiles of 64 (AoSoA): the 16-byte granule g of
+// pair p sits at granule (p / 64) * 18 * 64 + g * 64 + p % 64, so the 64
+// lanes of a wave that write (setup) or read (refill) consecutive pairs touch
+// 1 KB of contiguous memory per instruction instead of 64 scattered 16-byte
+// pieces.  A buffer for n pairs holds ceil(n / 64) * 64 records.
+constexpr int kPairTile = 64;
+constexpr int kPairGranules = int(sizeof(TuvPair) / 16);
+static_assert(sizeof(TuvPair) % 16 == 0, "TuvPair is a whole number of 16-byte granules");
+
+struct alignas(16) Granule {
+  uint32_t w[4];
+};
+
+__device__ inline const Granule* pair_granules(const TuvPair* base, int64_t p) {
+  return reinterpret_cast<const Granule*>(base) + (p / kPairTile) * (int64_t(kPairGranules) * kPairTile) +
+         (p % kPairTile);
+}
+
+__device__ inline TuvPair load_pair(const TuvPair* __restrict__ base, int64_t p) {
+  const Granule* src = pair_granules(base, p);
+  Granule g[kPairGranules];
+#pragma unroll
+  for (int i = 0; i < kPairGranules; ++i) g[i] = src[int64_t(i) * kPairTile];
+  TuvPair q;
+  __builtin_memcpy(&q, g, sizeof(TuvPair));
+  return q;
+}
+
+__device__ inline void store_pair(TuvPair* __restrict__ base, int64_t p, const TuvPair& q) {
+  Granule g[kPairGranules];
+  __builtin_memcpy(g, &q, sizeof(TuvPair));
+  Granule* dst = const_cast<Granule*>(pair_granules(base, p));
+#pragma unroll
+  for (int i = 0; i < kPairGranules; ++i) dst[int64_t(i) * kPairTile] = g[i];
+}
+
 __device__ inline void write_result(p2c_tuv_result* out, uint64_t best, uint64_t boxes, uint32_t flags) {
   p2c_tuv_result r;
   r.hit = best != kTuvNoHit ? 1u : 0u;
@@ -49,7 +85,7 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_setup_positions_kernel(int is_e
       for (int k = 0; k < 3; ++k) x[r][k] = v[i * 24 + r * 3 + k];
     TuvPair q;
     tuv_setup(&q, x, is_ee, ms, tol);
-    pairs[i] = q;
+    store_pair(pairs, i, q);
   }
 }
 
@@ -69,7 +105,7 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_setup_indexed_kernel(int is_ee,
     }
     TuvPair q;
     tuv_setup(&q, x, is_ee, ms, tol);
-    pairs[i] = q;
+    store_pair(pairs, i, q);
   }
 }
 
@@ -118,7 +154,7 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_setup_pairs_kernel(int is_ee, c
       }
     TuvPair q;
     tuv_setup(&q, x, is_ee, ms, tol);
-    if (off + i < out_cap) out[off + i] = q;
+    if (off + i < out_cap) store_pair(out, off + i, q);
   }
 }
 
@@ -136,7 +172,7 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_plain_kernel(const TuvPair* __r
                                                               uint32_t replay_cap) {
   const int64_t n = device_count(n_host, totals);
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
-    const TuvPair q = pairs[i];
+    const TuvPair q = load_pair(pairs, i);
     uint64_t best = kTuvNoHit, boxes = 0;
     const bool complete = tuv_walk(q, 0, 0, 0, 0, tol, &best, &boxes, max_itr);
     write_result(&out[i], best, boxes, complete ? 0u : P2C_TUV_FLAG_OVER_BUDGET);
@@ -276,7 +312,7 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_persistent_kernel(const TuvPair
         const int64_t p = (int64_t)(base + rank);
         if (p < n) {
           pair = p;
-          q = pairs[p];
+          q = load_pair(pairs, p);
           start_walk(w);
         } else {
           pair = -2;
@@ -382,7 +418,7 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_round_kernel(const TuvPair* __r
     bool keep = false, over = false;
     if (valid) {
       p = list_in == nullptr ? (uint32_t)j : list_in[j];
-      const TuvPair q = pairs[p];
+      const TuvPair q = load_pair(pairs, p);
       LaneWalk w;
       if (list_in == nullptr) start_walk(w);
       else load_walk(w, states[p]);
@@ -450,7 +486,7 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_stream_kernel(const TuvPair* __
         const int rank = __popcll(idle & lt);
         if (pair < 0 && rank < avail) {
           pair = next + rank;
-          q = pairs[pair];
+          q = load_pair(pairs, pair);
           start_walk(w);
         }
         const int64_t took = avail < __popcll(idle) ? avail : (int64_t)__popcll(idle);
@@ -559,7 +595,7 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_heavy_kernel(const TuvPair* __r
     h = __shfl(h, 0);
     if (h >= total) break;
     const uint32_t p = heavy_list[h];
-    const TuvPair q = pairs[p];
+    const TuvPair q = load_pair(pairs, p);
     LaneWalk w;
     uint32_t root = 0, root_ep = 0;
     bool has = lane == 0;  // the lane holds a walk (standing on a box, or moving to the next)
@@ -796,7 +832,7 @@ __global__ __launch_bounds__(64) void tuv_replay_kernel(const TuvPair* __restric
   if (m > list_cap) m = list_cap;
   for (int64_t j = slot; j < (int64_t)m; j += slots) {
     const uint32_t p = list[j];
-    const TuvPair q = pairs[p];
+    const TuvPair q = load_pair(pairs, p);
     uint64_t best, boxes;
     bool capped;
     tuv_canonical(q, tol, max_itr, heaps + slot * heap_cap, heap_cap, &best, &boxes, &capped);
@@ -818,6 +854,12 @@ int check_tuv(double ms, double tol, double t_max, int64_t max_itr) {
 extern "C" {
 
 size_t p2c_tuv_pair_bytes(void) { return sizeof(p2c::TuvPair); }
+
+size_t p2c_tuv_pairs_bytes(int64_t n) {
+  if (n < 0) n = 0;
+  const int64_t tiles = (n + p2c::kPairTile - 1) / p2c::kPairTile;
+  return size_t(tiles) * p2c::kPairTile * sizeof(p2c::TuvPair);
+}
 
 size_t p2c_tuv_heap_bytes(int64_t heap_cap) { return sizeof(p2c::HeapBox) * size_t(heap_cap); }
 

@@ -142,50 +142,100 @@ P2C_TUV_HD int tuv_level_dim(const TuvPair& q, uint32_t level) {
   return (int)((word >> ((level & 31u) * 2u)) & 3ull);
 }
 
+// Appends dimension d at `level` to the packed sequence (one register word,
+// placed into w0..w4 when it fills).
+P2C_TUV_HD void split_append(uint64_t& cur, uint32_t& level, int d, uint64_t& w0, uint64_t& w1, uint64_t& w2,
+                             uint64_t& w3, uint64_t& w4) {
+  const uint32_t sh = (level & 31u) * 2u;
+  cur = (cur & ~(3ull << sh)) | ((uint64_t)d << sh);
+  ++level;
+  if ((level & 31u) == 0) {
+    const uint32_t wi = (level >> 5) - 1;
+    w0 = wi == 0 ? cur : w0;
+    w1 = wi == 1 ? cur : w1;
+    w2 = wi == 2 ? cur : w2;
+    w3 = wi == 3 ? cur : w3;
+    w4 = wi == 4 ? cur : w4;
+    cur = ~0ull;
+  }
+}
+
 // The split sequence of tuv_split_dim, 2 bits per level (3 = no split),
 // and its length.  Each level splits the dimension with the largest
 // r_k = 2^-e_k * inv_tol_k above 1 (e_k < kTuvMaxExp; the first such k on
-// ties).  r_k is carried as a running value halved at each split of k: the
-// same double as the ldexp product while it is normal, and once below 1 it
-// never splits again, so the sequence equals the level-by-level
-// tuv_split_dim replay.  Levels are packed into one register word and a
-// full word is placed once per 32 levels.
+// ties).  Writing inv_tol_k = m_k 2^E_k (m_k in [1, 2)), r_k > 1 holds for
+// e_k < E_k and, when m_k > 1, for e_k = E_k: dimension k splits
+// n_k = min(52, E_k + [m_k > 1]) times, its j-th split ranking by
+// (E_k - j, m_k, -k).  So the sequence runs in rounds v = E_max, E_max - 1,
+// ...: in round v every dimension with a split of rank v, ordered by m
+// descending, then k.  Zero and normal inputs take the rounds; inf, NaN and
+// subnormal inputs replay the rule level by level with running ratios
+// (halving r_k is exact while it is normal, and below 1 it never splits).
+// The host check tests/native/split_sequence_check.hip compares this with
+// the level-by-level rule.
 P2C_TUV_HD void tuv_split_sequence(const double inv_tol[3], uint64_t split[5], uint32_t* done_level) {
   uint64_t w0 = ~0ull, w1 = ~0ull, w2 = ~0ull, w3 = ~0ull, w4 = ~0ull;
-  double r0 = inv_tol[0], r1 = inv_tol[1], r2 = inv_tol[2];
-  int e0 = 0, e1 = 0, e2 = 0;
   uint64_t cur = ~0ull;
   uint32_t level = 0;
-  for (;;) {
-    int d = 3;
-    double br = 1.0;
-    if (e0 < kTuvMaxExp && r0 > br) {
-      br = r0;
-      d = 0;
+  bool regular = true;
+  int E[3], n[3];
+  uint64_t mant[3];
+  for (int k = 0; k < 3; ++k) {
+    const uint64_t bits = __builtin_bit_cast(uint64_t, inv_tol[k]);
+    const int bexp = (int)((bits >> 52) & 0x7FF);
+    mant[k] = bits & ((1ull << 52) - 1ull);
+    E[k] = bexp - 1023;
+    const bool zero = (bits << 1) == 0;
+    regular = regular && (zero || (bexp != 0 && bexp != 0x7FF)) && !(bits >> 63);
+    int c = zero ? 0 : E[k] + (mant[k] != 0 ? 1 : 0);
+    c = c < 0 ? 0 : (c > kTuvMaxExp ? kTuvMaxExp : c);
+    n[k] = c;
+  }
+  if (regular) {
+    // order: m descending, then k (a three-element sorting network)
+    int o0 = 0, o1 = 1, o2 = 2;
+    auto before = [&](int a, int b) { return mant[a] > mant[b] || (mant[a] == mant[b] && a < b); };
+    if (before(o1, o0)) { const int t = o0; o0 = o1; o1 = t; }
+    if (before(o2, o1)) { const int t = o1; o1 = o2; o2 = t; }
+    if (before(o1, o0)) { const int t = o0; o0 = o1; o1 = t; }
+    const int ord[3] = {o0, o1, o2};
+    int vmax = -100000, vmin = 100000;
+    for (int k = 0; k < 3; ++k) {
+      if (n[k] == 0) continue;
+      vmax = E[k] > vmax ? E[k] : vmax;
+      vmin = E[k] - n[k] + 1 < vmin ? E[k] - n[k] + 1 : vmin;
     }
-    if (e1 < kTuvMaxExp && r1 > br) {
-      br = r1;
-      d = 1;
+    for (int v = vmax; v >= vmin; --v) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        const int k = ord[i];
+        const int j = get3(E, k) - v;
+        if (j >= 0 && j < get3(n, k)) split_append(cur, level, k, w0, w1, w2, w3, w4);
+      }
     }
-    if (e2 < kTuvMaxExp && r2 > br) d = 2;
-    if (d == 3) break;
-    r0 = d == 0 ? r0 * 0.5 : r0;
-    r1 = d == 1 ? r1 * 0.5 : r1;
-    r2 = d == 2 ? r2 * 0.5 : r2;
-    e0 += d == 0;
-    e1 += d == 1;
-    e2 += d == 2;
-    const uint32_t sh = (level & 31u) * 2u;
-    cur = (cur & ~(3ull << sh)) | ((uint64_t)d << sh);
-    ++level;
-    if ((level & 31u) == 0) {
-      const uint32_t wi = (level >> 5) - 1;
-      w0 = wi == 0 ? cur : w0;
-      w1 = wi == 1 ? cur : w1;
-      w2 = wi == 2 ? cur : w2;
-      w3 = wi == 3 ? cur : w3;
-      w4 = wi == 4 ? cur : w4;
-      cur = ~0ull;
+  } else {
+    double r0 = inv_tol[0], r1 = inv_tol[1], r2 = inv_tol[2];
+    int e0 = 0, e1 = 0, e2 = 0;
+    for (;;) {
+      int d = 3;
+      double br = 1.0;
+      if (e0 < kTuvMaxExp && r0 > br) {
+        br = r0;
+        d = 0;
+      }
+      if (e1 < kTuvMaxExp && r1 > br) {
+        br = r1;
+        d = 1;
+      }
+      if (e2 < kTuvMaxExp && r2 > br) d = 2;
+      if (d == 3) break;
+      r0 = d == 0 ? r0 * 0.5 : r0;
+      r1 = d == 1 ? r1 * 0.5 : r1;
+      r2 = d == 2 ? r2 * 0.5 : r2;
+      e0 += d == 0;
+      e1 += d == 1;
+      e2 += d == 2;
+      split_append(cur, level, d, w0, w1, w2, w3, w4);
     }
   }
   if ((level & 31u) != 0) {

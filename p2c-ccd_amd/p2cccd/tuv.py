@@ -44,6 +44,8 @@ nat._sig("p2c_tuv_rounds", [_vp, _i64, _vp, _f64, _i64, _vp, _i32, _vp, _vp, _vp
 nat.lib.p2c_tuv_state_bytes.restype = nat.ctypes.c_size_t
 nat._sig("p2c_tuv_replay", [_vp, _vp, _vp, _u32, _f64, _i64, _vp, _i64, _i64, _vp, _vp])
 nat.lib.p2c_tuv_pair_bytes.restype = nat.ctypes.c_size_t
+nat.lib.p2c_tuv_pairs_bytes.argtypes = [nat._i64]
+nat.lib.p2c_tuv_pairs_bytes.restype = nat.ctypes.c_size_t
 assert nat.lib.p2c_tuv_pair_bytes() == PAIR_BYTES
 
 
@@ -88,8 +90,8 @@ class TuvCertifier:
     def _reserve(self, n: int) -> None:
         if n <= self._cap:
             return
-        cap = max(n, 1024)
-        self.pairs = torch.empty(cap * PAIR_BYTES, dtype=torch.uint8, device=self.device)
+        cap = (max(n, 1024) + 63) // 64 * 64  # pair records are stored in tiles of 64
+        self.pairs = torch.empty(int(nat.lib.p2c_tuv_pairs_bytes(cap)), dtype=torch.uint8, device=self.device)
         self.results = torch.empty(cap * TUV_RESULT_DTYPE.itemsize, dtype=torch.uint8, device=self.device)
         self.replay_list = torch.empty(cap, dtype=torch.int32, device=self.device)
         self.heavy_list = torch.empty(cap, dtype=torch.int32, device=self.device)
