@@ -86,7 +86,7 @@ __device__ inline bool overlap(const Box6& a, const double* mn, const double* mx
 
 // ---- workspace ----------------------------------------------------------
 struct Layout {
-  size_t vbox, fbox, ebox, objkey, allowed, keys0, vals0, keys1, vals1, hist, fnodes, fparent, fflag, frcs,
+  size_t roots, vbox, fbox, ebox, objkey, allowed, keys0, vals0, keys1, vals1, hist, fnodes, fparent, fflag, frcs,
       fhi, flo, enodes, eparent, eflag, ercs, ehi, elo, qorder, eorder, counts, offsets, hist_off, scan_part, err, total;
   int64_t sort_cap, hist_blocks;
 };
@@ -134,6 +134,7 @@ Layout make_layout(int64_t nv, int64_t nf, int64_t ne, int32_t nobj) {
   const int64_t scan_items = std::max<int64_t>(nv + ne + 1, 256 * L.hist_blocks);
   L.scan_part = take(8 * (scan_items / 1024 + 2));
   L.err = take(16);
+  L.roots = take(4 * (size_t(sn) + 2));  // [0] count, then the nodes whose parent spans blocks
   L.total = o;
   return L;
 }
@@ -351,11 +352,17 @@ __global__ __launch_bounds__(kBlock) void leaves_kernel(const uint32_t* __restri
 // blocks use the global flag with agent scope, whose release also publishes
 // the boxes the climbing lane built below it.  The first child to arrive
 // stops; the second sees both children's boxes and builds the parent's.
+// Refit in two launches, so that no step needs coherence across the chip's
+// separate L2s.  1. One block per 256 leaves refits every internal node
+// whose leaf range lies in the block (in-block counters in LDS, workgroup
+// scope); a thread that reaches a node spanning blocks records the child it
+// came from.  2. One workgroup climbs from those children through the nodes
+// that span blocks, with per-node counters at workgroup scope.
 __global__ __launch_bounds__(kBlock) void refit_kernel(int64_t n, Node* __restrict__ nodes,
                                                        const int32_t* __restrict__ parent,
                                                        const int32_t* __restrict__ lo_of,
                                                        const int32_t* __restrict__ hi_of,
-                                                       uint32_t* __restrict__ flag) {
+                                                       uint32_t* __restrict__ roots) {
   __shared__ uint32_t local_flag[kBlock];
   local_flag[threadIdx.x] = 0;
   __syncthreads();
@@ -363,12 +370,16 @@ __global__ __launch_bounds__(kBlock) void refit_kernel(int64_t n, Node* __restri
   const int64_t b1 = (b0 + kBlock < n ? b0 + kBlock : n) - 1;  // last leaf of the block
   const int64_t q = b0 + threadIdx.x;
   if (q >= n) return;
-  int32_t node = parent[n - 1 + q];
+  int32_t child = int32_t(n - 1 + q);
+  int32_t node = parent[child];
   while (node >= 0) {
-    const bool local = lo_of[node] >= b0 && hi_of[node] <= b1;
+    if (!(lo_of[node] >= b0 && hi_of[node] <= b1)) {
+      const uint32_t at = atomicAdd(&roots[0], 1u);
+      roots[2 + at] = uint32_t(child);
+      break;
+    }
     const uint32_t prev =
-        local ? __hip_atomic_fetch_add(&local_flag[node - b0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP)
-              : __hip_atomic_fetch_add(&flag[node], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(&local_flag[node - b0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
     if (prev == 0) break;
     const Node& a = nodes[nodes[node].left];
     const Node& b = nodes[nodes[node].right];
@@ -378,7 +389,33 @@ __global__ __launch_bounds__(kBlock) void refit_kernel(int64_t n, Node* __restri
       nodes[node].mn[k] = bmn < amn ? bmn : amn;
       nodes[node].mx[k] = amx < bmx ? bmx : amx;
     }
+    child = node;
     node = parent[node];
+  }
+}
+
+constexpr int kTopBlock = 1024;
+
+__global__ __launch_bounds__(kTopBlock) void refit_top_kernel(Node* __restrict__ nodes,
+                                                              const int32_t* __restrict__ parent,
+                                                              const uint32_t* __restrict__ roots,
+                                                              uint32_t* __restrict__ flag) {
+  const uint32_t total = roots[0];
+  for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) {
+    int32_t node = parent[roots[2 + i]];
+    while (node >= 0) {
+      const uint32_t prev = __hip_atomic_fetch_add(&flag[node], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (prev == 0) break;
+      const Node& a = nodes[nodes[node].left];
+      const Node& b = nodes[nodes[node].right];
+      for (int k = 0; k < 3; ++k) {
+        const double amn = a.mn[k], bmn = b.mn[k];
+        const double amx = a.mx[k], bmx = b.mx[k];
+        nodes[node].mn[k] = bmn < amn ? bmn : amn;
+        nodes[node].mx[k] = amx < bmx ? bmx : amx;
+      }
+      node = parent[node];
+    }
   }
 }
 
@@ -635,8 +672,11 @@ int p2c_bp_build(const p2c_bp_mesh* m, const double* x0, const double* x1, doubl
     if (t.n > 1) {
       // one block per 256 leaves, exactly (the LDS flags cover the block's leaves)
       if ((t.n + kBlock - 1) / kBlock > (int64_t(1) << 20)) return set_error(P2C_ECAPACITY, "too many primitives");
+      uint32_t* roots = at<uint32_t>(ws, L.roots);
+      hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, roots, 1);
       hipLaunchKernelGGL(refit_kernel, dim3(unsigned((t.n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, t.n, nodes,
-                         parent, at<int32_t>(ws, t.lo), at<int32_t>(ws, t.hi), flag);
+                         parent, at<int32_t>(ws, t.lo), at<int32_t>(ws, t.hi), roots);
+      hipLaunchKernelGGL(refit_top_kernel, dim3(1), dim3(kTopBlock), 0, s, nodes, parent, roots, flag);
     }
     // the edge leaf order (sorted edge ids) is also the EE query order
     if (t.kind == 1)
