@@ -87,7 +87,7 @@ __device__ inline bool overlap(const Box6& a, const double* mn, const double* mx
 // ---- workspace ----------------------------------------------------------
 struct Layout {
   size_t roots, vbox, fbox, ebox, objkey, allowed, keys0, vals0, keys1, vals1, hist, fnodes, fparent, fflag, frcs,
-      fhi, flo, enodes, eparent, eflag, ercs, ehi, elo, qorder, eorder, counts, offsets, hist_off, scan_part, err, total;
+      fhi, flo, enodes, eparent, eflag, ercs, ehi, elo, qorder, eorder, counts, resume, offsets, hist_off, scan_part, err, total;
   int64_t sort_cap, hist_blocks;
 };
 
@@ -129,6 +129,7 @@ Layout make_layout(int64_t nv, int64_t nf, int64_t ne, int32_t nobj) {
   L.qorder = take(4 * std::max<int64_t>(nv, 1));
   L.eorder = take(4 * std::max<int64_t>(ne, 1));
   L.counts = take(4 * (nv + ne + 1));
+  L.resume = take(4 * (nv + ne + 1));
   L.offsets = take(8 * (nv + ne + 1));
   L.hist_off = take(8 * 256 * L.hist_blocks);
   const int64_t scan_items = std::max<int64_t>(nv + ne + 1, 256 * L.hist_blocks);
@@ -444,10 +445,13 @@ __device__ inline bool pair_allowed(const uint8_t* allowed, int32_t nobj, int32_
 // overlaps; returns the number of accepted leaves.
 // A pre-order walk visits each of the 2n-1 nodes at most once; `max_steps`
 // bounds the loop regardless, and an exhausted bound is counted in *err.
+// Starts at node `start` with `count0` hits already counted; emit(prim, k,
+// next) also gets the node the walk continues at, so a walk can be resumed
+// after any hit.
 template <typename Accept, typename Emit>
-__device__ inline uint32_t traverse(const Node* __restrict__ nodes, int32_t start, const Box6& q, int64_t max_steps,
-                                    uint32_t* err, Accept accept, Emit emit) {
-  uint32_t count = 0;
+__device__ inline uint32_t traverse(const Node* __restrict__ nodes, int32_t start, uint32_t count0, const Box6& q,
+                                    int64_t max_steps, uint32_t* err, Accept accept, Emit emit) {
+  uint32_t count = count0;
   int32_t node = start;
   int64_t steps = 0;
   while (node >= 0) {
@@ -462,7 +466,7 @@ __device__ inline uint32_t traverse(const Node* __restrict__ nodes, int32_t star
     if (overlap(q, mn, mx)) {
       if (left < 0) {
         if (accept(prim)) {
-          emit(prim, count);
+          emit(prim, count, escape);
           ++count;
         }
         node = escape;
@@ -482,10 +486,14 @@ __global__ __launch_bounds__(kBlock) void vf_query_kernel(const Node* __restrict
                                                           const uint32_t* __restrict__ qorder, int64_t nv, VfRule r,
                                                           uint32_t* __restrict__ err, uint32_t* __restrict__ counts,
                                                           int32_t* __restrict__ cache, int64_t slots,
+                                                          int32_t* __restrict__ resume,
                                                           const uint64_t* __restrict__ offsets,
                                                           int32_t* __restrict__ out, int64_t cap) {
   for (int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; s < nv; s += int64_t(gridDim.x) * blockDim.x) {
     if (kFill && offsets[s + 1] - offsets[s] <= uint64_t(slots)) continue;  // copied from the cache
+    // a query with more hits resumes after its last cached one
+    const int32_t start = kFill && slots > 0 ? resume[s] : 0;
+    const uint32_t count0 = kFill && slots > 0 ? uint32_t(slots) : 0u;
     const int32_t v = int32_t(qorder[s]);
     const int32_t ov = r.vobj[v];
     uint32_t c = 0;
@@ -493,7 +501,7 @@ __global__ __launch_bounds__(kBlock) void vf_query_kernel(const Node* __restrict
       const Box6 q = vbox[v];
       const uint64_t base = kFill ? offsets[s] : 0;
       c = traverse(
-          fnodes, 0, q, 2 * nf, err,
+          fnodes, start, count0, q, 2 * nf, err,
           [&](int32_t f) {
             const int32_t of = r.fobj[f];
             if (!pair_allowed(r.allowed, r.nobj, ov, of)) return false;
@@ -503,8 +511,11 @@ __global__ __launch_bounds__(kBlock) void vf_query_kernel(const Node* __restrict
             }
             return true;
           },
-          [&](int32_t f, uint32_t k) {
-            if (!kFill && int64_t(k) < slots) cache[s * slots + k] = f;
+          [&](int32_t f, uint32_t k, int32_t next) {
+            if (!kFill && int64_t(k) < slots) {
+              cache[s * slots + k] = f;
+              if (int64_t(k) == slots - 1) resume[s] = next;
+            }
             if (kFill && base + k < uint64_t(cap)) {
               out[(base + k) * 2] = v;
               out[(base + k) * 2 + 1] = f;
@@ -520,7 +531,8 @@ __global__ __launch_bounds__(kBlock) void ee_query_kernel(const Node* __restrict
                                                           const Box6* __restrict__ ebox, const uint32_t* __restrict__ eorder,
                                                           EeRule r, uint32_t* __restrict__ err,
                                                           uint32_t* __restrict__ counts, int32_t* __restrict__ cache,
-                                                          int64_t slots, const uint64_t* __restrict__ offsets,
+                                                          int64_t slots, int32_t* __restrict__ resume,
+                                                          const uint64_t* __restrict__ offsets,
                                                           int64_t nv, int32_t* __restrict__ out, int64_t cap) {
   for (int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; s < ne; s += int64_t(gridDim.x) * blockDim.x) {
     if (kFill && offsets[nv + s + 1] - offsets[nv + s] <= uint64_t(slots)) continue;  // copied from the cache
@@ -532,8 +544,9 @@ __global__ __launch_bounds__(kBlock) void ee_query_kernel(const Node* __restrict
     // Each unordered pair once: only the leaves after this edge's own leaf s
     // (in tree order) are visited -- in the escape-linked pre-order those are
     // exactly the nodes after leaf s -- and the pair is emitted as (min, max).
+    const bool resumed = kFill && slots > 0;  // more hits than cached: resume after the last cached one
     const uint32_t c = traverse(
-        enodes, enodes[ne - 1 + s].escape, q, 2 * ne, err,
+        enodes, resumed ? resume[s] : enodes[ne - 1 + s].escape, resumed ? uint32_t(slots) : 0u, q, 2 * ne, err,
         [&](int32_t j) {
           const int32_t oj = r.eobj[j];
           if (!pair_allowed(r.allowed, r.nobj, oi, oj)) return false;
@@ -543,8 +556,11 @@ __global__ __launch_bounds__(kBlock) void ee_query_kernel(const Node* __restrict
           }
           return true;
         },
-        [&](int32_t j, uint32_t k) {
-          if (!kFill && int64_t(k) < slots) cache[s * slots + k] = j;
+        [&](int32_t j, uint32_t k, int32_t next) {
+          if (!kFill && int64_t(k) < slots) {
+            cache[s * slots + k] = j;
+            if (int64_t(k) == slots - 1) resume[s] = next;
+          }
           if (kFill && base + k < uint64_t(cap)) {
             out[(base + k) * 2] = i < j ? i : j;
             out[(base + k) * 2 + 1] = i < j ? j : i;
@@ -554,7 +570,7 @@ __global__ __launch_bounds__(kBlock) void ee_query_kernel(const Node* __restrict
   }
 }
 
-// Fill for queries with at most `slots` hits: one lane per cache slot
+// Fill of the cached pairs (a query's first `slots`): one lane per cache slot
 // (query-major, so a wave reads whole cache lines), the pair written at the
 // query's scanned offset.  Vertex queries are s < nv, edge queries follow.
 __global__ __launch_bounds__(kBlock) void cache_copy_kernel(const int32_t* __restrict__ cache,
@@ -568,7 +584,7 @@ __global__ __launch_bounds__(kBlock) void cache_copy_kernel(const int32_t* __res
     const int64_t s = t / slots;
     const uint32_t k = uint32_t(t % slots);
     const uint64_t c = offsets[s + 1] - offsets[s];
-    if (c > uint64_t(slots) || k >= c) continue;
+    if (k >= c) continue;  // the first min(c, slots) pairs of every query come from the cache
     const int32_t p = cache[t];
     if (s < nv) {
       const uint64_t o = offsets[s] + k;
@@ -707,12 +723,13 @@ int p2c_bp_count(const p2c_bp_mesh* m, void* ws, uint64_t* totals, void* stream)
   if (nv > 0)
     hipLaunchKernelGGL(vf_query_kernel<false>, dim3(grid_for(nv, kBlock)), dim3(kBlock), 0, s,
                        at<Node>(ws, L.fnodes), nf, at<Box6>(ws, L.vbox), at<uint32_t>(ws, L.qorder), nv, vr,
-                       at<uint32_t>(ws, L.err), counts, m->hit_cache, slots, nullptr, nullptr, 0);
+                       at<uint32_t>(ws, L.err), counts, m->hit_cache, slots, at<int32_t>(ws, L.resume), nullptr,
+                       nullptr, 0);
   if (ne > 0)
     hipLaunchKernelGGL(ee_query_kernel<false>, dim3(grid_for(ne, kBlock)), dim3(kBlock), 0, s,
                        at<Node>(ws, L.enodes), ne, at<Box6>(ws, L.ebox), at<uint32_t>(ws, L.eorder), er,
-                       at<uint32_t>(ws, L.err), counts + nv, m->hit_cache + nv * slots, slots, nullptr, nv,
-                       nullptr, 0);
+                       at<uint32_t>(ws, L.err), counts + nv, m->hit_cache + nv * slots, slots,
+                       at<int32_t>(ws, L.resume) + nv, nullptr, nv, nullptr, 0);
   if (int rc = exclusive_scan_u32(counts, nv + ne, offsets, at<uint64_t>(ws, L.scan_part), s)) return rc;
   hipLaunchKernelGGL(totals_kernel, dim3(1), dim3(64), 0, s, offsets, nv, ne, at<uint32_t>(ws, L.err), totals);
   P2C_HIP_CHECK(hipGetLastError());
@@ -733,11 +750,13 @@ int p2c_bp_emit(const p2c_bp_mesh* m, void* ws, int32_t* vf_pairs, int64_t vf_ca
   if (nv > 0 && vf_pairs != nullptr)
     hipLaunchKernelGGL(vf_query_kernel<true>, dim3(grid_for(nv, kBlock)), dim3(kBlock), 0, s,
                        at<Node>(ws, L.fnodes), nf, at<Box6>(ws, L.vbox), at<uint32_t>(ws, L.qorder), nv, vr,
-                       at<uint32_t>(ws, L.err), nullptr, nullptr, slots, offsets, vf_pairs, vf_cap);
+                       at<uint32_t>(ws, L.err), nullptr, nullptr, slots, at<int32_t>(ws, L.resume), offsets,
+                       vf_pairs, vf_cap);
   if (ne > 0 && ee_pairs != nullptr)
     hipLaunchKernelGGL(ee_query_kernel<true>, dim3(grid_for(ne, kBlock)), dim3(kBlock), 0, s,
                        at<Node>(ws, L.enodes), ne, at<Box6>(ws, L.ebox), at<uint32_t>(ws, L.eorder), er,
-                       at<uint32_t>(ws, L.err), nullptr, nullptr, slots, offsets, nv, ee_pairs, ee_cap);
+                       at<uint32_t>(ws, L.err), nullptr, nullptr, slots, at<int32_t>(ws, L.resume) + nv, offsets,
+                       nv, ee_pairs, ee_cap);
   if (slots > 0 && nv + ne > 0 && (vf_pairs != nullptr || ee_pairs != nullptr))
     hipLaunchKernelGGL(cache_copy_kernel, dim3(grid_for((nv + ne) * slots, kBlock, 1 << 16)), dim3(kBlock), 0, s,
                        m->hit_cache, offsets, at<uint32_t>(ws, L.qorder), at<uint32_t>(ws, L.eorder), nv, ne, slots,
