@@ -51,14 +51,20 @@ class Request:
 
 @dataclass
 class Completed:
+    """A served timestep: the candidate counts, every certified contact (its
+    pair and TOI, in candidate order) and the earliest TOI.  With the
+    server's full_results, also every candidate pair and its hit flag and TOI."""
     rid: int
     scene_id: int
     arrival: float
     done: float
-    vf: np.ndarray
-    ee: np.ndarray
-    hit: np.ndarray      # uint32 per pair, VF then EE
-    toi: np.ndarray      # float64 per pair
+    n_vf: int
+    n_ee: int
+    vf_hits: np.ndarray  # int32 (k, 2): (vertex, face) pairs with a contact
+    vf_toi: np.ndarray   # float64 (k,)
+    ee_hits: np.ndarray  # int32 (m, 2): (edge, edge) pairs with a contact
+    ee_toi: np.ndarray   # float64 (m,)
+    full: dict | None = None  # vf, ee (all candidates), hit, toi (per candidate, VF then EE)
 
     @property
     def latency_ms(self) -> float:
@@ -66,7 +72,12 @@ class Completed:
 
     @property
     def pairs(self) -> int:
-        return len(self.hit)
+        return self.n_vf + self.n_ee
+
+    @property
+    def earliest_toi(self) -> float:
+        t = np.concatenate([self.vf_toi, self.ee_toi])
+        return float(t.min()) if len(t) else float("inf")
 
 
 # ---- admission (host logic only) ------------------------------------------------
@@ -158,8 +169,6 @@ class _Batch:
     counts: list = field(default_factory=list)
     total: int = 0
     event: torch.cuda.Event | None = None
-    host_res: torch.Tensor | None = None
-    host_pairs: list = field(default_factory=list)
 
 
 class Lane:
@@ -167,12 +176,19 @@ class Lane:
         self.stream = torch.cuda.Stream(device)
         self.cert = TuvCertifier(cfg, device)
         self.batch: _Batch | None = None
+        self.stage = torch.empty(0, dtype=torch.float64).pin_memory()  # pinned staging of the batch's positions
+
+    def staging(self, n: int) -> torch.Tensor:
+        if self.stage.numel() < n:
+            self.stage = torch.empty(int(n * 1.25) + 1024, dtype=torch.float64).pin_memory()
+        return self.stage
 
 
 class Server:
     def __init__(self, scene_table: dict[int, scenes.Scene], hbm_budget: int, device="cuda",
                  max_batch_requests: int = 64, max_batch_pairs: int = 1 << 23, cfg: TuvConfig | None = None,
-                 lanes: int = 2):
+                 lanes: int = 2, full_results: bool = False):
+        self.full_results = full_results
         self.table = scene_table
         self.device = torch.device(device)
         self.res = Residency(hbm_budget)
@@ -224,11 +240,20 @@ class Server:
 
     def _launch(self, lane: Lane, reqs: list) -> None:
         b = _Batch(reqs=reqs)
+        # positions go through the lane's pinned staging, so the copies are asynchronous
+        sizes = [2 * self.table[r.scene_id].n_vertices * 3 for r in reqs]
+        stage = lane.staging(sum(sizes))
+        host = stage.numpy()
         with torch.cuda.stream(lane.stream):
-            for r in reqs:
+            at = 0
+            for r, sz in zip(reqs, sizes):
                 rs = self.resident[r.scene_id]
-                rs.x0.copy_(torch.from_numpy(r.x0), non_blocking=False)
-                rs.x1.copy_(torch.from_numpy(r.x1), non_blocking=False)
+                half = sz // 2
+                host[at:at + half] = r.x0.reshape(-1)
+                host[at + half:at + sz] = r.x1.reshape(-1)
+                rs.x0.view(-1).copy_(stage[at:at + half], non_blocking=True)
+                rs.x1.view(-1).copy_(stage[at + half:at + sz], non_blocking=True)
+                at += sz
                 rs.bp.build(rs.x0, rs.x1, rs.scene.thickness, stream=lane.stream)
                 rs.bp.count(stream=lane.stream)
             totals = torch.stack([self.resident[r.scene_id].bp.totals for r in reqs]).to("cpu", non_blocking=True)
@@ -258,24 +283,46 @@ class Server:
                                      stream=lane.stream)
             if off:
                 cert.certify(off, "stream", stream=lane.stream)
-            b.host_res = cert.result_tensor(max(off, 1))[: off * TUV_RESULT_DTYPE.itemsize].to("cpu", non_blocking=True)
-            for r, (n_vf, n_ee) in zip(reqs, b.counts):
-                rs = self.resident[r.scene_id]
-                b.host_pairs.append((rs.vf[:n_vf].to("cpu", non_blocking=True), rs.ee[:n_ee].to("cpu", non_blocking=True)))
             b.event = torch.cuda.Event()
             b.event.record(lane.stream)
         lane.batch = b
         self.batches += 1
 
     def _complete(self, lane: Lane, out: list) -> None:
+        """Reads a finished batch back: only the contacts (pair, TOI) leave
+        the device, gathered there from the per-pair results."""
         b = lane.batch
         b.event.synchronize()
+        with torch.cuda.stream(lane.stream):
+            res = lane.cert.result_tensor(max(b.total, 1)).view(torch.float64).view(-1, 4)[: b.total]
+            hit = (res[:, 2].view(torch.int64) & 0xFFFFFFFF) != 0
+            idx = torch.nonzero(hit).squeeze(1)  # ascending: requests' VF then EE ranges in order
+            toi = res[idx, 0]
+            ends = [e for o, (n_vf, n_ee) in zip(b.offsets, b.counts) for e in (o, o + n_vf, o + n_vf + n_ee)]
+            cut = torch.searchsorted(idx, torch.tensor(ends, dtype=torch.int64, device=idx.device)).cpu().tolist()
+            host = []
+            for k, (r, o, (n_vf, n_ee)) in enumerate(zip(b.reqs, b.offsets, b.counts)):
+                rs = self.resident[r.scene_id]
+                lo, mid, hi = cut[3 * k: 3 * k + 3]
+                part = (rs.vf[idx[lo:mid] - o], toi[lo:mid], rs.ee[idx[mid:hi] - (o + n_vf)], toi[mid:hi])
+                host.append(tuple(t.to("cpu", non_blocking=True) for t in part))
+            full = None
+            if self.full_results:
+                full = [(self.resident[r.scene_id].vf[:n_vf].cpu().numpy(),
+                         self.resident[r.scene_id].ee[:n_ee].cpu().numpy(),
+                         res[o:o + n_vf + n_ee].cpu().numpy()) for r, o, (n_vf, n_ee) in
+                        zip(b.reqs, b.offsets, b.counts)]
+            lane.stream.synchronize()
         now = time.perf_counter()
-        res = b.host_res.numpy().view(TUV_RESULT_DTYPE)
-        for r, o, (n_vf, n_ee), (vf, ee) in zip(b.reqs, b.offsets, b.counts, b.host_pairs):
-            seg = res[o:o + n_vf + n_ee]
-            out.append(Completed(r.rid, r.scene_id, r.arrival, now, vf.numpy(), ee.numpy(), seg["hit"].copy(),
-                                 seg["toi"].copy()))
+        for k, (r, (n_vf, n_ee)) in enumerate(zip(b.reqs, b.counts)):
+            vf_hits, vf_toi, ee_hits, ee_toi = (t.numpy() for t in host[k])
+            extra = None
+            if full is not None:
+                vf, ee, rr = full[k]
+                view = rr.view(TUV_RESULT_DTYPE).reshape(-1)
+                extra = {"vf": vf, "ee": ee, "hit": view["hit"].copy(), "toi": view["toi"].copy()}
+            out.append(Completed(r.rid, r.scene_id, r.arrival, now, n_vf, n_ee, vf_hits, vf_toi, ee_hits, ee_toi,
+                                 extra))
             self.res.busy.discard(r.scene_id)
         lane.batch = None
 
