@@ -40,6 +40,20 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def profiled_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` (read + written) from the committed
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE summary of this workload, scaled
+    by a 1 GiB device copy; None when the summary has no line for it."""
+    f = Path(__file__).resolve().parent / "profiles" / "hbm_traffic_cloth256_step.md"
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        w = line.split()
+        if len(w) == 4 and w[0] == "traffic_bytes_per_launch" and w[1] == kernel:
+            return int(w[2].split("=")[1]) + int(w[3].split("=")[1])
+    return None
+
+
 def percentile(v, q):
     import numpy as np
 
@@ -469,7 +483,8 @@ def main() -> int:
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": profiled_traffic("tuv_stream_kernel") if args.workload == "cloth" else None,
+            "traffic_source": "rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE (profiles/hbm_traffic_cloth256_step.md)",
             "avg_ms": stream_avg_ms,
             "flops_per_launch": flops,
             "boxes_per_launch": stream_vf + stream_ee,

@@ -179,6 +179,28 @@ P2C_HD double ref_motion_radius(const RefPrim& q, double t0, double t1) {
   return ea + eb;
 }
 
+// The same radius with each vertex's |p1 - p0| computed once per primitive:
+// max_disp is (0.5 * len) * |p1 - p0|, so reusing the norm gives the same
+// product bit for bit.
+P2C_HD double disp_n(double half_len, double vn) { return half_len * vn; }
+
+template <int kFamily>
+P2C_HD double ref_motion_radius_n(const double vn[4], double t0, double t1) {
+  const double h = 0.5 * std_max(0.0, t1 - t0);
+  if (kFamily == P2C_FAMILY_POINT_TRIANGLE) {
+    const double pd = disp_n(h, vn[0]);
+    double td = disp_n(h, vn[1]);
+    const double y = disp_n(h, vn[2]);
+    const double z = disp_n(h, vn[3]);
+    if (td < y) td = y;
+    if (td < z) td = z;
+    return pd + td;
+  }
+  const double ea = std_max(disp_n(h, vn[0]), disp_n(h, vn[1]));
+  const double eb = std_max(disp_n(h, vn[2]), disp_n(h, vn[3]));
+  return ea + eb;
+}
+
 P2C_HD bool prim_valid(const p2c_prim& p) {
   if (p.id_a < 0 || p.id_b < 0) return false;
   for (int i = 0; i < 4; ++i)

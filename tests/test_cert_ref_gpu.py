@@ -2,7 +2,9 @@
 TOI, margin, interval, witness, reason and node count (FP64, no contraction),
 on the 20k-query plain workload, the reference's golden vectors, and through
 the p2cccd_cpp host-batch API (src/tests/cpp/test_certificate_cuda.cpp:74-100,
-src/tests/cpp/test_certificate_engine.cpp:220-280)."""
+src/tests/cpp/test_certificate_engine.cpp:220-280).  The deep configurations
+(trees of thousands of nodes) run through the wave pass, whose node counts
+must equal the sequential walk's."""
 import numpy as np
 import pytest
 
@@ -68,7 +70,7 @@ def test_plain_workload_bit_exact_vs_oracle(gpu):
 
 
 @pytest.mark.parametrize("kw", [dict(), dict(t0=0.125, t1=0.875), dict(eps_time=1e-6, eps_space=1e-4, depth=20),
-                                dict(depth=3)])
+                                dict(depth=3), dict(eps_time=1e-8, eps_space=1e-6, depth=40)])
 def test_configs_bit_exact(gpu, kw):
     import torch
 
