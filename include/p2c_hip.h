@@ -289,13 +289,19 @@ typedef struct {
   uint32_t reserved1, reserved2;
 } p2c_raw_hit;
 
-/* Raw hits per primitive i (pairs i < j), into counts[n] (device). */
-int p2c_proxy_raw_count(const p2c_proxy_set* set, uint32_t* counts, void* stream);
+/* Device workspace of the raw-hit count and fill: per (column segment, row)
+ * counts, 4 bytes x n x segments (at most 64 segments). */
+size_t p2c_proxy_raw_workspace_bytes(int64_t n);
+/* Raw hits per primitive i (pairs i < j), into counts[n] (device); the
+ * workspace keeps the per-segment counts for the fill.  Replaces the pair
+ * loop of GenerateRawCandidatesCpu (candidate_buffer.cpp:144-181). */
+int p2c_proxy_raw_count(const p2c_proxy_set* set, uint32_t* counts, void* workspace, size_t workspace_bytes,
+                        void* stream);
 /* Raw hits in the reference's (i, j) order: primitive i's at offsets[i]
- * (device int64, the exclusive scan of the counts); hits at or beyond `cap`
- * are dropped. */
-int p2c_proxy_raw_fill(const p2c_proxy_set* set, const int64_t* offsets, p2c_raw_hit* out, int64_t cap,
-                       void* stream);
+ * (device int64, the exclusive scan of the counts), with the workspace the
+ * count call filled; hits at or beyond `cap` are dropped. */
+int p2c_proxy_raw_fill(const p2c_proxy_set* set, const int64_t* offsets, const void* workspace,
+                       size_t workspace_bytes, p2c_raw_hit* out, int64_t cap, void* stream);
 
 /* CandidateRecord (src/cpp/common/runtime_contracts.h:41-56), 80 bytes with
  * the C++ struct's natural alignment (reserved fields are its padding). */

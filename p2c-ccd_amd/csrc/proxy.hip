@@ -7,10 +7,11 @@
 // (:31-35), flags valid | aabb-overlap and the per-field maximum of the two
 // motion bounds cast to float (:37-46).
 //
-// All pairs are tested (the reference's own complexity), one lane per i,
-// with the j side staged through LDS in tiles of 256 proxies; count, an
-// exclusive scan, then fill at offsets[i] in ascending j, which reproduces
-// the reference's (i, j) loop order exactly.
+// All pairs are tested (the reference's own complexity): one lane per row
+// i, the wave's 64 rows against one segment of the columns j, each column
+// read once per wave with a scalar load; count, an exclusive scan, then fill
+// at offsets[i] in ascending j, which reproduces the reference's (i, j) loop
+// order exactly.
 //
 // Compaction (CompactRawCandidates, candidate_buffer.cpp:183-228) on the
 // device: each raw hit is validated (ValidateRawHit, :66-102) and keyed
@@ -33,78 +34,89 @@
 namespace p2c {
 namespace {
 
-constexpr int kBlock = 256;
 
-struct Tile {
-  uint32_t object[kBlock];
-  uint32_t slab[kBlock];
-  double t0[kBlock];
-  double t1[kBlock];
-  double box[kBlock][6];
+constexpr int kRowWave = 64;            // rows i per wave, one per lane
+constexpr int64_t kMinSegment = 1024;   // columns j per segment, at least
+constexpr int kMaxSegments = 64;        // segments per row, at most
+
+// Columns are cut into at most 64 segments so that a launch has about
+// (n / 64) x (segments) / 2 live waves instead of n / 64 waves whose work
+// shrinks along the triangle.
+struct SegPlan {
+  int64_t len;
+  int count;
 };
 
+inline SegPlan seg_plan(int64_t n) {
+  int64_t len = (n + kMaxSegments - 1) / kMaxSegments;
+  if (len < kMinSegment) len = kMinSegment;
+  int count = int((n + len - 1) / len);
+  if (count < 1) count = 1;
+  return {len, count};
+}
+
+// Raw hits of 64 rows i (one per lane) against the columns j > i of one
+// segment.  Columns are uniform across the wave (scalar loads, one per column
+// for all 64 rows), and each lane meets its row's columns in ascending j, so
+// its hits come out in the reference's buffer order (candidate_buffer.cpp).
+// The count pass stores each (segment, row) count in `seg`; after
+// seg_prefix_kernel has turned them into per-row exclusive prefixes, the fill
+// pass writes row i's hits of segment g from offsets[i] + seg[g][i],
+// dropping slots at or beyond cap.
 template <bool kFill>
-__global__ __launch_bounds__(kBlock) void proxy_pairs_kernel(p2c_proxy_set s, uint32_t* __restrict__ counts,
-                                                             const int64_t* __restrict__ offsets,
-                                                             p2c_raw_hit* __restrict__ out, int64_t cap) {
-  __shared__ Tile tile;
+__global__ __launch_bounds__(kRowWave) void proxy_rows_kernel(p2c_proxy_set s, int64_t seg_len,
+                                                              uint32_t* __restrict__ seg,
+                                                              const int64_t* __restrict__ offsets,
+                                                              p2c_raw_hit* __restrict__ out, int64_t cap) {
   const int64_t n = s.n;
-  const int64_t first = int64_t(blockIdx.x) * kBlock;
+  const int64_t first = int64_t(blockIdx.x) * kRowWave;
   const int64_t i = first + threadIdx.x;
-  const bool live = i < n;
+  const int64_t g = blockIdx.y;
+  const bool row = i < n;
+  const int64_t js = g * seg_len;
+  const int64_t je = js + seg_len < n ? js + seg_len : n;
   uint32_t oi = 0, si = 0;
-  double t0i = 0, t1i = 0, bi[6] = {0, 0, 0, 0, 0, 0}, mi[4] = {0, 0, 0, 0};
-  if (live) {
+  double t0i = 0.0, t1i = 0.0, bi[6] = {0, 0, 0, 0, 0, 0};
+  if (row) {
     oi = s.object_id[i];
     si = s.slab_id[i];
     t0i = s.t0[i];
     t1i = s.t1[i];
 #pragma unroll
     for (int k = 0; k < 6; ++k) bi[k] = s.bounds[6 * i + k];
-    if (kFill)
-#pragma unroll
-      for (int k = 0; k < 4; ++k) mi[k] = s.motion[4 * i + k];
   }
+  int64_t base = 0;
+  if (kFill && row) base = offsets[i] + seg[g * n + i];
   uint32_t found = 0;
-  const int64_t base = kFill && live ? offsets[i] : 0;
-  for (int64_t t = first; t < n; t += kBlock) {
-    __syncthreads();
-    const int64_t j = t + threadIdx.x;
-    if (j < n) {
-      tile.object[threadIdx.x] = s.object_id[j];
-      tile.slab[threadIdx.x] = s.slab_id[j];
-      tile.t0[threadIdx.x] = s.t0[j];
-      tile.t1[threadIdx.x] = s.t1[j];
-#pragma unroll
-      for (int k = 0; k < 6; ++k) tile.box[threadIdx.x][k] = s.bounds[6 * j + k];
-    }
-    __syncthreads();
-    if (!live) continue;
-    const int m = n - t < kBlock ? int(n - t) : kBlock;
-    for (int q = 0; q < m; ++q) {
-      const int64_t jj = t + q;
-      if (jj <= i) continue;
-      if (tile.object[q] == oi || tile.slab[q] != si) continue;
-      if (!(t0i < tile.t1[q] && tile.t0[q] < t1i)) continue;
-      const double* bj = tile.box[q];
-      if (bi[3] < bj[0] || bj[3] < bi[0] || bi[4] < bj[1] || bj[4] < bi[1] || bi[5] < bj[2] || bj[5] < bi[2])
-        continue;
+  const int64_t j_begin = js > first + 1 ? js : first + 1;
+#pragma unroll 4
+  for (int64_t j = j_begin; j < je; ++j) {
+    const uint32_t oj = s.object_id[j];
+    const uint32_t sj = s.slab_id[j];
+    const double t0j = s.t0[j], t1j = s.t1[j];
+    const double* bj = s.bounds + 6 * j;
+    // all tests evaluated (bitwise, no short circuit): straight-line code
+    const bool apart = (bi[3] < bj[0]) | (bj[3] < bi[0]) | (bi[4] < bj[1]) | (bj[4] < bi[1]) | (bi[5] < bj[2]) |
+                       (bj[5] < bi[2]);
+    const bool hit = row & (j > i) & (oj != oi) & (sj == si) & (t0i < t1j) & (t0j < t1i) & !apart;
+    if (hit) {
       if (kFill) {
         const int64_t slot = base + found;
         if (slot < cap) {
           p2c_raw_hit h;
           h.query_id = s.query_id;
-          h.pair_key = ((uint64_t)(uint32_t)i << 32) | (uint32_t)jj;
+          h.pair_key = ((uint64_t)(uint32_t)i << 32) | (uint32_t)j;
           h.proxy_a_index = (uint32_t)i;
-          h.proxy_b_index = (uint32_t)jj;
+          h.proxy_b_index = (uint32_t)j;
           h.slab_id = si;
           h.rt_hit_count = 1;
           h.flags = P2C_RAW_VALID | P2C_RAW_AABB_OVERLAP;
           h.reserved0 = 0;
 #pragma unroll
           for (int k = 0; k < 4; ++k) {
-            const double mj = s.motion[4 * jj + k];
-            h.motion_bound[k] = (float)(mi[k] < mj ? mj : mi[k]);
+            const double mi = s.motion[4 * i + k];
+            const double mj = s.motion[4 * j + k];
+            h.motion_bound[k] = (float)(mi < mj ? mj : mi);
           }
           h.reserved1 = h.reserved2 = 0;
           out[slot] = h;
@@ -113,7 +125,22 @@ __global__ __launch_bounds__(kBlock) void proxy_pairs_kernel(p2c_proxy_set s, ui
       ++found;
     }
   }
-  if (!kFill && live) counts[i] = found;
+  if (!kFill && row) seg[g * n + i] = found;
+}
+
+// Per row: the segment counts become exclusive prefixes (in place) and their
+// sum is the row's count.
+__global__ __launch_bounds__(256) void seg_prefix_kernel(uint32_t* __restrict__ seg, int64_t n, int nseg,
+                                                         uint32_t* __restrict__ counts) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    uint32_t run = 0;
+    for (int g = 0; g < nseg; ++g) {
+      const uint32_t c = seg[int64_t(g) * n + i];
+      seg[int64_t(g) * n + i] = run;
+      run += c;
+    }
+    counts[i] = run;
+  }
 }
 
 // ---- compaction ----------------------------------------------------------
@@ -330,21 +357,46 @@ int check_set(const p2c_proxy_set* s) {
 
 extern "C" {
 
-int p2c_proxy_raw_count(const p2c_proxy_set* set, uint32_t* counts, void* stream) {
+size_t p2c_proxy_raw_workspace_bytes(int64_t n) {
+  if (n <= 0) return 256;
+  const p2c::SegPlan plan = p2c::seg_plan(n);
+  return size_t(plan.count) * size_t(n) * 4;
+}
+
+int p2c_proxy_raw_count(const p2c_proxy_set* set, uint32_t* counts, void* workspace, size_t workspace_bytes,
+                        void* stream) {
   if (int rc = p2c::check_set(set)) return rc;
-  if (set->n == 0) return P2C_OK;
-  hipLaunchKernelGGL(p2c::proxy_pairs_kernel<false>, dim3(p2c::grid_for(set->n, p2c::kBlock)), dim3(p2c::kBlock), 0,
-                     p2c::as_stream(stream), *set, counts, nullptr, nullptr, 0);
+  const int64_t n = set->n;
+  if (n == 0) return P2C_OK;
+  if (counts == nullptr || workspace == nullptr) return p2c::set_error(P2C_EINVAL, "counts and workspace are required");
+  if (workspace_bytes < p2c_proxy_raw_workspace_bytes(n))
+    return p2c::set_error(P2C_EINVAL, "raw-hit workspace is smaller than p2c_proxy_raw_workspace_bytes(n)");
+  const p2c::SegPlan plan = p2c::seg_plan(n);
+  const unsigned rows = unsigned((n + p2c::kRowWave - 1) / p2c::kRowWave);
+  uint32_t* seg = static_cast<uint32_t*>(workspace);
+  hipStream_t s = p2c::as_stream(stream);
+  hipLaunchKernelGGL(p2c::proxy_rows_kernel<false>, dim3(rows, unsigned(plan.count)), dim3(p2c::kRowWave), 0, s,
+                     *set, plan.len, seg, nullptr, nullptr, 0);
+  hipLaunchKernelGGL(p2c::seg_prefix_kernel, dim3(p2c::grid_for(n, 256)), dim3(256), 0, s, seg, n, plan.count,
+                     counts);
   P2C_HIP_CHECK(hipGetLastError());
   return P2C_OK;
 }
 
-int p2c_proxy_raw_fill(const p2c_proxy_set* set, const int64_t* offsets, p2c_raw_hit* out, int64_t cap,
-                       void* stream) {
+int p2c_proxy_raw_fill(const p2c_proxy_set* set, const int64_t* offsets, const void* workspace,
+                       size_t workspace_bytes, p2c_raw_hit* out, int64_t cap, void* stream) {
   if (int rc = p2c::check_set(set)) return rc;
-  if (set->n == 0) return P2C_OK;
-  hipLaunchKernelGGL(p2c::proxy_pairs_kernel<true>, dim3(p2c::grid_for(set->n, p2c::kBlock)), dim3(p2c::kBlock), 0,
-                     p2c::as_stream(stream), *set, nullptr, offsets, out, cap);
+  const int64_t n = set->n;
+  if (n == 0) return P2C_OK;
+  if (offsets == nullptr || workspace == nullptr) return p2c::set_error(P2C_EINVAL, "offsets and workspace are required");
+  if (workspace_bytes < p2c_proxy_raw_workspace_bytes(n))
+    return p2c::set_error(P2C_EINVAL, "raw-hit workspace is smaller than p2c_proxy_raw_workspace_bytes(n)");
+  if (cap > 0 && out == nullptr) return p2c::set_error(P2C_EINVAL, "raw-hit output is required");
+  const p2c::SegPlan plan = p2c::seg_plan(n);
+  const unsigned rows = unsigned((n + p2c::kRowWave - 1) / p2c::kRowWave);
+  hipLaunchKernelGGL(p2c::proxy_rows_kernel<true>, dim3(rows, unsigned(plan.count)), dim3(p2c::kRowWave), 0,
+                     p2c::as_stream(stream), *set, plan.len,
+                     const_cast<uint32_t*>(static_cast<const uint32_t*>(workspace)), offsets, out, cap);
   P2C_HIP_CHECK(hipGetLastError());
   return P2C_OK;
 }
@@ -505,14 +557,17 @@ int p2c_proxy_generate_host(const p2c_proxy_set* host_set, p2c_raw_hit** raw, in
   DeviceSet ds;
   if (int rc = ds.upload(*host_set)) return rc;
   const int64_t n = host_set->n;
-  DeviceBlock counts, offsets, part, d_raw;
+  DeviceBlock counts, offsets, part, d_raw, raw_ws;
+  const size_t raw_ws_bytes = p2c_proxy_raw_workspace_bytes(n);
+  P2C_HIP_CHECK(hipMalloc(&raw_ws.p, raw_ws_bytes));
   P2C_HIP_CHECK(hipMalloc(&counts.p, 4 * size_t(n > 0 ? n : 1)));
   P2C_HIP_CHECK(hipMalloc(&offsets.p, 8 * size_t(n + 1)));
   P2C_HIP_CHECK(hipMalloc(&part.p, 8 * size_t(n / 1024 + 2)));
   P2C_HIP_CHECK(hipEventRecord(ev[0], nullptr));
   uint64_t total = 0;
   if (n > 0) {
-    if (int rc = p2c_proxy_raw_count(&ds.d, static_cast<uint32_t*>(counts.p), nullptr)) return rc;
+    if (int rc = p2c_proxy_raw_count(&ds.d, static_cast<uint32_t*>(counts.p), raw_ws.p, raw_ws_bytes, nullptr))
+      return rc;
     if (int rc = p2c_exclusive_scan_u32(static_cast<uint32_t*>(counts.p), n, static_cast<uint64_t*>(offsets.p),
                                         static_cast<uint64_t*>(part.p), nullptr))
       return rc;
@@ -520,8 +575,8 @@ int p2c_proxy_generate_host(const p2c_proxy_set* host_set, p2c_raw_hit** raw, in
   }
   P2C_HIP_CHECK(hipMalloc(&d_raw.p, sizeof(p2c_raw_hit) * size_t(total > 0 ? total : 1)));
   if (total > 0) {
-    if (int rc = p2c_proxy_raw_fill(&ds.d, static_cast<const int64_t*>(offsets.p), static_cast<p2c_raw_hit*>(d_raw.p),
-                                    int64_t(total), nullptr))
+    if (int rc = p2c_proxy_raw_fill(&ds.d, static_cast<const int64_t*>(offsets.p), raw_ws.p, raw_ws_bytes,
+                                    static_cast<p2c_raw_hit*>(d_raw.p), int64_t(total), nullptr))
       return rc;
   }
   P2C_HIP_CHECK(hipEventRecord(ev[1], nullptr));

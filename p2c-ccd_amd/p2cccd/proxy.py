@@ -56,12 +56,14 @@ CANDIDATE_C_DTYPE = np.dtype({
     "itemsize": 80})
 
 _vp, _i64 = nat._vp, nat._i64
-nat._sig("p2c_proxy_raw_count", [_vp, _vp, _vp])
-nat._sig("p2c_proxy_raw_fill", [_vp, _vp, _vp, _i64, _vp])
+nat._sig("p2c_proxy_raw_count", [_vp, _vp, _vp, ctypes.c_size_t, _vp])
+nat._sig("p2c_proxy_raw_fill", [_vp, _vp, _vp, ctypes.c_size_t, _vp, _i64, _vp])
 nat._sig("p2c_exclusive_scan_u32", [_vp, _i64, _vp, _vp, _vp])
 nat._sig("p2c_proxy_compact", [_vp, _vp, _i64, _vp, ctypes.c_size_t, _vp, _i64, _vp, _vp])
 nat.lib.p2c_proxy_compact_workspace_bytes.argtypes = [_i64]
 nat.lib.p2c_proxy_compact_workspace_bytes.restype = ctypes.c_size_t
+nat.lib.p2c_proxy_raw_workspace_bytes.restype = ctypes.c_size_t
+nat.lib.p2c_proxy_raw_workspace_bytes.argtypes = [_i64]
 nat.lib.p2c_proxy_status_message.argtypes = [ctypes.c_uint64]
 nat.lib.p2c_proxy_status_message.restype = ctypes.c_char_p
 
@@ -121,14 +123,17 @@ def raw_hits(query_id: int, arr: dict, stream=None) -> torch.Tensor:
     counts = torch.zeros(max(n, 1), dtype=torch.int32, device=dev)
     offsets = torch.zeros(n + 1, dtype=torch.int64, device=dev)
     part = torch.empty(n // 1024 + 2, dtype=torch.int64, device=dev)
+    ws_bytes = int(nat.lib.p2c_proxy_raw_workspace_bytes(n))
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
     h = nat.stream_handle(stream)
     if n:
-        nat.call("p2c_proxy_raw_count", ctypes.addressof(s), counts.data_ptr(), h)
+        nat.call("p2c_proxy_raw_count", ctypes.addressof(s), counts.data_ptr(), ws.data_ptr(), ws_bytes, h)
         nat.call("p2c_exclusive_scan_u32", counts.data_ptr(), n, offsets.data_ptr(), part.data_ptr(), h)
     total = int(offsets[n]) if n else 0
     out = torch.empty(max(total, 1) * 64, dtype=torch.uint8, device=dev)
     if total:
-        nat.call("p2c_proxy_raw_fill", ctypes.addressof(s), offsets.data_ptr(), out.data_ptr(), total, h)
+        nat.call("p2c_proxy_raw_fill", ctypes.addressof(s), offsets.data_ptr(), ws.data_ptr(), ws_bytes,
+                 out.data_ptr(), total, h)
     return out[: total * 64]
 
 
