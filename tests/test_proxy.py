@@ -105,3 +105,22 @@ def test_pybind_validation_and_backend_names():
     assert msg(0) == b""
     assert msg(1 + ((5 << 4) | 9)) == b"raw candidate motion_bound must be finite and non-negative"
     assert msg(1 + ((0 << 4) | 10)) == b"candidate rt_hit_count overflow during compaction"
+
+
+def test_raw_hit_workspace_size_and_validation():
+    # Host-side checks of the C-ABI, before any device work: the workspace
+    # holds 4 bytes per (column segment, row), at most 64 segments of at
+    # least 1,024 columns; a short workspace is refused with a message.
+    import ctypes
+
+    from p2cccd import _native as nat
+
+    ws = nat.lib.p2c_proxy_raw_workspace_bytes
+    assert ws(1) == 4 and ws(1024) == 4 * 1024 and ws(1025) == 4 * 1025 * 2
+    assert ws(64 * 1024) == 4 * 64 * 1024 * 64 and ws(10 ** 7) == 4 * 10 ** 7 * 64
+    s = proxy.ProxySet(*([1] * len(proxy.FIELDS)), 5000, 7)
+    rc = nat.lib.p2c_proxy_raw_count(ctypes.addressof(s), ctypes.c_void_p(8), ctypes.c_void_p(8), ws(5000) - 1, None)
+    assert rc != 0 and b"workspace" in nat.lib.p2c_last_error()
+    rc = nat.lib.p2c_proxy_raw_fill(ctypes.addressof(s), ctypes.c_void_p(8), None, ws(5000), ctypes.c_void_p(8), 1,
+                                    None)
+    assert rc != 0 and b"workspace" in nat.lib.p2c_last_error()
