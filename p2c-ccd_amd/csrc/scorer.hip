@@ -266,30 +266,72 @@ static_assert(kStpfWeights == P2C_STPF_WEIGHTS, "STPF weight layout");
 
 __device__ inline float gelu(float v) { return 0.5f * v * (1.0f + erff(v * 0.70710678118654752f)); }
 
-__global__ __launch_bounds__(kBlock, 1) void stpf_forward_kernel(const float* __restrict__ w,
-                                                               const float* __restrict__ x, int64_t n,
-                                                               float* __restrict__ out) {
-  const int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (r >= n) return;
-  float xin[kF];
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+// One wave per 32 rows on FP32 matrix cores (v_mfma_f32_32x32x2f32), both
+// layers computed transposed (hidden x rows) so that no lane exchange is
+// needed between them.  In a 32x32 accumulator, lane l holds column
+// j = l % 32 (a row of the batch) and, in register v, hidden unit
+// i = 8 (v / 4) + 4 (l / 32) + v % 4.  Layer 1 takes A = W1 (lane: hidden
+// l % 32, feature k) and B = x^T (lane: feature k, row l % 32).  Layer 2
+// consumes the layer-1 registers as they are: step (t, v) multiplies the pair
+// of hidden units the two lane halves hold in register v of tile t, and the
+// A operand gathers the matching W2 columns, so the K order is a permutation
+// of 0..127 (FP32 sums in another order than torch's).  The head dot product
+// runs on each lane's 64 layer-2 values; the two halves meet by a lane swap.
+__global__ __launch_bounds__(256, 2) void stpf_mfma_kernel(const float* __restrict__ w, const float* __restrict__ x,
+                                                        int64_t n, float* __restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const int64_t r0 = ((int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6) * 32;
+  if (r0 >= n) return;  // whole wave
+  const int col = lane & 31;
+  const int h = lane >> 5;
+  const int64_t row = r0 + col;
+  const bool live = row < n;
+
+  floatx16 acc1[4];
 #pragma unroll
-  for (int i = 0; i < kF; ++i) xin[i] = x[kF * r + i];
-  float h1[kH];
+  for (int t = 0; t < 4; ++t)
 #pragma unroll
-  for (int o = 0; o < kH; ++o) {
-    float s = w[kSB1 + o];
+    for (int v = 0; v < 16; ++v) acc1[t][v] = w[kSB1 + 32 * t + 8 * (v >> 2) + 4 * h + (v & 3)];
 #pragma unroll
-    for (int i = 0; i < kF; ++i) s = fmaf(w[kSW1 + o * kF + i], xin[i], s);
-    h1[o] = gelu(s);
+  for (int s = 0; s < kF / 2; ++s) {
+    const int k = 2 * s + h;
+    const float bx = live ? x[row * kF + k] : 0.0f;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const float aw = w[kSW1 + (32 * t + col) * kF + k];
+      acc1[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(aw, bx, acc1[t], 0, 0, 0);
+    }
   }
-  float logit = w[kSBP];
-  for (int o = 0; o < kH; ++o) {
-    float s = w[kSB2 + o];
 #pragma unroll
-    for (int i = 0; i < kH; ++i) s = fmaf(w[kSW2 + o * kH + i], h1[i], s);
-    logit = fmaf(w[kSWP + o], gelu(s), logit);
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int v = 0; v < 16; ++v) acc1[t][v] = gelu(acc1[t][v]);
+
+  float logit = 0.0f;
+#pragma unroll 1
+  for (int mo = 0; mo < 4; ++mo) {
+    floatx16 acc2;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) acc2[v] = w[kSB2 + 32 * mo + 8 * (v >> 2) + 4 * h + (v & 3)];
+    const float* w2 = w + kSW2 + (32 * mo + col) * kH + 4 * h;
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        const float4 a4 = *reinterpret_cast<const float4*>(w2 + 32 * t + 8 * g);
+        acc2 = __builtin_amdgcn_mfma_f32_32x32x2f32(a4.x, acc1[t][4 * g + 0], acc2, 0, 0, 0);
+        acc2 = __builtin_amdgcn_mfma_f32_32x32x2f32(a4.y, acc1[t][4 * g + 1], acc2, 0, 0, 0);
+        acc2 = __builtin_amdgcn_mfma_f32_32x32x2f32(a4.z, acc1[t][4 * g + 2], acc2, 0, 0, 0);
+        acc2 = __builtin_amdgcn_mfma_f32_32x32x2f32(a4.w, acc1[t][4 * g + 3], acc2, 0, 0, 0);
+      }
+#pragma unroll
+    for (int v = 0; v < 16; ++v)
+      logit = fmaf(w[kSWP + 32 * mo + 8 * (v >> 2) + 4 * h + (v & 3)], gelu(acc2[v]), logit);
   }
-  out[r] = 1.0f / (1.0f + expf(-logit));
+  logit += __shfl_xor(logit, 32);
+  if (h == 0 && live) out[row] = 1.0f / (1.0f + expf(-(logit + w[kSBP])));
 }
 
 }  // namespace
@@ -337,7 +379,8 @@ int p2c_tiny_stpf_predict(const double* weights, const double* features, int64_t
 int p2c_stpf_priority(const float* weights, const float* features, int64_t n, float* out, void* stream) {
   if (weights == nullptr) return p2c::set_error(P2C_EINVAL, "STPF weights are required");
   if (n <= 0) return P2C_OK;
-  hipLaunchKernelGGL(p2c::stpf_forward_kernel, dim3(p2c::grid_for(n, p2c::kBlock)), dim3(p2c::kBlock), 0,
+  // 32 rows per wave, 4 waves per workgroup
+  hipLaunchKernelGGL(p2c::stpf_mfma_kernel, dim3(p2c::grid_for((n + 31) / 32, 4)), dim3(256), 0,
                      p2c::as_stream(stream), weights, features, n, out);
   P2C_HIP_CHECK(hipGetLastError());
   return P2C_OK;
