@@ -212,6 +212,21 @@ int p2c_tuv_stream(const void* pairs, int64_t n, const uint64_t* totals, double 
 int p2c_tuv_wave(const void* pairs, void* counters, const void* states, const uint32_t* heavy_list,
                  uint32_t list_cap, double tolerance, int64_t max_itr, p2c_tuv_result* out, uint32_t* replay_list,
                  int grid_blocks, void* stream);
+/* The wave pass with splitting: a pair that has taken more than split_at
+ * boxes (0: never) is cut into pieces (each lane's pending path siblings, its
+ * current subtree or unfinished walk, the wave's queued subtrees), written to
+ * `tasks` (task_cap * p2c_tuv_task_bytes() device bytes) and certified one
+ * wave per piece with the pair's best shared through its `states` slot; the
+ * split pairs (listed in split_list, list_cap entries) get their results
+ * once all pieces are done.  A pair whose pieces do not fit in `tasks` is
+ * finished by its own wave.  `counters` as for p2c_tuv_stream, with the task
+ * count, task cursor and split count at bytes 52, 56 and 60.  No counterpart
+ * in the reference (its certificate runs one query per CPU thread). */
+size_t p2c_tuv_task_bytes(void);
+int p2c_tuv_wave_split(const void* pairs, void* counters, void* states, const uint32_t* heavy_list,
+                       uint32_t list_cap, double tolerance, int64_t max_itr, p2c_tuv_result* out,
+                       uint32_t* replay_list, int64_t split_at, void* tasks, uint32_t task_cap, uint32_t* split_list,
+                       int grid_blocks, void* stream);
 /* Fallback replay of the listed pairs in canonical order; `heaps` holds
  * slots * p2c_tuv_heap_bytes(heap_cap) bytes of device memory. */
 int p2c_tuv_replay(const void* pairs, const uint32_t* replay_list, const uint32_t* replay_count,

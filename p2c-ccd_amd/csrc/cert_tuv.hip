@@ -574,6 +574,239 @@ __device__ inline uint32_t pending_siblings(const LaneWalk& w, uint32_t root, ui
   return (w.level - root) - set;
 }
 
+// A piece of a split pair's box tree: a walk (key, level, exponents, ready)
+// confined to the subtree rooted at `root` (exponents root_e).
+struct SplitTask {
+  uint64_t key[3];
+  uint32_t pair;
+  uint16_t level, root;
+  uint8_t e[3], ready;
+  uint8_t root_e[3], pad;
+};
+static_assert(sizeof(SplitTask) == 40, "split task layout");
+
+// A split pair's accumulators live in its WalkState slot once the walk it
+// held has been resumed: best (u64 at byte 24, atomicMin), boxes (u32 at byte
+// 32, atomicAdd), over-budget flag (u32 at byte 36, atomicOr).
+struct SplitAcc {
+  uint64_t pad[3];
+  unsigned long long best;
+  uint32_t boxes;
+  uint32_t over;
+  uint64_t pad2;
+};
+static_assert(sizeof(SplitAcc) == sizeof(WalkState), "split accumulators reuse the walk state slot");
+
+struct SplitOut {
+  uint64_t split_at;     // boxes after which a pair is split (0: never)
+  SplitTask* tasks;
+  uint32_t task_cap;
+  uint32_t* task_count;  // reserved task slots
+  uint32_t* split_list;  // split pairs
+  uint32_t* split_count;
+  SplitAcc* acc;         // per pair (the walk state slots)
+};
+
+__device__ inline SplitTask make_task(uint32_t pair, const uint64_t k[3], uint32_t level, uint32_t ep, bool ready,
+                                      uint32_t root, uint32_t root_ep) {
+  SplitTask t;
+  t.key[0] = k[0];
+  t.key[1] = k[1];
+  t.key[2] = k[2];
+  t.pair = pair;
+  t.level = (uint16_t)level;
+  t.root = (uint16_t)root;
+  for (int d = 0; d < 3; ++d) {
+    t.e[d] = (uint8_t)exp_of(ep, d);
+    t.root_e[d] = (uint8_t)exp_of(root_ep, d);
+  }
+  t.ready = ready ? 1 : 0;
+  t.pad = 0;
+  return t;
+}
+
+// The wave pass over one pair (heavy mode) or one split task (task mode).
+// Heavy mode starts from the pair's saved walk (or the root); once the pair
+// has taken more than split_at boxes it may split: every pending piece (each
+// lane's path siblings, each lane's current subtree or unfinished walk, every
+// queued task) becomes a task, the pair's best and box count go to its
+// accumulators, and the pair is listed for the finishing pass.  Task mode
+// walks one piece with the pair's shared best: read from the accumulator
+// every 16 steps and lowered there on every improvement.  Every form of the
+// result is the minimum t_lo over done boxes, so the pieces' order is free.
+enum : int { kWaveDone = 0, kWaveOver = 1, kWaveSplit = 2 };
+
+template <bool kTask>
+__device__ int wave_certify(const TuvPair& q, uint32_t p, WaveTask* wq, unsigned long long* best_lds, LaneWalk& w,
+                            bool has, uint32_t root, uint32_t root_ep, uint64_t& boxes, double tol,
+                            uint64_t max_itr, const SplitOut& so) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t lt = (1ull << lane) - 1ull;
+  int qsize = 0;
+  uint32_t step = 0;
+  for (;;) {
+    // 1. evaluate (a box pruned meanwhile by the shared best is skipped),
+    //    then descend, or make at most kAdvanceMoves moves
+    bool evaluated = false;
+    if (has && w.ready) {
+      int outcome = 0;
+      if (w.key[0] < w.best) {
+        evaluated = true;
+        outcome = evaluate(q, w, tol);
+        if (outcome == 1 && w.key[0] < w.best) {
+          w.best = w.key[0];
+          atomicMin(best_lds, (unsigned long long)w.key[0]);
+          if (kTask) atomicMin(&so.acc[p].best, (unsigned long long)w.key[0]);
+        }
+      }
+      if (outcome == 2) descend(q, w);
+      else w.ready = false;
+    }
+    if (has && !w.ready) {
+#pragma unroll
+      for (int m = 0; m < kAdvanceMoves; ++m) {
+        if (!advance_once(q, w, root)) {
+          has = false;
+          break;
+        }
+        if (w.ready) break;
+      }
+    }
+    boxes += (uint64_t)__popcll(__ballot(evaluated));
+    wave_sync_lds();
+    {
+      uint64_t shared_best = *best_lds;
+      if (kTask && (++step & 15u) == 0) {
+        const uint64_t g = __hip_atomic_load(&so.acc[p].best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (g < shared_best) shared_best = g;
+      }
+      if (shared_best < w.best) w.best = shared_best;
+    }
+    if (boxes > max_itr) return kWaveOver;
+    if (!kTask && so.split_at != 0 && boxes > so.split_at) {
+      // count the pieces: per lane its path siblings and its own walk, plus the queue
+      const uint32_t pend = has && w.ready ? pending_siblings(w, root, root_ep) : 0u;
+      const uint32_t mine = has ? pend + 1u : 0u;  // at most kTuvMaxLevel + 1 < 256
+      uint32_t lanes_tot = 0;
+      const uint32_t off = wave_prefix_u8(mine, &lanes_tot);
+      const uint32_t need = lanes_tot + (uint32_t)qsize;
+      uint32_t base = 0xFFFFFFFFu;
+      if (lane == 0) {
+        uint32_t cur = __hip_atomic_load(so.task_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int tries = 0; tries < 64; ++tries) {
+          if (cur + need > so.task_cap || cur + need < cur) break;
+          if (__hip_atomic_compare_exchange_strong(so.task_count, &cur, cur + need, __ATOMIC_RELAXED,
+                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            base = cur;
+            break;
+          }
+        }
+      }
+      base = __shfl(base, 0);
+      if (base != 0xFFFFFFFFu) {
+        SplitTask* out = so.tasks + base;
+        if (has) {
+          uint32_t slot = off;
+          if (w.ready) {
+            // path siblings, then the lane's own box as a subtree of its own
+            uint64_t k[3] = {w.key[0], w.key[1], w.key[2]};
+            uint32_t ep = w.ep;
+            for (uint32_t l = w.level; l > root; --l) {
+              const int d = tuv_level_dim(q, l - 1);
+              const uint64_t bit = kOne63 >> exp_of(ep, d);
+              const uint64_t kd = get3(k, d);
+              if (!(kd & bit)) {
+                uint64_t kk[3] = {k[0], k[1], k[2]};
+                put3(kk, d, kd | bit);
+                out[slot++] = make_task(p, kk, l, ep, true, l, ep);
+              }
+              put3(k, d, kd & ~bit);
+              ep -= 1u << (8 * d);
+            }
+            out[slot] = make_task(p, w.key, w.level, w.ep, true, w.level, w.ep);
+          } else {
+            out[slot] = make_task(p, w.key, w.level, w.ep, false, root, root_ep);
+          }
+        }
+        for (int k = lane; k < qsize; k += 64) {
+          const WaveTask t = wq[k];
+          const uint64_t kk[3] = {t.kt, t.ku, t.kv};
+          out[lanes_tot + (uint32_t)k] = make_task(p, kk, t.level, t.ep, true, t.level, t.ep);
+        }
+        if (lane == 0) {
+          so.acc[p].best = *best_lds < w.best ? *best_lds : w.best;
+          so.acc[p].boxes = (uint32_t)boxes;
+          so.acc[p].over = 0;
+          const uint32_t at = atomicAdd(so.split_count, 1u);
+          so.split_list[at] = p;  // at most one entry per listed pair
+        }
+        return kWaveSplit;
+      }
+    }
+    const int idle = 64 - __popcll(__ballot(has));
+    if (idle == 0) continue;
+    // 2. donation: lanes standing on a box give away the pending siblings on their path
+    if (idle > qsize) {
+      // at most kTuvMaxLevel (156) pending siblings: within wave_prefix_u8's range
+      const uint32_t pend = has && w.ready ? pending_siblings(w, root, root_ep) : 0u;
+      uint32_t tot = 0;
+      const uint32_t off = wave_prefix_u8(pend, &tot);
+      const bool push = pend > 0 && qsize + (int)(off + pend) <= kWaveQueue;
+      if (push) {
+        uint64_t k[3] = {w.key[0], w.key[1], w.key[2]};
+        uint32_t ep = w.ep;
+        int slot = qsize + (int)off;
+        for (uint32_t l = w.level; l > root; --l) {  // deepest first: shallowest ends on top
+          const int d = tuv_level_dim(q, l - 1);
+          const uint64_t bit = kOne63 >> exp_of(ep, d);
+          const uint64_t kd = get3(k, d);
+          if (!(kd & bit)) {
+            WaveTask t;
+            t.kt = k[0] | (d == 0 ? bit : 0ull);
+            t.ku = k[1] | (d == 1 ? bit : 0ull);
+            t.kv = k[2] | (d == 2 ? bit : 0ull);
+            t.level = l;
+            t.ep = ep;
+            wq[slot++] = t;
+          }
+          put3(k, d, kd & ~bit);
+          ep -= 1u << (8 * d);
+        }
+        root = w.level;
+        root_ep = w.ep;
+      }
+      // lanes whose block did not fit pushed nothing; count what was pushed
+      uint32_t pushed_tot = 0;
+      wave_prefix_u8(push ? pend : 0u, &pushed_tot);
+      qsize += (int)pushed_tot;
+      wave_sync_lds();
+    }
+    // 3. idle lanes take queued subtrees, the top of the stack first
+    const uint64_t want = __ballot(!has);
+    const int n_take = min(__popcll(want), qsize);
+    if (n_take > 0) {
+      if (!has) {
+        const int rank = __popcll(want & lt);
+        if (rank < n_take) {
+          const WaveTask t = wq[qsize - 1 - rank];
+          w.key[0] = t.kt;
+          w.key[1] = t.ku;
+          w.key[2] = t.kv;
+          w.level = t.level;
+          w.ep = t.ep;
+          root = t.level;
+          root_ep = t.ep;
+          w.ready = true;
+          has = !box_skipped(q, w);  // a skipped task's whole subtree is skipped
+        }
+      }
+      qsize -= n_take;
+      wave_sync_lds();
+    }
+    if (__ballot(has) == 0 && qsize == 0) return kWaveDone;
+  }
+}
+
 __global__ __launch_bounds__(kTuvBlock) void tuv_heavy_kernel(const TuvPair* __restrict__ pairs,
                                                               const uint32_t* __restrict__ heavy_list,
                                                               const uint32_t* __restrict__ heavy_count,
@@ -581,12 +814,10 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_heavy_kernel(const TuvPair* __r
                                                               const WalkState* __restrict__ resume, double tol,
                                                               uint64_t max_itr, p2c_tuv_result* __restrict__ out,
                                                               uint32_t* __restrict__ replay_list,
-                                                              uint32_t* __restrict__ replay_count) {
+                                                              uint32_t* __restrict__ replay_count, SplitOut so) {
   __shared__ WaveTask queue[kTuvBlock / 64][kWaveQueue];
   __shared__ unsigned long long best_slot[kTuvBlock / 64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  WaveTask* wq = queue[wv];
-  const uint64_t lt = (1ull << lane) - 1ull;
   uint32_t total = *heavy_count;
   if (total > list_cap) total = list_cap;
   for (;;) {
@@ -597,8 +828,6 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_heavy_kernel(const TuvPair* __r
     const uint32_t p = heavy_list[h];
     const TuvPair q = load_pair(pairs, p);
     LaneWalk w;
-    uint32_t root = 0, root_ep = 0;
-    bool has = lane == 0;  // the lane holds a walk (standing on a box, or moving to the next)
     uint64_t boxes = 0;
     start_walk(w);
     if (resume != nullptr) {
@@ -609,113 +838,76 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_heavy_kernel(const TuvPair* __r
     }
     if (lane == 0) best_slot[wv] = w.best;
     wave_sync_lds();
-    int qsize = 0;
-    bool over = false;
-    for (;;) {
-      // 1. evaluate (a box pruned meanwhile by the shared best is skipped),
-      //    then descend, or make at most kAdvanceMoves moves
-      bool evaluated = false;
-      if (has && w.ready) {
-        int outcome = 0;
-        if (w.key[0] < w.best) {
-          evaluated = true;
-          outcome = evaluate(q, w, tol);
-          if (outcome == 1 && w.key[0] < w.best) {
-            w.best = w.key[0];
-            atomicMin(&best_slot[wv], (unsigned long long)w.key[0]);
-          }
-        }
-        if (outcome == 2) descend(q, w);
-        else w.ready = false;
+    const int r = wave_certify<false>(q, p, queue[wv], &best_slot[wv], w, lane == 0, 0, 0, boxes, tol, max_itr, so);
+    if (r != kWaveSplit) {
+      const uint64_t final_best = best_slot[wv] < w.best ? best_slot[wv] : w.best;
+      if (lane == 0) {
+        write_result(&out[p], final_best, boxes, r == kWaveOver ? P2C_TUV_FLAG_OVER_BUDGET : 0u);
+        if (r == kWaveOver) flag_for_replay(p, replay_list, replay_count, list_cap);
       }
-      if (has && !w.ready) {
-#pragma unroll
-        for (int m = 0; m < kAdvanceMoves; ++m) {
-          if (!advance_once(q, w, root)) {
-            has = false;
-            break;
-          }
-          if (w.ready) break;
-        }
-      }
-      boxes += (uint64_t)__popcll(__ballot(evaluated));
-      wave_sync_lds();
-      {
-        const uint64_t shared_best = best_slot[wv];
-        if (shared_best < w.best) w.best = shared_best;
-      }
-      if (boxes > max_itr) {
-        over = true;
-        break;
-      }
-      const int idle = 64 - __popcll(__ballot(has));
-      if (idle == 0) continue;
-      // 2. donation: lanes standing on a box give away the pending siblings on their path
-      if (idle > qsize) {
-        // at most kTuvMaxLevel (156) pending siblings: within wave_prefix_u8's range
-        const uint32_t pend = has && w.ready ? pending_siblings(w, root, root_ep) : 0u;
-        uint32_t tot = 0;
-        const uint32_t off = wave_prefix_u8(pend, &tot);
-        const bool push = pend > 0 && qsize + (int)(off + pend) <= kWaveQueue;
-        if (push) {
-          uint64_t k[3] = {w.key[0], w.key[1], w.key[2]};
-          uint32_t ep = w.ep;
-          int slot = qsize + (int)off;
-          for (uint32_t l = w.level; l > root; --l) {  // deepest first: shallowest ends on top
-            const int d = tuv_level_dim(q, l - 1);
-            const uint64_t bit = kOne63 >> exp_of(ep, d);
-            const uint64_t kd = get3(k, d);
-            if (!(kd & bit)) {
-              WaveTask t;
-              t.kt = k[0] | (d == 0 ? bit : 0ull);
-              t.ku = k[1] | (d == 1 ? bit : 0ull);
-              t.kv = k[2] | (d == 2 ? bit : 0ull);
-              t.level = l;
-              t.ep = ep;
-              wq[slot++] = t;
-            }
-            put3(k, d, kd & ~bit);
-            ep -= 1u << (8 * d);
-          }
-          root = w.level;
-          root_ep = w.ep;
-        }
-        // lanes whose block did not fit pushed nothing; count what was pushed
-        uint32_t pushed_tot = 0;
-        wave_prefix_u8(push ? pend : 0u, &pushed_tot);
-        qsize += (int)pushed_tot;
-        wave_sync_lds();
-      }
-      // 3. idle lanes take queued subtrees, the top of the stack first
-      const uint64_t want = __ballot(!has);
-      const int n_take = min(__popcll(want), qsize);
-      if (n_take > 0) {
-        if (!has) {
-          const int rank = __popcll(want & lt);
-          if (rank < n_take) {
-            const WaveTask t = wq[qsize - 1 - rank];
-            w.key[0] = t.kt;
-            w.key[1] = t.ku;
-            w.key[2] = t.kv;
-            w.level = t.level;
-            w.ep = t.ep;
-            root = t.level;
-            root_ep = t.ep;
-            w.ready = true;
-            has = !box_skipped(q, w);  // a skipped task's whole subtree is skipped
-          }
-        }
-        qsize -= n_take;
-        wave_sync_lds();
-      }
-      if (__ballot(has) == 0 && qsize == 0) break;
-    }
-    const uint64_t final_best = best_slot[wv] < w.best ? best_slot[wv] : w.best;
-    if (lane == 0) {
-      write_result(&out[p], final_best, boxes, over ? P2C_TUV_FLAG_OVER_BUDGET : 0u);
-      if (over) flag_for_replay(p, replay_list, replay_count, list_cap);
     }
     __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// Split tasks, one wave per task; boxes and an over-budget mark go to the
+// pair's accumulators.
+__global__ __launch_bounds__(kTuvBlock) void tuv_task_kernel(const TuvPair* __restrict__ pairs,
+                                                             uint32_t* __restrict__ task_next, double tol,
+                                                             uint64_t max_itr, SplitOut so) {
+  __shared__ WaveTask queue[kTuvBlock / 64][kWaveQueue];
+  __shared__ unsigned long long best_slot[kTuvBlock / 64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  uint32_t total = *so.task_count;
+  if (total > so.task_cap) total = so.task_cap;
+  for (;;) {
+    uint32_t h = 0;
+    if (lane == 0) h = atomicAdd(task_next, 1u);
+    h = __shfl(h, 0);
+    if (h >= total) break;
+    const SplitTask t = so.tasks[h];
+    const uint32_t p = t.pair;
+    const TuvPair q = load_pair(pairs, p);
+    LaneWalk w;
+    start_walk(w);
+    w.best = __hip_atomic_load(&so.acc[p].best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    w.key[0] = t.key[0];
+    w.key[1] = t.key[1];
+    w.key[2] = t.key[2];
+    w.level = t.level;
+    w.ep = (uint32_t)t.e[0] | ((uint32_t)t.e[1] << 8) | ((uint32_t)t.e[2] << 16);
+    w.ready = t.ready != 0;
+    const uint32_t root_ep = (uint32_t)t.root_e[0] | ((uint32_t)t.root_e[1] << 8) | ((uint32_t)t.root_e[2] << 16);
+    const bool has = lane == 0 && (!w.ready || !box_skipped(q, w));
+    if (lane == 0) best_slot[wv] = w.best;
+    wave_sync_lds();
+    uint64_t boxes = 0;
+    const int r = wave_certify<true>(q, p, queue[wv], &best_slot[wv], w, has, t.root, root_ep, boxes, tol, max_itr,
+                                     so);
+    if (lane == 0) {
+      const uint64_t b = best_slot[wv] < w.best ? best_slot[wv] : w.best;
+      atomicMin(&so.acc[p].best, (unsigned long long)b);
+      atomicAdd(&so.acc[p].boxes, (uint32_t)(boxes < 0xFFFFFFFFull ? boxes : 0xFFFFFFFFull));
+      if (r == kWaveOver) atomicOr(&so.acc[p].over, 1u);
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// Results of the split pairs once all their tasks are done.
+__global__ __launch_bounds__(kTuvBlock) void tuv_split_finish_kernel(SplitOut so, uint64_t max_itr,
+                                                                     p2c_tuv_result* __restrict__ out,
+                                                                     uint32_t* __restrict__ replay_list,
+                                                                     uint32_t* __restrict__ replay_count,
+                                                                     uint32_t list_cap) {
+  uint32_t total = *so.split_count;
+  if (total > list_cap) total = list_cap;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+    const uint32_t p = so.split_list[i];
+    const SplitAcc a = so.acc[p];
+    const bool over = a.over != 0 || a.boxes > max_itr;
+    write_result(&out[p], a.best, a.boxes, over ? P2C_TUV_FLAG_OVER_BUDGET : 0u);
+    if (over) flag_for_replay(p, replay_list, replay_count, list_cap);
   }
 }
 
@@ -928,7 +1120,7 @@ int p2c_tuv_persistent(const void* pairs, int64_t n, const uint64_t* totals, dou
                      c64, heavy_list, c32 + 3, list_cap);
   hipLaunchKernelGGL(p2c::tuv_heavy_kernel, dim3(grid_blocks), dim3(p2c::kTuvBlock), 0, s,
                      static_cast<const p2c::TuvPair*>(pairs), heavy_list, c32 + 3, list_cap, c32 + 4, nullptr,
-                     tolerance, (uint64_t)max_itr, out, replay_list, c32 + 2);
+                     tolerance, (uint64_t)max_itr, out, replay_list, c32 + 2, p2c::SplitOut{});
   P2C_HIP_CHECK(hipGetLastError());
   return P2C_OK;
 }
@@ -962,7 +1154,7 @@ int p2c_tuv_rounds(const void* pairs, int64_t n, const uint64_t* totals, double 
   }
   const uint32_t* heavy = lists[(n_rounds - 1) & 1];
   hipLaunchKernelGGL(p2c::tuv_heavy_kernel, dim3(grid_blocks), dim3(p2c::kTuvBlock), 0, s, tp, heavy, c32 + 3,
-                     list_cap, c32 + 4, st, tolerance, (uint64_t)max_itr, out, replay_list, c32 + 2);
+                     list_cap, c32 + 4, st, tolerance, (uint64_t)max_itr, out, replay_list, c32 + 2, p2c::SplitOut{});
   P2C_HIP_CHECK(hipGetLastError());
   return P2C_OK;
 }
@@ -993,13 +1185,45 @@ int p2c_tuv_stream(const void* pairs, int64_t n, const uint64_t* totals, double 
 int p2c_tuv_wave(const void* pairs, void* counters, const void* states, const uint32_t* heavy_list,
                  uint32_t list_cap, double tolerance, int64_t max_itr, p2c_tuv_result* out, uint32_t* replay_list,
                  int grid_blocks, void* stream) {
+  return p2c_tuv_wave_split(pairs, counters, const_cast<void*>(states), heavy_list, list_cap, tolerance, max_itr, out,
+                            replay_list, 0, nullptr, 0, nullptr, grid_blocks, stream);
+}
+
+size_t p2c_tuv_task_bytes(void) { return sizeof(p2c::SplitTask); }
+
+int p2c_tuv_wave_split(const void* pairs, void* counters, void* states, const uint32_t* heavy_list,
+                       uint32_t list_cap, double tolerance, int64_t max_itr, p2c_tuv_result* out,
+                       uint32_t* replay_list, int64_t split_at, void* tasks, uint32_t task_cap, uint32_t* split_list,
+                       int grid_blocks, void* stream) {
   if (int rc = p2c::check_tuv(0.0, tolerance, 1.0, max_itr)) return rc;
   if (grid_blocks <= 0) return p2c::set_error(P2C_EINVAL, "grid_blocks must be positive");
+  if (split_at < 0) return p2c::set_error(P2C_EINVAL, "split_at must be non-negative");
+  const bool split = split_at > 0;
+  if (split && (tasks == nullptr || task_cap == 0 || split_list == nullptr))
+    return p2c::set_error(P2C_EINVAL, "splitting needs a task buffer and a split list");
   auto* c32 = static_cast<uint32_t*>(counters);
-  hipLaunchKernelGGL(p2c::tuv_heavy_kernel, dim3(grid_blocks), dim3(p2c::kTuvBlock), 0, p2c::as_stream(stream),
-                     static_cast<const p2c::TuvPair*>(pairs), heavy_list, c32 + 3, list_cap, c32 + 4,
-                     static_cast<const p2c::WalkState*>(states), tolerance, (uint64_t)max_itr, out, replay_list,
-                     c32 + 2);
+  hipStream_t s = p2c::as_stream(stream);
+  const auto* tp = static_cast<const p2c::TuvPair*>(pairs);
+  p2c::SplitOut so{};
+  if (split) {
+    // counters (u32): [13] reserved task slots, [14] task cursor, [15] split pairs
+    so.split_at = (uint64_t)split_at;
+    so.tasks = static_cast<p2c::SplitTask*>(tasks);
+    so.task_cap = task_cap;
+    so.task_count = c32 + 13;
+    so.split_list = split_list;
+    so.split_count = c32 + 15;
+    so.acc = static_cast<p2c::SplitAcc*>(states);
+  }
+  hipLaunchKernelGGL(p2c::tuv_heavy_kernel, dim3(grid_blocks), dim3(p2c::kTuvBlock), 0, s, tp, heavy_list, c32 + 3,
+                     list_cap, c32 + 4, static_cast<const p2c::WalkState*>(states), tolerance, (uint64_t)max_itr, out,
+                     replay_list, c32 + 2, so);
+  if (split) {
+    hipLaunchKernelGGL(p2c::tuv_task_kernel, dim3(grid_blocks), dim3(p2c::kTuvBlock), 0, s, tp, c32 + 14, tolerance,
+                       (uint64_t)max_itr, so);
+    hipLaunchKernelGGL(p2c::tuv_split_finish_kernel, dim3(16), dim3(p2c::kTuvBlock), 0, s, so, (uint64_t)max_itr, out,
+                       replay_list, c32 + 2, list_cap);
+  }
   P2C_HIP_CHECK(hipGetLastError());
   return P2C_OK;
 }

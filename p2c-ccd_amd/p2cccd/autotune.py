@@ -30,6 +30,8 @@ SPACE = {
     "handover_at": (0, 8, 16),
     "blocks_per_cu": (2, 3, 4),
 }
+# tried after the grid, with the grid's winner: when the wave pass cuts a pair into pieces
+SPLIT_CHOICES = (0, 16384, 65536)
 
 
 @dataclass
@@ -85,7 +87,8 @@ class CertificateTuner:
 
     def _apply(self, p: dict) -> None:
         cfg = dataclasses.replace(self.cert.cfg, refill_at=p["refill_at"], light_budget=p["light_budget"],
-                                  handover_at=p["handover_at"], blocks_per_cu=p["blocks_per_cu"])
+                                  handover_at=p["handover_at"], blocks_per_cu=p["blocks_per_cu"],
+                                  split_at=p.get("split_at", self.cert.cfg.split_at))
         self.cert.cfg = cfg
         props = torch.cuda.get_device_properties(self.cert.device)
         self.cert.grid_blocks = int(props.multi_processor_count) * cfg.blocks_per_cu
@@ -115,6 +118,8 @@ class CertificateTuner:
             if self.check(cached):
                 return self.cert.cfg, []
         params, trials = select(grid(space), self.measure, self.check)
+        params, more = select((dict(params, split_at=v) for v in SPLIT_CHOICES), self.measure, self.check)
+        trials += more
         self._apply(params)
         db = self._load()
         db[sig] = params

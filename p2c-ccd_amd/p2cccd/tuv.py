@@ -19,6 +19,7 @@ tolerance 1e-6, t_max 1, max_itr 1e6.
 """
 from __future__ import annotations
 
+import ctypes
 from dataclasses import dataclass
 
 import numpy as np
@@ -40,6 +41,8 @@ nat._sig("p2c_tuv_plain", [_vp, _i64, _vp, _f64, _i64, _vp, _vp, _vp, _u32, _vp]
 nat._sig("p2c_tuv_persistent", [_vp, _i64, _vp, _f64, _i64, _i64, _vp, _vp, _vp, _vp, _u32, _i32, _vp])
 nat._sig("p2c_tuv_stream", [_vp, _i64, _vp, _f64, _i64, _i64, _i32, _i32, _vp, _vp, _vp, _vp, _u32, _i32, _vp])
 nat._sig("p2c_tuv_wave", [_vp, _vp, _vp, _vp, _u32, _f64, _i64, _vp, _vp, _i32, _vp])
+nat._sig("p2c_tuv_wave_split", [_vp, _vp, _vp, _vp, _u32, _f64, _i64, _vp, _vp, _i64, _vp, _u32, _vp, _i32, _vp])
+nat.lib.p2c_tuv_task_bytes.restype = ctypes.c_size_t
 nat._sig("p2c_tuv_rounds", [_vp, _i64, _vp, _f64, _i64, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _u32, _i32, _vp])
 nat.lib.p2c_tuv_state_bytes.restype = nat.ctypes.c_size_t
 nat._sig("p2c_tuv_replay", [_vp, _vp, _vp, _u32, _f64, _i64, _vp, _i64, _i64, _vp, _vp])
@@ -60,6 +63,8 @@ class TuvConfig:
     light_budget: int = 1024     # boxes a pair may take in the stream/persistent pass before the wave pass
     refill_at: int = 8           # stream: idle lanes of a wave that trigger a refill
     handover_at: int = 8         # stream: busy lanes of a drained wave at or below which all go to the wave pass
+    split_at: int = 16384        # wave pass: boxes after which a pair is cut into pieces for many waves (0: never)
+    split_tasks: int = 1 << 16   # capacity of the piece buffer
     round_budgets: tuple = (8, 32, 128, 512)  # walk steps per pair per round before the wave pass
 
 
@@ -79,10 +84,13 @@ class TuvCertifier:
         self.grid_blocks = int(props.multi_processor_count) * self.cfg.blocks_per_cu
         self.heaps = torch.empty(self.cfg.replay_slots * self.cfg.heap_cap * 32, dtype=torch.uint8,
                                  device=self.device)
-        # u32 view: [0:2] next pair, [2] replay count, [3] heavy count, [4] heavy cursor, [5:13] round survivors
+        # u32 view: [0:2] next pair, [2] replay count, [3] heavy count, [4] heavy cursor, [5:13] round survivors,
+        # [13] split pieces, [14] piece cursor, [15] split pairs
         self.counters = torch.zeros(8, dtype=torch.int64, device=self.device)
         self.budgets = np.ascontiguousarray(self.cfg.round_budgets, dtype=np.uint32)
         self.state_bytes = int(nat.lib.p2c_tuv_state_bytes())
+        self.tasks = torch.empty(max(self.cfg.split_tasks, 1) * int(nat.lib.p2c_tuv_task_bytes()), dtype=torch.uint8,
+                                 device=self.device)
 
     def reserve(self, n: int) -> None:
         self._reserve(n)
@@ -97,6 +105,7 @@ class TuvCertifier:
         self.heavy_list = torch.empty(cap, dtype=torch.int32, device=self.device)
         self.round_list = torch.empty(cap, dtype=torch.int32, device=self.device)
         self.states = torch.empty(cap * self.state_bytes, dtype=torch.uint8, device=self.device)
+        self.split_list = torch.empty(cap, dtype=torch.int32, device=self.device)
         self._cap = cap
 
     # -- stages ------------------------------------------------------------
@@ -144,9 +153,10 @@ class TuvCertifier:
                      c.data_ptr(), self.states.data_ptr(), self.heavy_list.data_ptr(), self._cap, self.grid_blocks, s)
             if events is not None and len(events) > 2:
                 events[2].record()
-            nat.call("p2c_tuv_wave", self.pairs.data_ptr(), c.data_ptr(), self.states.data_ptr(),
+            nat.call("p2c_tuv_wave_split", self.pairs.data_ptr(), c.data_ptr(), self.states.data_ptr(),
                      self.heavy_list.data_ptr(), self._cap, self.cfg.tolerance, self.cfg.max_itr,
-                     self.results.data_ptr(), self.replay_list.data_ptr(), self.grid_blocks, s)
+                     self.results.data_ptr(), self.replay_list.data_ptr(), self.cfg.split_at, self.tasks.data_ptr(),
+                     self.cfg.split_tasks, self.split_list.data_ptr(), self.grid_blocks, s)
         elif mode == "persistent":
             nat.call("p2c_tuv_persistent", self.pairs.data_ptr(), n, tp, self.cfg.tolerance, self.cfg.max_itr,
                      self.cfg.light_budget, self.results.data_ptr(), c.data_ptr(), self.heavy_list.data_ptr(),
@@ -168,6 +178,11 @@ class TuvCertifier:
 
     def replay_count(self) -> int:
         return int(self.counters[1].item() & 0xFFFFFFFF)
+
+    def split_counts(self) -> tuple:
+        """(pieces written, pairs split) by the last stream-form wave pass."""
+        v = int(self.counters[7].item())
+        return int(self.counters[6].item() >> 32) & 0xFFFFFFFF, (v >> 32) & 0xFFFFFFFF
 
     def heavy_count(self) -> int:
         return int((self.counters[1].item() >> 32) & 0xFFFFFFFF)

@@ -5,6 +5,7 @@ import pytest
 import torch
 
 from p2cccd import scenes
+from p2cccd.autotune import SPLIT_CHOICES
 from p2cccd.pipeline import CcdPipeline
 
 pytestmark = pytest.mark.gpu
@@ -18,7 +19,7 @@ def test_tuned_configuration_is_exact_and_cached(gpu, tmp_path):
     p.step_plain()
     plain = p.results()
     info = p.autotune(SPACE, cache=tmp_path / "tune.json")
-    assert info["passed"] == info["tried"] == 16
+    assert info["passed"] == info["tried"] == 16 + len(SPLIT_CHOICES)  # the grid, then the split threshold
     assert (tmp_path / "tune.json").exists()
     p.step_batched()
     r = p.results()

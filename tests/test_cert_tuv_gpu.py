@@ -113,3 +113,33 @@ def test_single_lane_forms_walk_the_same_boxes(gpu):
             assert c.heavy_count() == 0
             _check(got, plain, (mode, is_ee))
             assert np.array_equal(got["boxes"], plain["boxes"]), (mode, is_ee)
+
+
+@pytest.mark.parametrize("split_at,tasks", [(32, 1 << 16), (32, 300), (512, 1 << 16)])
+def test_split_wave_pass_matches_oracle(gpu, split_at, tasks):
+    # Pairs over split_at boxes in the wave pass are cut into pieces that
+    # many waves certify with a shared best; with room for only 300 pieces
+    # most splits are refused and those pairs finish in their own wave.
+    qs = workloads.make_query_set(3000, 3000, seed=53)
+    cert = TuvCertifier(TuvConfig(light_budget=8, split_at=split_at, split_tasks=tasks))
+    for is_ee, x in ((False, qs.vf), (True, qs.ee)):
+        got = cert.certify_positions(x, is_ee, "stream")
+        ref = oracle.tuv_cert(is_ee, x, threads=8)
+        _check(got, ref, (split_at, tasks, is_ee))
+        pieces, split = cert.split_counts()
+        assert split > 0 and 0 < pieces <= tasks and cert.replay_count() == 0
+
+
+def test_split_wave_pass_forced_replay(gpu):
+    # Split pairs whose pieces run out of budget (or whose total does) go to
+    # the canonical replay like any other over-budget pair.
+    qs = workloads.make_query_set(2000, 2000, seed=31)
+    cfg = TuvConfig(max_itr=40, heap_cap=1 << 12, replay_slots=16, light_budget=4, split_at=8)
+    cert = TuvCertifier(cfg)
+    for is_ee, x in ((False, qs.vf), (True, qs.ee)):
+        got = cert.certify_positions(x, is_ee, "stream")
+        ref = oracle.tuv_cert(is_ee, x, max_itr=40, heap_cap=1 << 12, threads=8)
+        _check(got, ref, is_ee)
+        replayed = (got["flags"] & FLAG_REPLAYED) != 0
+        assert replayed.sum() == cert.replay_count() > 0
+        assert cert.split_counts()[1] > 0
