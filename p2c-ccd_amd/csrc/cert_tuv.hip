@@ -636,7 +636,7 @@ __device__ inline SplitTask make_task(uint32_t pair, const uint64_t k[3], uint32
 // result is the minimum t_lo over done boxes, so the pieces' order is free.
 enum : int { kWaveDone = 0, kWaveOver = 1, kWaveSplit = 2 };
 
-template <bool kTask>
+template <bool kTask, bool kSplit>
 __device__ int wave_certify(const TuvPair& q, uint32_t p, WaveTask* wq, unsigned long long* best_lds, LaneWalk& w,
                             bool has, uint32_t root, uint32_t root_ep, uint64_t& boxes, double tol,
                             uint64_t max_itr, const SplitOut& so) {
@@ -683,7 +683,7 @@ __device__ int wave_certify(const TuvPair& q, uint32_t p, WaveTask* wq, unsigned
       if (shared_best < w.best) w.best = shared_best;
     }
     if (boxes > max_itr) return kWaveOver;
-    if (!kTask && so.split_at != 0 && boxes > so.split_at) {
+    if (kSplit && boxes > so.split_at) {
       // count the pieces: per lane its path siblings and its own walk, plus the queue
       const uint32_t pend = has && w.ready ? pending_siblings(w, root, root_ep) : 0u;
       const uint32_t mine = has ? pend + 1u : 0u;  // at most kTuvMaxLevel + 1 < 256
@@ -807,6 +807,7 @@ __device__ int wave_certify(const TuvPair& q, uint32_t p, WaveTask* wq, unsigned
   }
 }
 
+template <bool kSplit>
 __global__ __launch_bounds__(kTuvBlock) void tuv_heavy_kernel(const TuvPair* __restrict__ pairs,
                                                               const uint32_t* __restrict__ heavy_list,
                                                               const uint32_t* __restrict__ heavy_count,
@@ -838,7 +839,8 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_heavy_kernel(const TuvPair* __r
     }
     if (lane == 0) best_slot[wv] = w.best;
     wave_sync_lds();
-    const int r = wave_certify<false>(q, p, queue[wv], &best_slot[wv], w, lane == 0, 0, 0, boxes, tol, max_itr, so);
+    const int r =
+        wave_certify<false, kSplit>(q, p, queue[wv], &best_slot[wv], w, lane == 0, 0, 0, boxes, tol, max_itr, so);
     if (r != kWaveSplit) {
       const uint64_t final_best = best_slot[wv] < w.best ? best_slot[wv] : w.best;
       if (lane == 0) {
@@ -882,8 +884,8 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_task_kernel(const TuvPair* __re
     if (lane == 0) best_slot[wv] = w.best;
     wave_sync_lds();
     uint64_t boxes = 0;
-    const int r = wave_certify<true>(q, p, queue[wv], &best_slot[wv], w, has, t.root, root_ep, boxes, tol, max_itr,
-                                     so);
+    const int r = wave_certify<true, false>(q, p, queue[wv], &best_slot[wv], w, has, t.root, root_ep, boxes, tol,
+                                            max_itr, so);
     if (lane == 0) {
       const uint64_t b = best_slot[wv] < w.best ? best_slot[wv] : w.best;
       atomicMin(&so.acc[p].best, (unsigned long long)b);
@@ -1118,7 +1120,7 @@ int p2c_tuv_persistent(const void* pairs, int64_t n, const uint64_t* totals, dou
   hipLaunchKernelGGL(p2c::tuv_persistent_kernel, dim3(grid_blocks), dim3(p2c::kTuvBlock), 0, s,
                      static_cast<const p2c::TuvPair*>(pairs), n, totals, tolerance, (uint64_t)light_budget, out,
                      c64, heavy_list, c32 + 3, list_cap);
-  hipLaunchKernelGGL(p2c::tuv_heavy_kernel, dim3(grid_blocks), dim3(p2c::kTuvBlock), 0, s,
+  hipLaunchKernelGGL(p2c::tuv_heavy_kernel<false>, dim3(grid_blocks), dim3(p2c::kTuvBlock), 0, s,
                      static_cast<const p2c::TuvPair*>(pairs), heavy_list, c32 + 3, list_cap, c32 + 4, nullptr,
                      tolerance, (uint64_t)max_itr, out, replay_list, c32 + 2, p2c::SplitOut{});
   P2C_HIP_CHECK(hipGetLastError());
@@ -1153,7 +1155,7 @@ int p2c_tuv_rounds(const void* pairs, int64_t n, const uint64_t* totals, double 
                        c32 + 2, list_cap);
   }
   const uint32_t* heavy = lists[(n_rounds - 1) & 1];
-  hipLaunchKernelGGL(p2c::tuv_heavy_kernel, dim3(grid_blocks), dim3(p2c::kTuvBlock), 0, s, tp, heavy, c32 + 3,
+  hipLaunchKernelGGL(p2c::tuv_heavy_kernel<false>, dim3(grid_blocks), dim3(p2c::kTuvBlock), 0, s, tp, heavy, c32 + 3,
                      list_cap, c32 + 4, st, tolerance, (uint64_t)max_itr, out, replay_list, c32 + 2, p2c::SplitOut{});
   P2C_HIP_CHECK(hipGetLastError());
   return P2C_OK;
@@ -1215,9 +1217,13 @@ int p2c_tuv_wave_split(const void* pairs, void* counters, void* states, const ui
     so.split_count = c32 + 15;
     so.acc = static_cast<p2c::SplitAcc*>(states);
   }
-  hipLaunchKernelGGL(p2c::tuv_heavy_kernel, dim3(grid_blocks), dim3(p2c::kTuvBlock), 0, s, tp, heavy_list, c32 + 3,
-                     list_cap, c32 + 4, static_cast<const p2c::WalkState*>(states), tolerance, (uint64_t)max_itr, out,
-                     replay_list, c32 + 2, so);
+  const auto* resume = static_cast<const p2c::WalkState*>(states);
+  if (split)
+    hipLaunchKernelGGL(p2c::tuv_heavy_kernel<true>, dim3(grid_blocks), dim3(p2c::kTuvBlock), 0, s, tp, heavy_list,
+                       c32 + 3, list_cap, c32 + 4, resume, tolerance, (uint64_t)max_itr, out, replay_list, c32 + 2, so);
+  else
+    hipLaunchKernelGGL(p2c::tuv_heavy_kernel<false>, dim3(grid_blocks), dim3(p2c::kTuvBlock), 0, s, tp, heavy_list,
+                       c32 + 3, list_cap, c32 + 4, resume, tolerance, (uint64_t)max_itr, out, replay_list, c32 + 2, so);
   if (split) {
     hipLaunchKernelGGL(p2c::tuv_task_kernel, dim3(grid_blocks), dim3(p2c::kTuvBlock), 0, s, tp, c32 + 14, tolerance,
                        (uint64_t)max_itr, so);

@@ -154,7 +154,7 @@ class CcdPipeline:
         cfg, trials = tuner.tune(space)
         self.graph = None
         return {"refill_at": cfg.refill_at, "light_budget": cfg.light_budget, "handover_at": cfg.handover_at,
-                "blocks_per_cu": cfg.blocks_per_cu, **summarize(trials)}
+                "blocks_per_cu": cfg.blocks_per_cu, "split_at": cfg.split_at, **summarize(trials)}
 
     # -- results ------------------------------------------------------------
     def results(self) -> dict:
