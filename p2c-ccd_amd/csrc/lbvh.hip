@@ -420,6 +420,41 @@ __global__ __launch_bounds__(kTopBlock) void refit_top_kernel(Node* __restrict__
   }
 }
 
+// The same climb with the arrival flags as bits in LDS (64 KB: trees of up to
+// 524,288 internal nodes): an LDS atomic per level instead of an L2 one.
+// Workgroup-scope ordering covers the child boxes written to global memory by
+// the other lanes of this workgroup, as in refit_top_kernel.
+constexpr int kTopLdsWords = 16384;
+
+__global__ __launch_bounds__(kTopBlock) void refit_top_lds_kernel(Node* __restrict__ nodes,
+                                                                  const int32_t* __restrict__ parent,
+                                                                  const uint32_t* __restrict__ roots,
+                                                                  int64_t n_internal) {
+  __shared__ uint32_t bits[kTopLdsWords];
+  const int words = int((n_internal + 31) / 32);
+  for (int i = threadIdx.x; i < words; i += blockDim.x) bits[i] = 0;
+  __syncthreads();
+  const uint32_t total = roots[0];
+  for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) {
+    int32_t node = parent[roots[2 + i]];
+    while (node >= 0) {
+      const uint32_t bit = 1u << (node & 31);
+      const uint32_t prev =
+          __hip_atomic_fetch_or(&bits[node >> 5], bit, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (!(prev & bit)) break;
+      const Node& a = nodes[nodes[node].left];
+      const Node& b = nodes[nodes[node].right];
+      for (int k = 0; k < 3; ++k) {
+        const double amn = a.mn[k], bmn = b.mn[k];
+        const double amx = a.mx[k], bmx = b.mx[k];
+        nodes[node].mn[k] = bmn < amn ? bmn : amn;
+        nodes[node].mx[k] = amx < bmx ? bmx : amx;
+      }
+      node = parent[node];
+    }
+  }
+}
+
 // ---- 4. traversal ------------------------------------------------------
 struct VfRule {
   const int32_t* faces;
@@ -692,7 +727,10 @@ int p2c_bp_build(const p2c_bp_mesh* m, const double* x0, const double* x1, doubl
       hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, roots, 1);
       hipLaunchKernelGGL(refit_kernel, dim3(unsigned((t.n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, t.n, nodes,
                          parent, at<int32_t>(ws, t.lo), at<int32_t>(ws, t.hi), roots);
-      hipLaunchKernelGGL(refit_top_kernel, dim3(1), dim3(kTopBlock), 0, s, nodes, parent, roots, flag);
+      if (t.n - 1 <= int64_t(kTopLdsWords) * 32)
+        hipLaunchKernelGGL(refit_top_lds_kernel, dim3(1), dim3(kTopBlock), 0, s, nodes, parent, roots, t.n - 1);
+      else
+        hipLaunchKernelGGL(refit_top_kernel, dim3(1), dim3(kTopBlock), 0, s, nodes, parent, roots, flag);
     }
     // the edge leaf order (sorted edge ids) is also the EE query order
     if (t.kind == 1)
