@@ -89,7 +89,8 @@ def test_stream_refill_and_handover_match_oracle(gpu, refill_at, light_budget, h
     # pass; handover_at 64 hands every walk still open when the pairs run
     # out (some between two boxes) to the wave pass.
     qs = workloads.make_query_set(3000, 3000, seed=67)
-    cert = TuvCertifier(TuvConfig(light_budget=light_budget, refill_at=refill_at, handover_at=handover_at))
+    cert = TuvCertifier(TuvConfig(light_budget=light_budget, refill_at=refill_at, handover_at=handover_at,
+                                  small_batch_budget=light_budget))
     for is_ee, x in ((False, qs.vf), (True, qs.ee)):
         got = cert.certify_positions(x, is_ee, "stream")
         ref = oracle.tuv_cert(is_ee, x, threads=8)
@@ -106,7 +107,8 @@ def test_single_lane_forms_walk_the_same_boxes(gpu):
     qs = workloads.make_query_set(4000, 4000, seed=73)
     for is_ee, x in ((False, qs.vf), (True, qs.ee)):
         plain = TuvCertifier().certify_positions(x, is_ee, "plain")
-        big = TuvConfig(light_budget=1_000_000, round_budgets=(1_000_000,), handover_at=0)
+        big = TuvConfig(light_budget=1_000_000, small_batch_budget=1_000_000, round_budgets=(1_000_000,),
+                        handover_at=0)
         for mode in ("stream", "rounds"):
             c = TuvCertifier(big)
             got = c.certify_positions(x, is_ee, mode)
@@ -143,3 +145,15 @@ def test_split_wave_pass_forced_replay(gpu):
         replayed = (got["flags"] & FLAG_REPLAYED) != 0
         assert replayed.sum() == cert.replay_count() > 0
         assert cert.split_counts()[1] > 0
+
+
+def test_small_batch_budget_matches_oracle(gpu):
+    # A batch with fewer pairs than the grid has lanes walks at most
+    # small_batch_budget boxes per lane before the wave pass takes over.
+    qs = workloads.make_query_set(3000, 3000, seed=71)
+    for budget in (1, 32):
+        cert = TuvCertifier(TuvConfig(small_batch_budget=budget))
+        for is_ee, x in ((False, qs.vf), (True, qs.ee)):
+            got = cert.certify_positions(x, is_ee, "stream")
+            _check(got, oracle.tuv_cert(is_ee, x, threads=8), (budget, is_ee))
+            assert cert.heavy_count() > 0 and cert.replay_count() == 0
