@@ -66,6 +66,22 @@ __device__ inline void store_pair(TuvPair* __restrict__ base, int64_t p, const T
   for (int i = 0; i < kPairGranules; ++i) dst[int64_t(i) * kPairTile] = g[i];
 }
 
+// Setup's store: the records are read back once, by the certificate kernels,
+// long after (the whole buffer exceeds the caches), so the writes stream past
+// the caches (non-temporal).
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ inline void store_pair_streaming(TuvPair* __restrict__ base, int64_t p, const TuvPair& q) {
+  Granule g[kPairGranules];
+  __builtin_memcpy(g, &q, sizeof(TuvPair));
+  Granule* dst = const_cast<Granule*>(pair_granules(base, p));
+#pragma unroll
+  for (int i = 0; i < kPairGranules; ++i) {
+    const u32x4 v = {g[i].w[0], g[i].w[1], g[i].w[2], g[i].w[3]};
+    __builtin_nontemporal_store(v, reinterpret_cast<u32x4*>(dst + int64_t(i) * kPairTile));
+  }
+}
+
 __device__ inline void write_result(p2c_tuv_result* out, uint64_t best, uint64_t boxes, uint32_t flags) {
   p2c_tuv_result r;
   r.hit = best != kTuvNoHit ? 1u : 0u;
@@ -154,7 +170,7 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_setup_pairs_kernel(int is_ee, c
       }
     TuvPair q;
     tuv_setup(&q, x, is_ee, ms, tol);
-    if (off + i < out_cap) store_pair(out, off + i, q);
+    if (off + i < out_cap) store_pair_streaming(out, off + i, q);
   }
 }
 
