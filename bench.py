@@ -54,6 +54,42 @@ def profiled_traffic(kernel: str):
     return None
 
 
+def stream_roofline(pipe, steps: int):
+    """The stream kernel's algorithmic FP64 rate over `steps` event-timed
+    batched steps: boxes it evaluated (every pair's boxes minus what the wave
+    pass added to the walks it took over) x FLOPs per box, over its average
+    duration.  Returns (roofline dict, certificate-stage ms list, pairs handed
+    to the wave pass)."""
+    import numpy as np
+    import torch
+
+    stream_ms, stage_ms = [], []
+    for _ in range(steps):
+        ev = tuple(torch.cuda.Event(enable_timing=True) for _ in range(3))
+        pipe.step_batched_timed(ev)
+        stream_ms.append(ev[0].elapsed_time(ev[2]))
+        stage_ms.append(ev[0].elapsed_time(ev[1]))
+    rt = pipe.results()
+    n_vf = pipe.last.n_vf
+    boxes = np.concatenate([rt["vf_res"]["boxes"], rt["ee_res"]["boxes"]]).astype(np.int64)
+    cert = pipe.cert
+    hc = cert.heavy_count()
+    heavy = cert.heavy_list[:hc].long()
+    # the box count a handed-over walk had reached (WalkState.boxes, byte 32)
+    saved = cert.states.view(-1, cert.state_bytes)[heavy][:, 32:36].contiguous().view(torch.int32)
+    heavy = heavy.cpu().numpy()
+    wave_part = np.maximum(boxes[heavy] - saved.cpu().numpy()[:, 0].astype(np.int64), 0)
+    is_vf = np.arange(len(boxes)) < n_vf
+    stream_vf = int(boxes[is_vf].sum() - wave_part[heavy < n_vf].sum())
+    stream_ee = int(boxes[~is_vf].sum() - wave_part[heavy >= n_vf].sum())
+    flops = stream_vf * FLOP_BOX_VF + stream_ee * FLOP_BOX_EE
+    avg = float(np.mean(stream_ms))
+    achieved = flops / (avg * 1e-3) / 1e12
+    return ({"bound": "fp64-valu", "kernel": "tuv_stream_kernel", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+             "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "avg_ms": avg,
+             "flops_per_launch": flops, "boxes_per_launch": stream_vf + stream_ee}, stage_ms, hc)
+
+
 def percentile(v, q):
     import numpy as np
 
@@ -163,6 +199,8 @@ def rows_suite() -> int:
     its roofline where the work has a closed-form count, and the reference's
     CPU path (compiled in place where it builds, else the oracle restatement)
     timed on this host, one thread, with a parity check of the two outputs."""
+    import ctypes
+
     import numpy as np
     import torch
 
@@ -237,13 +275,14 @@ def rows_suite() -> int:
     pipe = CcdPipeline(sc, device=dev)
     pipe.step_batched()
     lat = pipe.timed(pipe.step_batched, 10)
+    rl, _, _ = stream_roofline(pipe, 5)
     pipe.step_plain()
     plain = pipe.timed(pipe.step_plain, 3)
     npairs = pipe.last.pairs
     cpu = cpu_baseline(sc, 100_000)
     _row("icospheres_step", f"{sc.name}: {npairs} candidates per step",
          {"value": npairs / (np.mean(lat) * 1e-3), "unit": "pairs/s", "ms_per_step": float(np.mean(lat)),
-          "p99_ms": percentile(lat, 99), "plain_ms_per_step": float(np.mean(plain))}, cpu, None,
+          "p99_ms": percentile(lat, 99), "plain_ms_per_step": float(np.mean(plain))}, cpu, rl,
          "covered by tests/test_pipeline_gpu.py (identical decisions and TOI against the oracle)")
 
     # learned group scheduling: the two scorer kernels
@@ -311,6 +350,16 @@ def rows_suite() -> int:
         holder["cand"] = proxy.compact(91, arr, raw)
     gen()
     ms = _gpu_ms(gen, 3)
+    # the count pass alone, for its roofline: 8 FP64 compares per pair test (two
+    # time, six box), at the FP64 VALU op rate (half the FMA-counted peak)
+    n20 = arr["t0"].numel()
+    ps = proxy._set(91, arr)
+    counts = torch.zeros(n20, dtype=torch.int32, device=dev)
+    wsb = int(nat.lib.p2c_proxy_raw_workspace_bytes(n20))
+    ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
+    ms_count = _gpu_ms(lambda: nat.call("p2c_proxy_raw_count", ctypes.addressof(ps), counts.data_ptr(),
+                                        ws.data_ptr(), wsb, nat.stream_handle()), 5)
+    cmp_rate = (n20 * (n20 - 1) // 2) * 8 / (ms_count * 1e-3) / 1e12
     compiled = oracle.ref_available()
     f = oracle.ref_proxy_generate if compiled else oracle.proxy_generate
     ref = [None]
@@ -321,10 +370,13 @@ def rows_suite() -> int:
     npair = 20_000 * 19_999 // 2
     _row("proxy_scene_candidates", f"20,000 proxies, 8 objects, 4 slabs: {len(r_raw)} raw hits, "
          f"{len(r_cand)} candidates",
-         {"value": npair / (ms * 1e-3), "unit": "pair tests/s", "ms": ms},
+         {"value": npair / (ms * 1e-3), "unit": "pair tests/s", "ms": ms, "count_pass_ms": ms_count},
          {"value": npair / t_cpu, "unit": "pair tests/s", "cores": 1, "kind": "reference" if compiled else "port",
           "sample": "the whole scene through " + ("the compiled candidate_buffer.cpp" if compiled else "the oracle")},
-         None, {"identical_counts_and_ids": same})
+         {"bound": "fp64-valu", "kernel": "proxy_rows_kernel (count pass)", "achieved": cmp_rate,
+          "peak": FP64_PEAK_TFLOPS / 2, "unit": "T compares/s", "frac": cmp_rate / (FP64_PEAK_TFLOPS / 2),
+          "traffic": None},
+         {"identical_counts_and_ids": same})
 
     # serving: Poisson requests over 256 resident scenes
     log("rows: serving")
@@ -409,30 +461,8 @@ def main() -> int:
 
     # dominant kernel: the stream certificate kernel, timed with HIP events on
     # its launch stream (the current torch stream), then the wave pass
-    stream_ms, stage_ms = [], []
-    for _ in range(args.steps):
-        ev = tuple(torch.cuda.Event(enable_timing=True) for _ in range(3))
-        pipe.step_batched_timed(ev)
-        stream_ms.append(ev[0].elapsed_time(ev[2]))
-        stage_ms.append(ev[0].elapsed_time(ev[1]))
-    rt = pipe.results()
-    n_vf = pipe.last.n_vf
-    boxes = np.concatenate([rt["vf_res"]["boxes"], rt["ee_res"]["boxes"]]).astype(np.int64)
-    # boxes the wave pass added for the pairs it took over (their saved walks
-    # hold the count the stream kernel reached)
-    cert = pipe.cert
-    hc = cert.heavy_count()
-    heavy = cert.heavy_list[:hc].long()
-    saved = cert.states.view(-1, cert.state_bytes)[heavy][:, 32:36].contiguous().view(torch.int32)
-    heavy = heavy.cpu().numpy()
-    wave_part = np.maximum(boxes[heavy] - saved.cpu().numpy()[:, 0].astype(np.int64), 0)
-    is_vf = np.arange(len(boxes)) < n_vf
-    stream_vf = int(boxes[is_vf].sum() - wave_part[heavy < n_vf].sum())
-    stream_ee = int(boxes[~is_vf].sum() - wave_part[heavy >= n_vf].sum())
-    flops = stream_vf * FLOP_BOX_VF + stream_ee * FLOP_BOX_EE
-    stream_avg_ms = float(np.mean(stream_ms))
-    achieved = flops / (stream_avg_ms * 1e-3) / 1e12
-    log(f"stream kernel {stream_avg_ms:.2f} ms ({stream_vf + stream_ee} boxes), certificate stage "
+    rl, stage_ms, hc = stream_roofline(pipe, args.steps)
+    log(f"stream kernel {rl['avg_ms']:.2f} ms ({rl['boxes_per_launch']} boxes), certificate stage "
         f"{np.mean(stage_ms):.2f} ms, wave pass took {hc} pairs")
 
     # plain one-launch-per-stage form, same scene
@@ -476,19 +506,9 @@ def main() -> int:
         "speedup_vs_plain": (pairs / total_s) / plain_rate,
         "certificate_ms": float(np.mean(stage_ms)),
         "wave_pass_pairs": hc,
-        "roofline": {
-            "bound": "fp64-valu",
-            "kernel": "tuv_stream_kernel",
-            "achieved": achieved,
-            "peak": FP64_PEAK_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": achieved / FP64_PEAK_TFLOPS,
-            "traffic": profiled_traffic("tuv_stream_kernel") if args.workload == "cloth" else None,
-            "traffic_source": "rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE (profiles/hbm_traffic_cloth256_step.md)",
-            "avg_ms": stream_avg_ms,
-            "flops_per_launch": flops,
-            "boxes_per_launch": stream_vf + stream_ee,
-        },
+        "roofline": dict(rl, traffic=profiled_traffic("tuv_stream_kernel") if args.workload == "cloth" else None,
+                         traffic_source="rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE "
+                                        "(profiles/hbm_traffic_cloth256_step.md)"),
     }
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle, 1 thread)")

@@ -601,15 +601,16 @@ struct SplitTask {
 };
 static_assert(sizeof(SplitTask) == 40, "split task layout");
 
-// A split pair's accumulators live in its WalkState slot once the walk it
-// held has been resumed: best (u64 at byte 24, atomicMin), boxes (u32 at byte
-// 32, atomicAdd), over-budget flag (u32 at byte 36, atomicOr).
+// A split pair's accumulators live in the key bytes of its WalkState slot
+// once the walk it held has been resumed: best (u64 at byte 0, atomicMin),
+// boxes (u32 at byte 8, atomicAdd), over-budget flag (u32 at byte 12,
+// atomicOr).  The rest of the slot, with the box count the stream kernel had
+// reached, is left as it was.
 struct SplitAcc {
-  uint64_t pad[3];
   unsigned long long best;
   uint32_t boxes;
   uint32_t over;
-  uint64_t pad2;
+  uint64_t keep[4];
 };
 static_assert(sizeof(SplitAcc) == sizeof(WalkState), "split accumulators reuse the walk state slot");
 
