@@ -285,6 +285,33 @@ def rows_suite() -> int:
           "p99_ms": percentile(lat, 99), "plain_ms_per_step": float(np.mean(plain))}, cpu, rl,
          "covered by tests/test_pipeline_gpu.py (identical decisions and TOI against the oracle)")
 
+    # candidate sets alone (swept boxes, LBVH build, count, fill) against the
+    # reference's sort-and-sweep, restated by the oracle, on the same scene
+    log("rows: candidate sets (icospheres)")
+    from p2cccd.broadphase import BroadPhase
+
+    bp = BroadPhase.for_scene(sc, dev)
+    x0d, x1d = torch.from_numpy(sc.x0).to(dev), torch.from_numpy(sc.x1).to(dev)
+    vf_d, ee_d = bp.candidates(x0d, x1d, sc.thickness)
+
+    def sets():
+        bp.build(x0d, x1d, sc.thickness)
+        bp.count()
+        bp.emit(vf_d, ee_d)
+    ms_bp = _gpu_ms(sets, 5)
+    t_c = time.perf_counter()
+    ref_c = oracle.candidates(sc)
+    t_sweep = time.perf_counter() - t_c
+    same_sets = all(np.array_equal(np.unique(g.cpu().numpy(), axis=0), np.unique(ref_c[k], axis=0))
+                    for k, g in (("vf", vf_d), ("ee", ee_d)))
+    n_c = len(ref_c["vf"]) + len(ref_c["ee"])
+    _row("candidate_sets_icospheres", f"{sc.name}: swept boxes, two LBVHs, {n_c} VF+EE candidates",
+         {"value": n_c / (ms_bp * 1e-3), "unit": "candidates/s", "ms": ms_bp},
+         {"value": n_c / t_sweep, "unit": "candidates/s", "cores": 1, "kind": "port",
+          "sample": f"the oracle's sort-and-sweep over the whole scene ({t_sweep:.1f} s)"},
+         None, {"identical_sets": same_sets})
+    del bp
+
     # learned group scheduling: the two scorer kernels
     log("rows: scorer kernels")
     packed = stpf.load_stpf(stpf.DEFAULT_PRIORITY_WEIGHTS, dev)

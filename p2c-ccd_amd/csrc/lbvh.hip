@@ -524,7 +524,8 @@ __global__ __launch_bounds__(kBlock) void vf_query_kernel(const Node* __restrict
                                                           int32_t* __restrict__ resume,
                                                           const uint64_t* __restrict__ offsets,
                                                           int32_t* __restrict__ out, int64_t cap) {
-  for (int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; s < nv; s += int64_t(gridDim.x) * blockDim.x) {
+  const int64_t b0 = xcd_block(blockIdx.x, gridDim.x);  // Morton-neighbouring queries on one XCD
+  for (int64_t s = b0 * blockDim.x + threadIdx.x; s < nv; s += int64_t(gridDim.x) * blockDim.x) {
     if (kFill && offsets[s + 1] - offsets[s] <= uint64_t(slots)) continue;  // copied from the cache
     // a query with more hits resumes after its last cached one
     const int32_t start = kFill && slots > 0 ? resume[s] : 0;
@@ -569,7 +570,8 @@ __global__ __launch_bounds__(kBlock) void ee_query_kernel(const Node* __restrict
                                                           int64_t slots, int32_t* __restrict__ resume,
                                                           const uint64_t* __restrict__ offsets,
                                                           int64_t nv, int32_t* __restrict__ out, int64_t cap) {
-  for (int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; s < ne; s += int64_t(gridDim.x) * blockDim.x) {
+  const int64_t b0 = xcd_block(blockIdx.x, gridDim.x);  // Morton-neighbouring queries on one XCD
+  for (int64_t s = b0 * blockDim.x + threadIdx.x; s < ne; s += int64_t(gridDim.x) * blockDim.x) {
     if (kFill && offsets[nv + s + 1] - offsets[nv + s] <= uint64_t(slots)) continue;  // copied from the cache
     const int32_t i = int32_t(eorder[s]);
     const Box6 q = ebox[i];

@@ -24,6 +24,18 @@ inline unsigned grid_for(int64_t n, int block, int64_t max_blocks = int64_t(1) <
   return static_cast<unsigned>(g);
 }
 
+// Workgroups are dispatched to the 8 XCDs of an MI355X round-robin (block i
+// to XCD i % 8).  This maps block i to a logical block so that each XCD gets
+// one contiguous run of logical blocks: kernels whose neighbouring blocks
+// touch neighbouring data (Morton-sorted queries) then keep each XCD's L2 on
+// one region.  A bijection of [0, grid) for any grid size.
+constexpr unsigned kXcds = 8;
+
+__device__ inline unsigned xcd_block(unsigned i, unsigned grid) {
+  const unsigned per = grid / kXcds, rem = grid % kXcds, x = i % kXcds;
+  return x * per + (x < rem ? x : rem) + i / kXcds;
+}
+
 }  // namespace p2c
 
 #define P2C_HIP_CHECK(expr)                         \
