@@ -824,8 +824,10 @@ __device__ int wave_certify(const TuvPair& q, uint32_t p, WaveTask* wq, unsigned
   }
 }
 
+// Three workgroups per CU (12 waves: 3 per SIMD; 147 KB of LDS): at most 168
+// VGPRs, a few spilled; the wave pass ran 20 % longer at 2 waves per SIMD.
 template <bool kSplit>
-__global__ __launch_bounds__(kTuvBlock) void tuv_heavy_kernel(const TuvPair* __restrict__ pairs,
+__global__ __launch_bounds__(kTuvBlock, 3) void tuv_heavy_kernel(const TuvPair* __restrict__ pairs,
                                                               const uint32_t* __restrict__ heavy_list,
                                                               const uint32_t* __restrict__ heavy_count,
                                                               uint32_t list_cap, uint32_t* __restrict__ heavy_next,
@@ -871,7 +873,7 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_heavy_kernel(const TuvPair* __r
 
 // Split tasks, one wave per task; boxes and an over-budget mark go to the
 // pair's accumulators.
-__global__ __launch_bounds__(kTuvBlock) void tuv_task_kernel(const TuvPair* __restrict__ pairs,
+__global__ __launch_bounds__(kTuvBlock, 3) void tuv_task_kernel(const TuvPair* __restrict__ pairs,
                                                              uint32_t* __restrict__ task_next, double tol,
                                                              uint64_t max_itr, SplitOut so) {
   __shared__ WaveTask queue[kTuvBlock / 64][kWaveQueue];
