@@ -21,8 +21,8 @@ Per-pair answers do not depend on what else shares a batch: candidate order
 is fixed per scene and the certificate's result per pair is order-free, so a
 request's result is bit-identical alone and in the mix (tests check it).
 
-The workload of the serving benchmark: resident
-scenes, each a pair of jittered icospheres or grid meshes with log-uniform
+The workload of the serving benchmark (256 resident scenes of 1k-500k
+triangles by default): resident scenes, each a pair of jittered icospheres or grid meshes with log-uniform
 triangle counts; Poisson arrivals; each request one step of random rigid
 motion plus per-vertex N(0, 1e-3) noise.
 """
