@@ -21,9 +21,9 @@ Per-pair answers do not depend on what else shares a batch: candidate order
 is fixed per scene and the certificate's result per pair is order-free, so a
 request's result is bit-identical alone and in the mix (tests check it).
 
-The workload of the serving benchmark (256 resident scenes of 1k-500k
-triangles by default): resident scenes, each a pair of jittered icospheres or grid meshes with log-uniform
-triangle counts; Poisson arrivals; each request one step of random rigid
+The workload of the serving benchmark: resident scenes (256 by default),
+each a pair of jittered icospheres or grid meshes with log-uniform triangle
+counts (1k-500k); Poisson arrivals; each request one step of random rigid
 motion plus per-vertex N(0, 1e-3) noise.
 """
 from __future__ import annotations
