@@ -273,7 +273,10 @@ __device__ inline bool walk_advance(const TuvPair& q, LaneWalk& w, uint32_t root
 // long climb costs the wave a few cheap moves per step rather than a serial
 // loop.  Returns true when the walk has finished; sets *over (leaving the box
 // unevaluated) when the box budget ran out.
-constexpr int kAdvanceMoves = 2;
+#ifndef P2C_ADVANCE_MOVES
+#define P2C_ADVANCE_MOVES 2
+#endif
+constexpr int kAdvanceMoves = P2C_ADVANCE_MOVES;
 
 __device__ inline bool lane_step(const TuvPair& q, LaneWalk& w, double tol, uint64_t budget, bool* over) {
   if (w.ready) {
