@@ -215,11 +215,21 @@ def rows_suite() -> int:
 
     # (t,u,v) certificate on the plain query set (10k VF + 10k EE)
     log("rows: (t,u,v) certificate, 10k VF + 10k EE queries")
-    cert = TuvCertifier(device=dev)
+    import tempfile
+
+    from p2cccd.autotune import CertificateTuner, summarize
+
     for kind, x in (("vf", qs.vf), ("ee", qs.ee)):
+        cert = TuvCertifier(device=dev)
         xd = torch.from_numpy(np.ascontiguousarray(x)).to(dev)
+        n = cert.setup_positions(xd, kind == "ee")
+        # the same autotuner as the headline, on these pairs (every candidate must reproduce the plain kernel)
+        with tempfile.TemporaryDirectory() as td:
+            _, trials = CertificateTuner(cert, n, cache=os.path.join(td, "tune.json")).tune(use_cache=False)
+        tuned = {k: getattr(cert.cfg, k) for k in ("refill_at", "light_budget", "handover_at", "blocks_per_cu",
+                                                   "split_at")}
+        tuned.update(summarize(trials))
         res = cert.certify_positions(xd, kind == "ee")
-        n = xd.shape[0]
         ms = {m: _gpu_ms(lambda m=m: (cert.setup_positions(xd, kind == "ee"), cert.certify(n, m)))
               for m in ("stream", "plain")}
         boxes = int(res["boxes"].sum())
@@ -231,7 +241,7 @@ def rows_suite() -> int:
         ach = flops / (ms["stream"] * 1e-3) / 1e12
         _row(f"tuv_certificate_{kind}", f"plain query set, {n} {kind.upper()} queries (tolerance 1e-6, ms 0)",
              {"value": n / (ms["stream"] * 1e-3), "unit": "queries/s", "ms": ms["stream"],
-              "plain_form_ms": ms["plain"], "boxes": boxes},
+              "plain_form_ms": ms["plain"], "boxes": boxes, "autotuned": tuned},
              {"value": n / t_cpu, "unit": "queries/s", "cores": 1, "kind": "port",
               "sample": f"all {n} queries through the oracle's (t,u,v) restatement"},
              {"bound": "fp64-valu", "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",

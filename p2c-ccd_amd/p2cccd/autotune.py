@@ -26,7 +26,7 @@ from .tuv import TuvCertifier, TuvConfig
 
 SPACE = {
     "refill_at": (4, 8, 16),
-    "light_budget": (512, 1024, 2048),
+    "light_budget": (32, 512, 1024, 2048),
     "handover_at": (0, 8, 16),
     "blocks_per_cu": (2, 3, 4),
 }

@@ -187,8 +187,11 @@ class Lane:
 class Server:
     def __init__(self, scene_table: dict[int, scenes.Scene], hbm_budget: int, device="cuda",
                  max_batch_requests: int = 64, max_batch_pairs: int = 1 << 23, cfg: TuvConfig | None = None,
-                 lanes: int = 2, full_results: bool = False):
+                 lanes: int = 2, full_results: bool = False, side_streams: int = 4):
         self.full_results = full_results
+        # the per-scene stages of a batch (uploads, broad phase, pair setup) are spread over these streams so the
+        # many small kernels of small scenes run side by side; default priority, like every stream here
+        self.side = [torch.cuda.Stream(torch.device(device)) for _ in range(side_streams)]
         self.table = scene_table
         self.device = torch.device(device)
         self.res = Residency(hbm_budget)
@@ -238,49 +241,72 @@ class Server:
         queue.extendleft(reversed(skipped))
         return batch
 
+    def _streams(self, lane: Lane, k: int, nside: int):
+        return self.side[k % nside] if nside > 0 else lane.stream
+
+    def _fork(self, lane: Lane, n: int) -> None:
+        for s in self.side[:n]:
+            s.wait_stream(lane.stream)
+
+    def _join(self, lane: Lane, n: int) -> None:
+        for s in self.side[:n]:
+            lane.stream.wait_stream(s)
+
     def _launch(self, lane: Lane, reqs: list) -> None:
         b = _Batch(reqs=reqs)
         # positions go through the lane's pinned staging, so the copies are asynchronous
         sizes = [2 * self.table[r.scene_id].n_vertices * 3 for r in reqs]
         stage = lane.staging(sum(sizes))
         host = stage.numpy()
-        with torch.cuda.stream(lane.stream):
-            at = 0
-            for r, sz in zip(reqs, sizes):
-                rs = self.resident[r.scene_id]
-                half = sz // 2
-                host[at:at + half] = r.x0.reshape(-1)
-                host[at + half:at + sz] = r.x1.reshape(-1)
+        # only deep batches (a loaded server) fan out; a light load keeps each batch on its lane's stream, where
+        # it does not compete with the other lane's certificate
+        nside = min(len(self.side), len(reqs)) if len(reqs) >= 4 else 0
+        self._fork(lane, nside)
+        at = 0
+        for k, (r, sz) in enumerate(zip(reqs, sizes)):
+            rs = self.resident[r.scene_id]
+            st = self._streams(lane, k, nside)
+            half = sz // 2
+            host[at:at + half] = r.x0.reshape(-1)
+            host[at + half:at + sz] = r.x1.reshape(-1)
+            with torch.cuda.stream(st):
                 rs.x0.view(-1).copy_(stage[at:at + half], non_blocking=True)
                 rs.x1.view(-1).copy_(stage[at + half:at + sz], non_blocking=True)
-                at += sz
-                rs.bp.build(rs.x0, rs.x1, rs.scene.thickness, stream=lane.stream)
-                rs.bp.count(stream=lane.stream)
+                rs.bp.build(rs.x0, rs.x1, rs.scene.thickness, stream=st)
+                rs.bp.count(stream=st)
+            at += sz
+        self._join(lane, nside)
+        with torch.cuda.stream(lane.stream):
             totals = torch.stack([self.resident[r.scene_id].bp.totals for r in reqs]).to("cpu", non_blocking=True)
             lane.stream.synchronize()  # the counts size this batch; waits on this lane only
-            off = 0
-            for r, t in zip(reqs, totals.tolist()):
-                n_vf, n_ee, bad = t
-                if bad:
-                    raise RuntimeError(f"{bad} BVH traversals exceeded their step bound (scene {r.scene_id})")
-                rs = self.resident[r.scene_id]
+        off = 0
+        for r, t in zip(reqs, totals.tolist()):
+            n_vf, n_ee, bad = t
+            if bad:
+                raise RuntimeError(f"{bad} BVH traversals exceeded their step bound (scene {r.scene_id})")
+            rs = self.resident[r.scene_id]
+            with torch.cuda.stream(lane.stream):
                 rs.ensure(n_vf, n_ee)
-                self.res.bytes[r.scene_id] = rs.nbytes()  # the estimate, corrected by the real footprint
-                rs.bp.emit(rs.vf, rs.ee, stream=lane.stream)
-                b.offsets.append(off)
-                b.counts.append((n_vf, n_ee))
-                off += n_vf + n_ee
-                self.last_pairs[r.scene_id] = n_vf + n_ee
-            b.total = off
-            cert = lane.cert
+            self.res.bytes[r.scene_id] = rs.nbytes()  # the estimate, corrected by the real footprint
+            b.offsets.append(off)
+            b.counts.append((n_vf, n_ee))
+            off += n_vf + n_ee
+            self.last_pairs[r.scene_id] = n_vf + n_ee
+        b.total = off
+        cert = lane.cert
+        with torch.cuda.stream(lane.stream):
             cert.reserve(max(off, 1))
-            for r, o, (n_vf, n_ee) in zip(reqs, b.offsets, b.counts):
-                rs = self.resident[r.scene_id]
-                if n_vf:
-                    cert.setup_pairs(rs.x0, rs.x1, rs.bp.faces, rs.vf, False, n_vf, out_offset=o, stream=lane.stream)
-                if n_ee:
-                    cert.setup_pairs(rs.x0, rs.x1, rs.bp.edges, rs.ee, True, n_ee, out_offset=o + n_vf,
-                                     stream=lane.stream)
+        self._fork(lane, nside)
+        for k, (r, o, (n_vf, n_ee)) in enumerate(zip(reqs, b.offsets, b.counts)):
+            rs = self.resident[r.scene_id]
+            st = self._streams(lane, k, nside)
+            rs.bp.emit(rs.vf, rs.ee, stream=st)
+            if n_vf:
+                cert.setup_pairs(rs.x0, rs.x1, rs.bp.faces, rs.vf, False, n_vf, out_offset=o, stream=st)
+            if n_ee:
+                cert.setup_pairs(rs.x0, rs.x1, rs.bp.edges, rs.ee, True, n_ee, out_offset=o + n_vf, stream=st)
+        self._join(lane, nside)
+        with torch.cuda.stream(lane.stream):
             if off:
                 cert.certify(off, "stream", stream=lane.stream)
             b.event = torch.cuda.Event()

@@ -64,7 +64,7 @@ class TuvConfig:
     refill_at: int = 8           # stream: idle lanes of a wave that trigger a refill
     handover_at: int = 8         # stream: busy lanes of a drained wave at or below which all go to the wave pass
     split_at: int = 16384        # wave pass: boxes after which a pair is cut into pieces for many waves (0: never)
-    small_batch_budget: int = 32  # stream: light budget when the batch has fewer pairs than the grid has lanes
+    small_batch_budget: int = 0   # stream: light budget of a batch filling at most 1/8 of the lanes (0: off)
     split_tasks: int = 1 << 16   # capacity of the piece buffer
     round_budgets: tuple = (8, 32, 128, 512)  # walk steps per pair per round before the wave pass
 
@@ -149,10 +149,10 @@ class TuvCertifier:
                      self.states.data_ptr(), self.round_list.data_ptr(), self.heavy_list.data_ptr(),
                      self.replay_list.data_ptr(), self._cap, self.grid_blocks, s)
         elif mode == "stream":
-            # a batch smaller than the grid leaves lanes idle behind its longest walks: hand
-            # pairs to the wave-shared pass sooner
+            # optional: a batch that fills at most an eighth of the grid's lanes may use a smaller
+            # light budget, moving its longest walks to the wave-shared pass at once
             budget = self.cfg.light_budget
-            if n < self.grid_blocks * 256:
+            if self.cfg.small_batch_budget > 0 and n * 8 <= self.grid_blocks * 256:
                 budget = min(budget, self.cfg.small_batch_budget)
             nat.call("p2c_tuv_stream", self.pairs.data_ptr(), n, tp, self.cfg.tolerance, self.cfg.max_itr,
                      budget, self.cfg.refill_at, self.cfg.handover_at, self.results.data_ptr(),
