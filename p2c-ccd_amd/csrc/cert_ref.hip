@@ -246,167 +246,167 @@ __global__ __launch_bounds__(kHeavyBlock) void cert_ref_heavy_kernel(const p2c_p
   __shared__ HeavyStack st;
   const int lane = threadIdx.x;
   for (int64_t first = int64_t(blockIdx.x) * kHeavyChunk; first < n; first += int64_t(gridDim.x) * kHeavyChunk) {
-  const int64_t mine = first + lane;
-  uint64_t todo = __ballot(lane < kHeavyChunk && mine < n && out[mine].status == kHeavyMark);
-  while (todo) {
-    const int pick = __builtin_ctzll(todo);
-    todo &= todo - 1;
-    const int64_t i = first + pick;
-    const p2c_prim p = prims[i];
-    const RefPrim q = load_prim(p);
-    double vn[4];
-    vertex_norms(q, vn);
-    const double d0 = ref_distance<kFamily>(q, c.t0);
-    const double d1 = ref_distance<kFamily>(q, c.t1);
+    const int64_t mine = first + lane;
+    uint64_t todo = __ballot(lane < kHeavyChunk && mine < n && out[mine].status == kHeavyMark);
+    while (todo) {
+      const int pick = __builtin_ctzll(todo);
+      todo &= todo - 1;
+      const int64_t i = first + pick;
+      const p2c_prim p = prims[i];
+      const RefPrim q = load_prim(p);
+      double vn[4];
+      vertex_norms(q, vn);
+      const double d0 = ref_distance<kFamily>(q, c.t0);
+      const double d1 = ref_distance<kFamily>(q, c.t1);
 
-    bool have_best = false;
-    uint64_t best_path = 0;
-    uint32_t best_depth = 0;
-    double best_a = 0.0, best_b = 0.0, best_toi = 0.0;
-    uint32_t count = 0;  // per lane
-    double margin_min = 0.0;
-    bool have_margin = false, undecided = false;
+      bool have_best = false;
+      uint64_t best_path = 0;
+      uint32_t best_depth = 0;
+      double best_a = 0.0, best_b = 0.0, best_toi = 0.0;
+      uint32_t count = 0;  // per lane
+      double margin_min = 0.0;
+      bool have_margin = false, undecided = false;
 
-    for (int pass = 0; pass < 2; ++pass) {
-      if (pass == 1) {
-        if (!have_best) break;
-        count = 0;
-      }
-      __syncthreads();
-      if (lane == 0) {
-        st.path[0] = 0;
-        st.depth[0] = 0;
-        st.da[0] = d0;
-        st.db[0] = d1;
-      }
-      int size = 1;
-      __syncthreads();
-      while (size > 0) {
-        if (size + kHeavyBlock > kStackCap) {
-          // no room for a full round: lane 0 walks the leftmost pending
-          // subtree to its end (or to its first collision)
-          RefWalk w = {};
-          if (lane == 0)
-            w = walk_subtree<kFamily>(q, vn, c, st.path[size - 1], st.depth[size - 1], st.da[size - 1],
-                                      st.db[size - 1], 0xFFFFFFFFu);
-          const bool hit = __shfl(w.state == kWalkCollision ? 1 : 0, 0) != 0;
-          if (lane == 0) {
-            count += w.nodes;
-            if (pass == 0 && !hit) {
-              if (w.have_margin) {
-                margin_min = have_margin ? std_min(margin_min, w.min_margin) : w.min_margin;
-                have_margin = true;
-              }
-              undecided = undecided || w.undecided;
-            }
-          }
-          if (hit && pass == 0) {
-            // the earliest collision so far: everything still pending follows it
-            have_best = true;
-            best_path = __shfl(w.path, 0);
-            best_depth = __shfl(w.depth, 0);
-            best_a = __shfl(w.a, 0);
-            best_b = __shfl(w.b, 0);
-            best_toi = __shfl(w.toi, 0);
-            size = 0;
-          } else {
-            size -= 1;
-          }
-          __syncthreads();
-          continue;
+      for (int pass = 0; pass < 2; ++pass) {
+        if (pass == 1) {
+          if (!have_best) break;
+          count = 0;
         }
-        const int take = size < kHeavyBlock ? size : kHeavyBlock;
-        const bool act = lane < take;
-        uint64_t path = 0;
-        uint32_t depth = 0;
-        double da = 0.0, db = 0.0;
-        if (act) {
-          const int at = size - 1 - lane;
-          path = st.path[at];
-          depth = st.depth[at];
-          da = st.da[at];
-          db = st.db[at];
-        }
-        __syncthreads();  // every read of this round before any write
-        int o = kNodeUndecided;
-        double a = 0.0, b = 0.0, dm = 0.0, toi = 0.0, margin = 0.0;
-        if (act) {
-          node_interval(c, path, depth, &a, &b);
-          o = eval_node<kFamily>(q, vn, c, a, b, depth, da, db, &dm, &toi, &margin);
-        }
-        int keep = take;
-        int base = size - take;
-        if (pass == 0) {
-          const uint64_t hits = __ballot(act && o == kNodeCollision);
-          if (hits) {
-            const int j = __builtin_ctzll(hits);
-            keep = j;
-            base = 0;
-            have_best = true;
-            best_path = __shfl(path, j);
-            best_depth = __shfl(depth, j);
-            best_a = __shfl(a, j);
-            best_b = __shfl(b, j);
-            best_toi = __shfl(toi, j);
-          }
-          if (act && lane < keep) {
-            if (o == kNodeSeparated) {
-              margin_min = have_margin ? std_min(margin_min, margin) : margin;
-              have_margin = true;
-            } else if (o == kNodeUndecided) {
-              undecided = true;
-            }
-          }
-        }
-        if (act && lane <= keep) count += 1;  // pass 1's count is used only without a collision
-        const bool kept = act && lane < keep && o == kNodeExpand;
-        const uint64_t right = path | (1ull << depth);
-        bool push_l = kept, push_r = kept;
-        if (pass == 1 && kept) {
-          push_l = !node_after(path, depth + 1, best_path, best_depth);
-          push_r = !node_after(right, depth + 1, best_path, best_depth);
-        }
-        const uint64_t ml = __ballot(push_l), mr = __ballot(push_r);
-        const uint64_t below = lanes_below(lane);
-        const int before = __popcll(ml & below) + __popcll(mr & below);
-        const int total = __popcll(ml) + __popcll(mr);
-        int slot = base + total - 1 - before;  // leftmost on top
-        if (push_l) {
-          st.path[slot] = path;
-          st.depth[slot] = (uint8_t)(depth + 1);
-          st.da[slot] = da;
-          st.db[slot] = dm;
-          --slot;
-        }
-        if (push_r) {
-          st.path[slot] = right;
-          st.depth[slot] = (uint8_t)(depth + 1);
-          st.da[slot] = dm;
-          st.db[slot] = db;
-        }
-        size = base + total;
         __syncthreads();
+        if (lane == 0) {
+          st.path[0] = 0;
+          st.depth[0] = 0;
+          st.da[0] = d0;
+          st.db[0] = d1;
+        }
+        int size = 1;
+        __syncthreads();
+        while (size > 0) {
+          if (size + kHeavyBlock > kStackCap) {
+            // no room for a full round: lane 0 walks the leftmost pending
+            // subtree to its end (or to its first collision)
+            RefWalk w = {};
+            if (lane == 0)
+              w = walk_subtree<kFamily>(q, vn, c, st.path[size - 1], st.depth[size - 1], st.da[size - 1],
+                                        st.db[size - 1], 0xFFFFFFFFu);
+            const bool hit = __shfl(w.state == kWalkCollision ? 1 : 0, 0) != 0;
+            if (lane == 0) {
+              count += w.nodes;
+              if (pass == 0 && !hit) {
+                if (w.have_margin) {
+                  margin_min = have_margin ? std_min(margin_min, w.min_margin) : w.min_margin;
+                  have_margin = true;
+                }
+                undecided = undecided || w.undecided;
+              }
+            }
+            if (hit && pass == 0) {
+              // the earliest collision so far: everything still pending follows it
+              have_best = true;
+              best_path = __shfl(w.path, 0);
+              best_depth = __shfl(w.depth, 0);
+              best_a = __shfl(w.a, 0);
+              best_b = __shfl(w.b, 0);
+              best_toi = __shfl(w.toi, 0);
+              size = 0;
+            } else {
+              size -= 1;
+            }
+            __syncthreads();
+            continue;
+          }
+          const int take = size < kHeavyBlock ? size : kHeavyBlock;
+          const bool act = lane < take;
+          uint64_t path = 0;
+          uint32_t depth = 0;
+          double da = 0.0, db = 0.0;
+          if (act) {
+            const int at = size - 1 - lane;
+            path = st.path[at];
+            depth = st.depth[at];
+            da = st.da[at];
+            db = st.db[at];
+          }
+          __syncthreads();  // every read of this round before any write
+          int o = kNodeUndecided;
+          double a = 0.0, b = 0.0, dm = 0.0, toi = 0.0, margin = 0.0;
+          if (act) {
+            node_interval(c, path, depth, &a, &b);
+            o = eval_node<kFamily>(q, vn, c, a, b, depth, da, db, &dm, &toi, &margin);
+          }
+          int keep = take;
+          int base = size - take;
+          if (pass == 0) {
+            const uint64_t hits = __ballot(act && o == kNodeCollision);
+            if (hits) {
+              const int j = __builtin_ctzll(hits);
+              keep = j;
+              base = 0;
+              have_best = true;
+              best_path = __shfl(path, j);
+              best_depth = __shfl(depth, j);
+              best_a = __shfl(a, j);
+              best_b = __shfl(b, j);
+              best_toi = __shfl(toi, j);
+            }
+            if (act && lane < keep) {
+              if (o == kNodeSeparated) {
+                margin_min = have_margin ? std_min(margin_min, margin) : margin;
+                have_margin = true;
+              } else if (o == kNodeUndecided) {
+                undecided = true;
+              }
+            }
+          }
+          if (act && lane <= keep) count += 1;  // pass 1's count is used only without a collision
+          const bool kept = act && lane < keep && o == kNodeExpand;
+          const uint64_t right = path | (1ull << depth);
+          bool push_l = kept, push_r = kept;
+          if (pass == 1 && kept) {
+            push_l = !node_after(path, depth + 1, best_path, best_depth);
+            push_r = !node_after(right, depth + 1, best_path, best_depth);
+          }
+          const uint64_t ml = __ballot(push_l), mr = __ballot(push_r);
+          const uint64_t below = lanes_below(lane);
+          const int before = __popcll(ml & below) + __popcll(mr & below);
+          const int total = __popcll(ml) + __popcll(mr);
+          int slot = base + total - 1 - before;  // leftmost on top
+          if (push_l) {
+            st.path[slot] = path;
+            st.depth[slot] = (uint8_t)(depth + 1);
+            st.da[slot] = da;
+            st.db[slot] = dm;
+            --slot;
+          }
+          if (push_r) {
+            st.path[slot] = right;
+            st.depth[slot] = (uint8_t)(depth + 1);
+            st.da[slot] = dm;
+            st.db[slot] = db;
+          }
+          size = base + total;
+          __syncthreads();
+        }
       }
-    }
 
-    // wave reductions
-    for (int m = 32; m >= 1; m >>= 1) count += __shfl_xor(count, m);
-    if (have_best) {
-      if (lane == 0)
-        write_result<kFamily>(c, p.id_a, p.id_b, true, best_a, best_b, best_toi, false, 0.0, count, &out[i]);
-    } else {
-      double mm = have_margin ? margin_min : __builtin_inf();
-      for (int m = 32; m >= 1; m >>= 1) {
-        const double other = __shfl_xor(mm, m);
-        mm = std_min(mm, other);
+      // wave reductions
+      for (int m = 32; m >= 1; m >>= 1) count += __shfl_xor(count, m);
+      if (have_best) {
+        if (lane == 0)
+          write_result<kFamily>(c, p.id_a, p.id_b, true, best_a, best_b, best_toi, false, 0.0, count, &out[i]);
+      } else {
+        double mm = have_margin ? margin_min : __builtin_inf();
+        for (int m = 32; m >= 1; m >>= 1) {
+          const double other = __shfl_xor(mm, m);
+          mm = std_min(mm, other);
+        }
+        const bool und = __ballot(undecided) != 0;
+        const bool any_margin = __ballot(have_margin) != 0;
+        if (lane == 0)
+          write_result<kFamily>(c, p.id_a, p.id_b, false, 0.0, 0.0, 0.0, und, any_margin ? mm : 0.0, count,
+                                &out[i]);
       }
-      const bool und = __ballot(undecided) != 0;
-      const bool any_margin = __ballot(have_margin) != 0;
-      if (lane == 0)
-        write_result<kFamily>(c, p.id_a, p.id_b, false, 0.0, 0.0, 0.0, und, any_margin ? mm : 0.0, count,
-                              &out[i]);
     }
-  }
   }
 }
 
