@@ -515,95 +515,172 @@ __device__ inline uint32_t traverse(const Node* __restrict__ nodes, int32_t star
   return count;
 }
 
-template <bool kFill>
-__global__ __launch_bounds__(kBlock) void vf_query_kernel(const Node* __restrict__ fnodes, int64_t nf,
-                                                          const Box6* __restrict__ vbox,
-                                                          const uint32_t* __restrict__ qorder, int64_t nv, VfRule r,
-                                                          uint32_t* __restrict__ err, uint32_t* __restrict__ counts,
-                                                          int32_t* __restrict__ cache, int64_t slots,
-                                                          int32_t* __restrict__ resume,
-                                                          const uint64_t* __restrict__ offsets,
-                                                          int32_t* __restrict__ out, int64_t cap) {
-  const int64_t b0 = xcd_block(blockIdx.x, gridDim.x);  // Morton-neighbouring queries on one XCD
-  for (int64_t s = b0 * blockDim.x + threadIdx.x; s < nv; s += int64_t(gridDim.x) * blockDim.x) {
-    if (kFill && offsets[s + 1] - offsets[s] <= uint64_t(slots)) continue;  // copied from the cache
-    // a query with more hits resumes after its last cached one
-    const int32_t start = kFill && slots > 0 ? resume[s] : 0;
-    const uint32_t count0 = kFill && slots > 0 ? uint32_t(slots) : 0u;
-    const int32_t v = int32_t(qorder[s]);
-    const int32_t ov = r.vobj[v];
-    uint32_t c = 0;
-    if (ov >= 0 && nf > 0) {
-      const Box6 q = vbox[v];
-      const uint64_t base = kFill ? offsets[s] : 0;
-      c = traverse(
-          fnodes, start, count0, q, 2 * nf, err,
-          [&](int32_t f) {
-            const int32_t of = r.fobj[f];
-            if (!pair_allowed(r.allowed, r.nobj, ov, of)) return false;
-            if (ov == of) {
-              const int32_t* fv = r.faces + int64_t(f) * 3;
-              if (fv[0] == v || fv[1] == v || fv[2] == v) return false;
-            }
-            return true;
-          },
-          [&](int32_t f, uint32_t k, int32_t next) {
-            if (!kFill && int64_t(k) < slots) {
-              cache[s * slots + k] = f;
-              if (int64_t(k) == slots - 1) resume[s] = next;
-            }
-            if (kFill && base + k < uint64_t(cap)) {
-              out[(base + k) * 2] = v;
-              out[(base + k) * 2 + 1] = f;
-            }
-          });
-    }
-    if (!kFill) counts[s] = c;
-  }
-}
+struct VfQuery {
+  const Node* fnodes;
+  int64_t nf;
+  const Box6* vbox;
+  const uint32_t* qorder;
+  int64_t nv;
+  VfRule r;
+  uint32_t* err;
+  uint32_t* counts;
+  int32_t* cache;
+  int64_t slots;
+  int32_t* resume;
+  const uint64_t* offsets;
+  int32_t* out;
+  int64_t cap;
+};
 
+// Vertex query s (in Morton order) against the face tree: count pass (kFill
+// false: count, cache the first `slots` partners, note where to resume) or
+// fill pass for the queries with more partners than the cache holds.
 template <bool kFill>
-__global__ __launch_bounds__(kBlock) void ee_query_kernel(const Node* __restrict__ enodes, int64_t ne,
-                                                          const Box6* __restrict__ ebox, const uint32_t* __restrict__ eorder,
-                                                          EeRule r, uint32_t* __restrict__ err,
-                                                          uint32_t* __restrict__ counts, int32_t* __restrict__ cache,
-                                                          int64_t slots, int32_t* __restrict__ resume,
-                                                          const uint64_t* __restrict__ offsets,
-                                                          int64_t nv, int32_t* __restrict__ out, int64_t cap) {
-  const int64_t b0 = xcd_block(blockIdx.x, gridDim.x);  // Morton-neighbouring queries on one XCD
-  for (int64_t s = b0 * blockDim.x + threadIdx.x; s < ne; s += int64_t(gridDim.x) * blockDim.x) {
-    if (kFill && offsets[nv + s + 1] - offsets[nv + s] <= uint64_t(slots)) continue;  // copied from the cache
-    const int32_t i = int32_t(eorder[s]);
-    const Box6 q = ebox[i];
-    const int32_t oi = r.eobj[i];
-    const int32_t a0 = r.edges[int64_t(i) * 2], a1 = r.edges[int64_t(i) * 2 + 1];
-    const uint64_t base = kFill ? offsets[nv + s] - offsets[nv] : 0;
-    // Each unordered pair once: only the leaves after this edge's own leaf s
-    // (in tree order) are visited -- in the escape-linked pre-order those are
-    // exactly the nodes after leaf s -- and the pair is emitted as (min, max).
-    const bool resumed = kFill && slots > 0;  // more hits than cached: resume after the last cached one
-    const uint32_t c = traverse(
-        enodes, resumed ? resume[s] : enodes[ne - 1 + s].escape, resumed ? uint32_t(slots) : 0u, q, 2 * ne, err,
-        [&](int32_t j) {
-          const int32_t oj = r.eobj[j];
-          if (!pair_allowed(r.allowed, r.nobj, oi, oj)) return false;
-          if (oi == oj) {
-            const int32_t b0 = r.edges[int64_t(j) * 2], b1 = r.edges[int64_t(j) * 2 + 1];
-            if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) return false;
+__device__ inline void vf_query_one(const VfQuery& Q, int64_t s) {
+  const Node* __restrict__ fnodes = Q.fnodes;
+  const int64_t nf = Q.nf, slots = Q.slots, cap = Q.cap;
+  const VfRule& r = Q.r;
+  const uint64_t* __restrict__ offsets = Q.offsets;
+  int32_t* __restrict__ cache = Q.cache;
+  int32_t* __restrict__ resume = Q.resume;
+  int32_t* __restrict__ out = Q.out;
+  uint32_t* __restrict__ counts = Q.counts;
+  uint32_t* err = Q.err;
+  const uint32_t* __restrict__ qorder = Q.qorder;
+  const Box6* __restrict__ vbox = Q.vbox;
+  if (kFill && offsets[s + 1] - offsets[s] <= uint64_t(slots)) return;  // copied from the cache
+  // a query with more hits resumes after its last cached one
+  const int32_t start = kFill && slots > 0 ? resume[s] : 0;
+  const uint32_t count0 = kFill && slots > 0 ? uint32_t(slots) : 0u;
+  const int32_t v = int32_t(qorder[s]);
+  const int32_t ov = r.vobj[v];
+  uint32_t c = 0;
+  if (ov >= 0 && nf > 0) {
+    const Box6 q = vbox[v];
+    const uint64_t base = kFill ? offsets[s] : 0;
+    c = traverse(
+        fnodes, start, count0, q, 2 * nf, err,
+        [&](int32_t f) {
+          const int32_t of = r.fobj[f];
+          if (!pair_allowed(r.allowed, r.nobj, ov, of)) return false;
+          if (ov == of) {
+            const int32_t* fv = r.faces + int64_t(f) * 3;
+            if (fv[0] == v || fv[1] == v || fv[2] == v) return false;
           }
           return true;
         },
-        [&](int32_t j, uint32_t k, int32_t next) {
+        [&](int32_t f, uint32_t k, int32_t next) {
           if (!kFill && int64_t(k) < slots) {
-            cache[s * slots + k] = j;
+            cache[s * slots + k] = f;
             if (int64_t(k) == slots - 1) resume[s] = next;
           }
           if (kFill && base + k < uint64_t(cap)) {
-            out[(base + k) * 2] = i < j ? i : j;
-            out[(base + k) * 2 + 1] = i < j ? j : i;
+            out[(base + k) * 2] = v;
+            out[(base + k) * 2 + 1] = f;
           }
         });
-    if (!kFill) counts[s] = c;
+  }
+  if (!kFill) counts[s] = c;
+}
+
+template <bool kFill>
+__global__ __launch_bounds__(kBlock) void vf_query_kernel(VfQuery Q) {
+  const int64_t b0 = xcd_block(blockIdx.x, gridDim.x);  // Morton-neighbouring queries on one XCD
+  for (int64_t s = b0 * blockDim.x + threadIdx.x; s < Q.nv; s += int64_t(gridDim.x) * blockDim.x)
+    vf_query_one<kFill>(Q, s);
+}
+
+struct EeQuery {
+  const Node* enodes;
+  int64_t ne;
+  const Box6* ebox;
+  const uint32_t* eorder;
+  EeRule r;
+  uint32_t* err;
+  uint32_t* counts;
+  int32_t* cache;
+  int64_t slots;
+  int32_t* resume;
+  const uint64_t* offsets;
+  int64_t nv;
+  int32_t* out;
+  int64_t cap;
+};
+
+// Edge query s (in tree order) against the edge tree, count or fill pass as
+// for vf_query_one.
+template <bool kFill>
+__device__ inline void ee_query_one(const EeQuery& Q, int64_t s) {
+  const Node* __restrict__ enodes = Q.enodes;
+  const int64_t ne = Q.ne, slots = Q.slots, nv = Q.nv, cap = Q.cap;
+  const EeRule& r = Q.r;
+  const uint64_t* __restrict__ offsets = Q.offsets;
+  int32_t* __restrict__ cache = Q.cache;
+  int32_t* __restrict__ resume = Q.resume;
+  int32_t* __restrict__ out = Q.out;
+  uint32_t* __restrict__ counts = Q.counts;
+  uint32_t* err = Q.err;
+  const uint32_t* __restrict__ eorder = Q.eorder;
+  const Box6* __restrict__ ebox = Q.ebox;
+  if (kFill && offsets[nv + s + 1] - offsets[nv + s] <= uint64_t(slots)) return;  // copied from the cache
+  const int32_t i = int32_t(eorder[s]);
+  const Box6 q = ebox[i];
+  const int32_t oi = r.eobj[i];
+  const int32_t a0 = r.edges[int64_t(i) * 2], a1 = r.edges[int64_t(i) * 2 + 1];
+  const uint64_t base = kFill ? offsets[nv + s] - offsets[nv] : 0;
+  // Each unordered pair once: only the leaves after this edge's own leaf s
+  // (in tree order) are visited -- in the escape-linked pre-order those are
+  // exactly the nodes after leaf s -- and the pair is emitted as (min, max).
+  const bool resumed = kFill && slots > 0;  // more hits than cached: resume after the last cached one
+  const uint32_t c = traverse(
+      enodes, resumed ? resume[s] : enodes[ne - 1 + s].escape, resumed ? uint32_t(slots) : 0u, q, 2 * ne, err,
+      [&](int32_t j) {
+        const int32_t oj = r.eobj[j];
+        if (!pair_allowed(r.allowed, r.nobj, oi, oj)) return false;
+        if (oi == oj) {
+          const int32_t b0 = r.edges[int64_t(j) * 2], b1 = r.edges[int64_t(j) * 2 + 1];
+          if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) return false;
+        }
+        return true;
+      },
+      [&](int32_t j, uint32_t k, int32_t next) {
+        if (!kFill && int64_t(k) < slots) {
+          cache[s * slots + k] = j;
+          if (int64_t(k) == slots - 1) resume[s] = next;
+        }
+        if (kFill && base + k < uint64_t(cap)) {
+          out[(base + k) * 2] = i < j ? i : j;
+          out[(base + k) * 2 + 1] = i < j ? j : i;
+        }
+      });
+  if (!kFill) counts[s] = c;
+}
+
+template <bool kFill>
+__global__ __launch_bounds__(kBlock) void ee_query_kernel(EeQuery Q) {
+  const int64_t b0 = xcd_block(blockIdx.x, gridDim.x);  // Morton-neighbouring queries on one XCD
+  for (int64_t s = b0 * blockDim.x + threadIdx.x; s < Q.ne; s += int64_t(gridDim.x) * blockDim.x)
+    ee_query_one<kFill>(Q, s);
+}
+
+// The count passes of both trees in one launch (vb vertex-query blocks, eb
+// edge-query blocks): each XCD (hardware block i goes to XCD i % 8) gets one
+// contiguous range of vertex blocks and then one contiguous range of edge
+// blocks, so the smaller vertex pass no longer runs alone on a half-empty
+// chip and each XCD's L2 still sees one region of each tree.
+__global__ __launch_bounds__(kBlock) void count_both_kernel(VfQuery V, EeQuery E, unsigned vb) {
+  const unsigned G = gridDim.x, x = blockIdx.x % kXcds, j = blockIdx.x / kXcds;
+  const unsigned per = G / kXcds, rem = G % kXcds, vper = vb / kXcds, vrem = vb % kXcds;
+  const unsigned vb_x = vper + (x < vrem ? 1u : 0u);
+  if (j < vb_x) {
+    const int64_t blk = int64_t(x) * vper + (x < vrem ? x : vrem) + j;
+    const int64_t s = blk * kBlock + threadIdx.x;
+    if (s < V.nv) vf_query_one<false>(V, s);
+  } else {
+    const int64_t estart = (int64_t(x) * per + (x < rem ? x : rem)) - (int64_t(x) * vper + (x < vrem ? x : vrem));
+    const int64_t blk = estart + (j - vb_x);
+    const int64_t s = blk * kBlock + threadIdx.x;
+    if (s < E.ne) ee_query_one<false>(E, s);
   }
 }
 
@@ -760,16 +837,19 @@ int p2c_bp_count(const p2c_bp_mesh* m, void* ws, uint64_t* totals, void* stream)
   const int64_t slots = m->hit_cache != nullptr ? m->hit_cache_slots : 0;
   VfRule vr{m->faces, m->face_object, m->vertex_object, at<uint8_t>(ws, L.allowed), m->n_objects};
   EeRule er{m->edges, m->edge_object, at<uint8_t>(ws, L.allowed), m->n_objects};
-  if (nv > 0)
-    hipLaunchKernelGGL(vf_query_kernel<false>, dim3(grid_for(nv, kBlock)), dim3(kBlock), 0, s,
-                       at<Node>(ws, L.fnodes), nf, at<Box6>(ws, L.vbox), at<uint32_t>(ws, L.qorder), nv, vr,
-                       at<uint32_t>(ws, L.err), counts, m->hit_cache, slots, at<int32_t>(ws, L.resume), nullptr,
-                       nullptr, 0);
-  if (ne > 0)
-    hipLaunchKernelGGL(ee_query_kernel<false>, dim3(grid_for(ne, kBlock)), dim3(kBlock), 0, s,
-                       at<Node>(ws, L.enodes), ne, at<Box6>(ws, L.ebox), at<uint32_t>(ws, L.eorder), er,
-                       at<uint32_t>(ws, L.err), counts + nv, m->hit_cache + nv * slots, slots,
-                       at<int32_t>(ws, L.resume) + nv, nullptr, nv, nullptr, 0);
+  const VfQuery V{at<Node>(ws, L.fnodes), nf, at<Box6>(ws, L.vbox), at<uint32_t>(ws, L.qorder), nv, vr,
+                  at<uint32_t>(ws, L.err), counts, m->hit_cache, slots, at<int32_t>(ws, L.resume), nullptr,
+                  nullptr, 0};
+  const EeQuery E{at<Node>(ws, L.enodes), ne, at<Box6>(ws, L.ebox), at<uint32_t>(ws, L.eorder), er,
+                  at<uint32_t>(ws, L.err), counts + nv, m->hit_cache + nv * slots, slots,
+                  at<int32_t>(ws, L.resume) + nv, nullptr, nv, nullptr, 0};
+  const int64_t vb = (nv + kBlock - 1) / kBlock, eb = (ne + kBlock - 1) / kBlock;
+  if (vb > 0 && eb > 0 && vb + eb <= (int64_t(1) << 20)) {
+    hipLaunchKernelGGL(count_both_kernel, dim3(unsigned(vb + eb)), dim3(kBlock), 0, s, V, E, unsigned(vb));
+  } else {
+    if (nv > 0) hipLaunchKernelGGL(vf_query_kernel<false>, dim3(grid_for(nv, kBlock)), dim3(kBlock), 0, s, V);
+    if (ne > 0) hipLaunchKernelGGL(ee_query_kernel<false>, dim3(grid_for(ne, kBlock)), dim3(kBlock), 0, s, E);
+  }
   if (int rc = exclusive_scan_u32(counts, nv + ne, offsets, at<uint64_t>(ws, L.scan_part), s)) return rc;
   hipLaunchKernelGGL(totals_kernel, dim3(1), dim3(64), 0, s, offsets, nv, ne, at<uint32_t>(ws, L.err), totals);
   P2C_HIP_CHECK(hipGetLastError());
@@ -789,14 +869,14 @@ int p2c_bp_emit(const p2c_bp_mesh* m, void* ws, int32_t* vf_pairs, int64_t vf_ca
   EeRule er{m->edges, m->edge_object, at<uint8_t>(ws, L.allowed), m->n_objects};
   if (nv > 0 && vf_pairs != nullptr)
     hipLaunchKernelGGL(vf_query_kernel<true>, dim3(grid_for(nv, kBlock)), dim3(kBlock), 0, s,
-                       at<Node>(ws, L.fnodes), nf, at<Box6>(ws, L.vbox), at<uint32_t>(ws, L.qorder), nv, vr,
-                       at<uint32_t>(ws, L.err), nullptr, nullptr, slots, at<int32_t>(ws, L.resume), offsets,
-                       vf_pairs, vf_cap);
+                       VfQuery{at<Node>(ws, L.fnodes), nf, at<Box6>(ws, L.vbox), at<uint32_t>(ws, L.qorder), nv, vr,
+                               at<uint32_t>(ws, L.err), nullptr, nullptr, slots, at<int32_t>(ws, L.resume), offsets,
+                               vf_pairs, vf_cap});
   if (ne > 0 && ee_pairs != nullptr)
     hipLaunchKernelGGL(ee_query_kernel<true>, dim3(grid_for(ne, kBlock)), dim3(kBlock), 0, s,
-                       at<Node>(ws, L.enodes), ne, at<Box6>(ws, L.ebox), at<uint32_t>(ws, L.eorder), er,
-                       at<uint32_t>(ws, L.err), nullptr, nullptr, slots, at<int32_t>(ws, L.resume) + nv, offsets,
-                       nv, ee_pairs, ee_cap);
+                       EeQuery{at<Node>(ws, L.enodes), ne, at<Box6>(ws, L.ebox), at<uint32_t>(ws, L.eorder), er,
+                               at<uint32_t>(ws, L.err), nullptr, nullptr, slots, at<int32_t>(ws, L.resume) + nv,
+                               offsets, nv, ee_pairs, ee_cap});
   if (slots > 0 && nv + ne > 0 && (vf_pairs != nullptr || ee_pairs != nullptr))
     hipLaunchKernelGGL(cache_copy_kernel, dim3(grid_for((nv + ne) * slots, kBlock, 1 << 16)), dim3(kBlock), 0, s,
                        m->hit_cache, offsets, at<uint32_t>(ws, L.qorder), at<uint32_t>(ws, L.eorder), nv, ne, slots,
