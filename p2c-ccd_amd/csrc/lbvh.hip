@@ -87,7 +87,8 @@ __device__ inline bool overlap(const Box6& a, const double* mn, const double* mx
 // ---- workspace ----------------------------------------------------------
 struct Layout {
   size_t roots, vbox, fbox, ebox, objkey, allowed, keys0, vals0, keys1, vals1, hist, fnodes, fparent, fflag, frcs,
-      fhi, flo, enodes, eparent, eflag, ercs, ehi, elo, qorder, eorder, counts, resume, offsets, hist_off, scan_part, err, total;
+      fhi, flo, enodes, eparent, eflag, ercs, ehi, elo, qorder, eorder, counts, resume, offsets, hist_off, scan_part, err,
+      eroots, total;
   int64_t sort_cap, hist_blocks;
 };
 
@@ -135,7 +136,8 @@ Layout make_layout(int64_t nv, int64_t nf, int64_t ne, int32_t nobj) {
   const int64_t scan_items = std::max<int64_t>(nv + ne + 1, 256 * L.hist_blocks);
   L.scan_part = take(8 * (scan_items / 1024 + 2));
   L.err = take(16);
-  L.roots = take(4 * (size_t(sn) + 2));  // [0] count, then the nodes whose parent spans blocks
+  L.roots = take(4 * (size_t(sn) + 2));  // [0] count, then the nodes whose parent spans blocks (face tree)
+  L.eroots = take(4 * (size_t(sn) + 2));  // the same for the edge tree
   L.total = o;
   return L;
 }
@@ -426,17 +428,26 @@ __global__ __launch_bounds__(kTopBlock) void refit_top_kernel(Node* __restrict__
 // the other lanes of this workgroup, as in refit_top_kernel.
 constexpr int kTopLdsWords = 16384;
 
-__global__ __launch_bounds__(kTopBlock) void refit_top_lds_kernel(Node* __restrict__ nodes,
-                                                                  const int32_t* __restrict__ parent,
-                                                                  const uint32_t* __restrict__ roots,
-                                                                  int64_t n_internal) {
+struct TopTree {
+  Node* nodes;
+  const int32_t* parent;
+  const uint32_t* roots;
+  int64_t n_internal;
+};
+
+// Block b climbs tree b (faces 0, edges 1): the two trees' top refits in one
+// launch, on two CUs.
+__global__ __launch_bounds__(kTopBlock) void refit_top_lds_kernel(TopTree t0, TopTree t1) {
   __shared__ uint32_t bits[kTopLdsWords];
-  const int words = int((n_internal + 31) / 32);
+  const TopTree& t = blockIdx.x == 0 ? t0 : t1;
+  Node* __restrict__ nodes = t.nodes;
+  const int32_t* __restrict__ parent = t.parent;
+  const int words = int((t.n_internal + 31) / 32);
   for (int i = threadIdx.x; i < words; i += blockDim.x) bits[i] = 0;
   __syncthreads();
-  const uint32_t total = roots[0];
+  const uint32_t total = t.roots[0];
   for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) {
-    int32_t node = parent[roots[2 + i]];
+    int32_t node = parent[t.roots[2 + i]];
     while (node >= 0) {
       const uint32_t bit = 1u << (node & 31);
       const uint32_t prev =
@@ -783,9 +794,11 @@ int p2c_bp_build(const p2c_bp_mesh* m, const double* x0, const double* x1, doubl
     int kind;
     const int32_t* idx;
     int64_t n;
-    size_t box, nodes, parent, flag, rcs, hi, lo;
-  } trees[2] = {{0, m->faces, nf, L.fbox, L.fnodes, L.fparent, L.fflag, L.frcs, L.fhi, L.flo},
-                {1, m->edges, ne, L.ebox, L.enodes, L.eparent, L.eflag, L.ercs, L.ehi, L.elo}};
+    size_t box, nodes, parent, flag, rcs, hi, lo, roots;
+  } trees[2] = {{0, m->faces, nf, L.fbox, L.fnodes, L.fparent, L.fflag, L.frcs, L.fhi, L.flo, L.roots},
+                {1, m->edges, ne, L.ebox, L.enodes, L.eparent, L.eflag, L.ercs, L.ehi, L.elo, L.eroots}};
+  TopTree lds_top[2];
+  int n_lds_top = 0;
   for (const Tree& t : trees) {
     if (t.n == 0) continue;
     hipLaunchKernelGGL(prim_box_kernel, dim3(grid_for(t.n, kBlock)), dim3(kBlock), 0, s, t.kind, x0, x1, t.idx, t.n,
@@ -802,12 +815,12 @@ int p2c_bp_build(const p2c_bp_mesh* m, const double* x0, const double* x1, doubl
     if (t.n > 1) {
       // one block per 256 leaves, exactly (the LDS flags cover the block's leaves)
       if ((t.n + kBlock - 1) / kBlock > (int64_t(1) << 20)) return set_error(P2C_ECAPACITY, "too many primitives");
-      uint32_t* roots = at<uint32_t>(ws, L.roots);
+      uint32_t* roots = at<uint32_t>(ws, t.roots);
       hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, roots, 1);
       hipLaunchKernelGGL(refit_kernel, dim3(unsigned((t.n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, t.n, nodes,
                          parent, at<int32_t>(ws, t.lo), at<int32_t>(ws, t.hi), roots);
       if (t.n - 1 <= int64_t(kTopLdsWords) * 32)
-        hipLaunchKernelGGL(refit_top_lds_kernel, dim3(1), dim3(kTopBlock), 0, s, nodes, parent, roots, t.n - 1);
+        lds_top[n_lds_top++] = TopTree{nodes, parent, roots, t.n - 1};  // climbed below, both trees at once
       else
         hipLaunchKernelGGL(refit_top_kernel, dim3(1), dim3(kTopBlock), 0, s, nodes, parent, roots, flag);
     }
@@ -815,6 +828,9 @@ int p2c_bp_build(const p2c_bp_mesh* m, const double* x0, const double* x1, doubl
     if (t.kind == 1)
       P2C_HIP_CHECK(hipMemcpyAsync(at<uint32_t>(ws, L.eorder), v0, sizeof(uint32_t) * t.n, hipMemcpyDeviceToDevice, s));
   }
+  if (n_lds_top > 0)
+    hipLaunchKernelGGL(refit_top_lds_kernel, dim3(unsigned(n_lds_top)), dim3(kTopBlock), 0, s, lds_top[0],
+                       n_lds_top > 1 ? lds_top[1] : lds_top[0]);
   // vertex query order: vertices sorted by Morton code of their swept box
   if (nv > 0) {
     hipLaunchKernelGGL(prim_box_kernel, dim3(grid_for(nv, kBlock)), dim3(kBlock), 0, s, 2, x0, x1, nullptr, nv,
