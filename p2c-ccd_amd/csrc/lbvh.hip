@@ -103,17 +103,19 @@ Layout make_layout(int64_t nv, int64_t nf, int64_t ne, int32_t nobj) {
     return at;
   };
   const int64_t sn = std::max<int64_t>(std::max(nv, nf), std::max<int64_t>(ne, 1));
-  L.sort_cap = sn;
-  L.hist_blocks = (sn + kSortTile - 1) / kSortTile;
+  // one sort over the face, edge and vertex keys together
+  const int64_t sall = std::max<int64_t>(nf + ne + nv, 1);
+  L.sort_cap = sall;
+  L.hist_blocks = (sall + kSortTile - 1) / kSortTile;
   L.vbox = take(sizeof(Box6) * nv);
   L.fbox = take(sizeof(Box6) * nf);
   L.ebox = take(sizeof(Box6) * ne);
   L.objkey = take(sizeof(uint64_t) * 6 * nobj);
   L.allowed = take(size_t(nobj) * nobj);
-  L.keys0 = take(4 * sn);
-  L.vals0 = take(4 * sn);
-  L.keys1 = take(4 * sn);
-  L.vals1 = take(4 * sn);
+  L.keys0 = take(4 * sall);
+  L.vals0 = take(4 * sall);
+  L.keys1 = take(4 * sall);
+  L.vals1 = take(4 * sall);
   L.hist = take(4 * 256 * L.hist_blocks);
   L.fnodes = take(sizeof(Node) * std::max<int64_t>(2 * nf - 1, 1));
   L.fparent = take(4 * std::max<int64_t>(2 * nf - 1, 1));
@@ -248,14 +250,17 @@ __device__ inline void scene_bounds(const uint64_t* objkey, int32_t nobj, double
   }
 }
 
-// face (kind 0) or edge (kind 1) boxes + Morton keys; vertex keys (kind 2)
+// face (kind 0) or edge (kind 1) boxes + Morton keys; vertex keys (kind 2).
+// All three key sets are sorted together, tagged with their kind in bits
+// 30-31: a stable sort keeps each kind contiguous, in the order (and with
+// the tie order) a sort of that kind alone would give.
 __global__ __launch_bounds__(kBlock) void prim_box_kernel(int kind, const double* __restrict__ x0,
                                                           const double* __restrict__ x1,
                                                           const int32_t* __restrict__ idx, int64_t n, double th,
                                                           const Box6* __restrict__ vbox,
                                                           const uint64_t* __restrict__ objkey, int32_t nobj,
                                                           Box6* __restrict__ box, uint32_t* __restrict__ keys,
-                                                          uint32_t* __restrict__ vals) {
+                                                          uint32_t* __restrict__ vals, uint32_t tag) {
   double lo[3], inv[3];
   scene_bounds(objkey, nobj, lo, inv);
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
@@ -273,7 +278,7 @@ __global__ __launch_bounds__(kBlock) void prim_box_kernel(int kind, const double
       box_expand(b, th);
       box[i] = b;
     }
-    keys[i] = morton_of(b, lo, inv);
+    keys[i] = morton_of(b, lo, inv) | tag;  // 30-bit code; the tag (bits 30-31) keeps the three kinds apart
     vals[i] = (uint32_t)i;
   }
 }
@@ -799,18 +804,29 @@ int p2c_bp_build(const p2c_bp_mesh* m, const double* x0, const double* x1, doubl
                 {1, m->edges, ne, L.ebox, L.enodes, L.eparent, L.eflag, L.ercs, L.ehi, L.elo, L.eroots}};
   TopTree lds_top[2];
   int n_lds_top = 0;
+  // boxes and tagged keys of every kind, then one sort: faces at 0, edges at nf, vertices at nf + ne
+  const int64_t off_of[3] = {0, nf, nf + ne};
+  for (const Tree& t : trees)
+    if (t.n > 0)
+      hipLaunchKernelGGL(prim_box_kernel, dim3(grid_for(t.n, kBlock)), dim3(kBlock), 0, s, t.kind, x0, x1, t.idx, t.n,
+                         thickness, vbox, objkey, nobj, at<Box6>(ws, t.box), k0 + off_of[t.kind],
+                         v0 + off_of[t.kind], uint32_t(t.kind) << 30);
+  if (nv > 0)
+    hipLaunchKernelGGL(prim_box_kernel, dim3(grid_for(nv, kBlock)), dim3(kBlock), 0, s, 2, x0, x1, nullptr, nv,
+                       thickness, vbox, objkey, nobj, nullptr, k0 + off_of[2], v0 + off_of[2], 2u << 30);
+  if (nf + ne + nv > 0)
+    if (int rc = radix_sort(k0, v0, k1, v1, nf + ne + nv, hist, hist_off, part, s)) return rc;
   for (const Tree& t : trees) {
     if (t.n == 0) continue;
-    hipLaunchKernelGGL(prim_box_kernel, dim3(grid_for(t.n, kBlock)), dim3(kBlock), 0, s, t.kind, x0, x1, t.idx, t.n,
-                       thickness, vbox, objkey, nobj, at<Box6>(ws, t.box), k0, v0);
-    if (int rc = radix_sort(k0, v0, k1, v1, t.n, hist, hist_off, part, s)) return rc;
+    uint32_t* tk = k0 + off_of[t.kind];
+    uint32_t* tv = v0 + off_of[t.kind];
     Node* nodes = at<Node>(ws, t.nodes);
     int32_t* parent = at<int32_t>(ws, t.parent);
     uint32_t* flag = at<uint32_t>(ws, t.flag);
     if (t.n > 1)
-      hipLaunchKernelGGL(karras_kernel, dim3(grid_for(t.n - 1, kBlock)), dim3(kBlock), 0, s, k0, t.n, nodes, parent,
+      hipLaunchKernelGGL(karras_kernel, dim3(grid_for(t.n - 1, kBlock)), dim3(kBlock), 0, s, tk, t.n, nodes, parent,
                          at<int32_t>(ws, t.rcs), at<int32_t>(ws, t.hi), at<int32_t>(ws, t.lo));
-    hipLaunchKernelGGL(leaves_kernel, dim3(grid_for(2 * t.n - 1, kBlock)), dim3(kBlock), 0, s, v0, t.n,
+    hipLaunchKernelGGL(leaves_kernel, dim3(grid_for(2 * t.n - 1, kBlock)), dim3(kBlock), 0, s, tv, t.n,
                        at<Box6>(ws, t.box), nodes, at<int32_t>(ws, t.rcs), at<int32_t>(ws, t.hi), parent, flag);
     if (t.n > 1) {
       // one block per 256 leaves, exactly (the LDS flags cover the block's leaves)
@@ -826,18 +842,15 @@ int p2c_bp_build(const p2c_bp_mesh* m, const double* x0, const double* x1, doubl
     }
     // the edge leaf order (sorted edge ids) is also the EE query order
     if (t.kind == 1)
-      P2C_HIP_CHECK(hipMemcpyAsync(at<uint32_t>(ws, L.eorder), v0, sizeof(uint32_t) * t.n, hipMemcpyDeviceToDevice, s));
+      P2C_HIP_CHECK(hipMemcpyAsync(at<uint32_t>(ws, L.eorder), tv, sizeof(uint32_t) * t.n, hipMemcpyDeviceToDevice, s));
   }
   if (n_lds_top > 0)
     hipLaunchKernelGGL(refit_top_lds_kernel, dim3(unsigned(n_lds_top)), dim3(kTopBlock), 0, s, lds_top[0],
                        n_lds_top > 1 ? lds_top[1] : lds_top[0]);
   // vertex query order: vertices sorted by Morton code of their swept box
-  if (nv > 0) {
-    hipLaunchKernelGGL(prim_box_kernel, dim3(grid_for(nv, kBlock)), dim3(kBlock), 0, s, 2, x0, x1, nullptr, nv,
-                       thickness, vbox, objkey, nobj, nullptr, k0, v0);
-    if (int rc = radix_sort(k0, v0, k1, v1, nv, hist, hist_off, part, s)) return rc;
-    P2C_HIP_CHECK(hipMemcpyAsync(at<uint32_t>(ws, L.qorder), v0, sizeof(uint32_t) * nv, hipMemcpyDeviceToDevice, s));
-  }
+  if (nv > 0)
+    P2C_HIP_CHECK(hipMemcpyAsync(at<uint32_t>(ws, L.qorder), v0 + off_of[2], sizeof(uint32_t) * nv,
+                                 hipMemcpyDeviceToDevice, s));
   P2C_HIP_CHECK(hipGetLastError());
   return P2C_OK;
 }
