@@ -150,9 +150,15 @@ T* at(void* ws, size_t off) {
 }
 
 // ---- 1. boxes and object envelopes -------------------------------------
-__global__ void objkey_init_kernel(uint64_t* objkey, int32_t nobj) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// Resets the object envelopes, the error counters (4 words) and the counts
+// of both trees' top-refit root lists; one workgroup of kMaxObjects * 6 lanes.
+__global__ void objkey_init_kernel(uint64_t* objkey, int32_t nobj, uint32_t* err, uint32_t* froots,
+                                   uint32_t* eroots) {
+  const int i = threadIdx.x;
   if (i < nobj * 6) objkey[i] = (i % 6) < 3 ? ~0ull : 0ull;
+  if (i < 4) err[i] = 0u;
+  if (i == 4) froots[0] = 0u;
+  if (i == 5) eroots[0] = 0u;
 }
 
 __global__ __launch_bounds__(kBlock) void vertex_box_kernel(const double* __restrict__ x0,
@@ -250,36 +256,52 @@ __device__ inline void scene_bounds(const uint64_t* objkey, int32_t nobj, double
   }
 }
 
-// face (kind 0) or edge (kind 1) boxes + Morton keys; vertex keys (kind 2).
-// All three key sets are sorted together, tagged with their kind in bits
-// 30-31: a stable sort keeps each kind contiguous, in the order (and with
-// the tie order) a sort of that kind alone would give.
-__global__ __launch_bounds__(kBlock) void prim_box_kernel(int kind, const double* __restrict__ x0,
-                                                          const double* __restrict__ x1,
-                                                          const int32_t* __restrict__ idx, int64_t n, double th,
-                                                          const Box6* __restrict__ vbox,
-                                                          const uint64_t* __restrict__ objkey, int32_t nobj,
-                                                          Box6* __restrict__ box, uint32_t* __restrict__ keys,
-                                                          uint32_t* __restrict__ vals, uint32_t tag) {
+// Face boxes and keys (items [0, nf)), edge boxes and keys ([nf, nf + ne)),
+// vertex keys ([nf + ne, nf + ne + nv)) in one launch; values are the index
+// within the kind.  All three key sets are sorted together, tagged with
+// their kind in bits 30-31 above the 30-bit Morton code: a stable sort keeps
+// each kind contiguous, in the order (and with the tie order) a sort of that
+// kind alone would give.
+struct PrimKeys {
+  const double* x0;
+  const double* x1;
+  const int32_t* faces;
+  const int32_t* edges;
+  const Box6* vbox;
+  const uint64_t* objkey;
+  Box6* fbox;
+  Box6* ebox;
+  uint32_t* keys;
+  uint32_t* vals;
+  int64_t nf, ne, nv;
+  double th;
+  int32_t nobj;
+};
+
+__global__ __launch_bounds__(kBlock) void prim_box_kernel(PrimKeys P) {
   double lo[3], inv[3];
-  scene_bounds(objkey, nobj, lo, inv);
-  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+  scene_bounds(P.objkey, P.nobj, lo, inv);
+  const int64_t total = P.nf + P.ne + P.nv;
+  for (int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; g < total; g += int64_t(gridDim.x) * blockDim.x) {
+    const uint32_t kind = g < P.nf ? 0u : (g < P.nf + P.ne ? 1u : 2u);
+    const int64_t i = kind == 0 ? g : (kind == 1 ? g - P.nf : g - P.nf - P.ne);
     Box6 b;
     if (kind == 2) {
-      b = vbox[i];
+      b = P.vbox[i];
     } else {
       box_reset(b);
       const int m = kind == 0 ? 3 : 2;
+      const int32_t* idx = kind == 0 ? P.faces : P.edges;
       for (int c = 0; c < m; ++c) {
         const int64_t v = idx[i * m + c];
-        box_extend(b, x0 + v * 3);
-        box_extend(b, x1 + v * 3);
+        box_extend(b, P.x0 + v * 3);
+        box_extend(b, P.x1 + v * 3);
       }
-      box_expand(b, th);
-      box[i] = b;
+      box_expand(b, P.th);
+      (kind == 0 ? P.fbox : P.ebox)[i] = b;
     }
-    keys[i] = morton_of(b, lo, inv) | tag;  // 30-bit code; the tag (bits 30-31) keeps the three kinds apart
-    vals[i] = (uint32_t)i;
+    P.keys[g] = morton_of(b, lo, inv) | (kind << 30);
+    P.vals[g] = (uint32_t)i;
   }
 }
 
@@ -781,8 +803,8 @@ int p2c_bp_build(const p2c_bp_mesh* m, const double* x0, const double* x1, doubl
   const int32_t nobj = m->n_objects;
   uint64_t* objkey = at<uint64_t>(ws, L.objkey);
   Box6* vbox = at<Box6>(ws, L.vbox);
-  hipLaunchKernelGGL(objkey_init_kernel, dim3(1), dim3(kMaxObjects * 6), 0, s, objkey, nobj);
-  hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, at<uint32_t>(ws, L.err), 4);
+  hipLaunchKernelGGL(objkey_init_kernel, dim3(1), dim3(kMaxObjects * 6), 0, s, objkey, nobj, at<uint32_t>(ws, L.err),
+                     at<uint32_t>(ws, L.roots), at<uint32_t>(ws, L.eroots));
   if (nv > 0)
     hipLaunchKernelGGL(vertex_box_kernel, dim3(grid_for(nv, kBlock, 2048)), dim3(kBlock), 0, s, x0, x1, nv, thickness,
                        m->vertex_object, nobj, vbox, objkey);
@@ -806,16 +828,12 @@ int p2c_bp_build(const p2c_bp_mesh* m, const double* x0, const double* x1, doubl
   int n_lds_top = 0;
   // boxes and tagged keys of every kind, then one sort: faces at 0, edges at nf, vertices at nf + ne
   const int64_t off_of[3] = {0, nf, nf + ne};
-  for (const Tree& t : trees)
-    if (t.n > 0)
-      hipLaunchKernelGGL(prim_box_kernel, dim3(grid_for(t.n, kBlock)), dim3(kBlock), 0, s, t.kind, x0, x1, t.idx, t.n,
-                         thickness, vbox, objkey, nobj, at<Box6>(ws, t.box), k0 + off_of[t.kind],
-                         v0 + off_of[t.kind], uint32_t(t.kind) << 30);
-  if (nv > 0)
-    hipLaunchKernelGGL(prim_box_kernel, dim3(grid_for(nv, kBlock)), dim3(kBlock), 0, s, 2, x0, x1, nullptr, nv,
-                       thickness, vbox, objkey, nobj, nullptr, k0 + off_of[2], v0 + off_of[2], 2u << 30);
-  if (nf + ne + nv > 0)
+  if (nf + ne + nv > 0) {
+    const PrimKeys P{x0, x1, m->faces, m->edges, vbox, objkey, at<Box6>(ws, L.fbox), at<Box6>(ws, L.ebox), k0, v0,
+                     nf, ne, nv, thickness, nobj};
+    hipLaunchKernelGGL(prim_box_kernel, dim3(grid_for(nf + ne + nv, kBlock)), dim3(kBlock), 0, s, P);
     if (int rc = radix_sort(k0, v0, k1, v1, nf + ne + nv, hist, hist_off, part, s)) return rc;
+  }
   for (const Tree& t : trees) {
     if (t.n == 0) continue;
     uint32_t* tk = k0 + off_of[t.kind];
@@ -831,8 +849,7 @@ int p2c_bp_build(const p2c_bp_mesh* m, const double* x0, const double* x1, doubl
     if (t.n > 1) {
       // one block per 256 leaves, exactly (the LDS flags cover the block's leaves)
       if ((t.n + kBlock - 1) / kBlock > (int64_t(1) << 20)) return set_error(P2C_ECAPACITY, "too many primitives");
-      uint32_t* roots = at<uint32_t>(ws, t.roots);
-      hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, roots, 1);
+      uint32_t* roots = at<uint32_t>(ws, t.roots);  // count reset by objkey_init_kernel
       hipLaunchKernelGGL(refit_kernel, dim3(unsigned((t.n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, t.n, nodes,
                          parent, at<int32_t>(ws, t.lo), at<int32_t>(ws, t.hi), roots);
       if (t.n - 1 <= int64_t(kTopLdsWords) * 32)

@@ -5,7 +5,8 @@
 //     (digit-major), then a stable scatter ranked with wave ballots and
 //     per-wave LDS digit counters.
 //   scan: u32 -> u64 exclusive scan in three phases (per-1024 block scans,
-//     a scan of the block totals, the add-back).
+//     a scan of the block totals, the add-back), or in one workgroup for
+//     short inputs.
 #pragma once
 
 #include <utility>
@@ -176,6 +177,78 @@ __global__ __launch_bounds__(kScanBlock) void scan_add_kernel(uint64_t* __restri
     if (base + i < n) out[base + i] += add;
 }
 
+// One-workgroup scan for short inputs (up to kScanSmallMax items), one
+// launch in place of three: for the digit histograms of a sort pass and the
+// query counts of small scenes the launches, not the bytes, are the cost.
+// A chunk is 16 rows of 1024 items; lane t of the workgroup holds item t of
+// every row (coalesced loads and stores), scans the 16 rows within its wave
+// step by step (each step's 16 cross-lane reads issued together), and one
+// wave scans the 16 x 16 (row, wave) totals in item order.  Acc is the
+// running-sum type: u32 where every prefix is known to fit (the digit
+// histograms of fewer than 2^32 keys), u64 otherwise.
+constexpr int kScanSmallBlock = 1024;
+constexpr int kScanSmallRows = 16;
+constexpr int64_t kScanSmallChunk = int64_t(kScanSmallBlock) * kScanSmallRows;
+constexpr int64_t kScanSmallMax = 4 * kScanSmallChunk;
+
+template <typename Acc>
+__global__ __launch_bounds__(kScanSmallBlock) void scan_small_kernel(const uint32_t* __restrict__ in, int64_t n,
+                                                                     uint64_t* __restrict__ out,
+                                                                     uint64_t* __restrict__ grand) {
+  constexpr int kWaves = kScanSmallBlock / 64;
+  constexpr int kParts = kScanSmallRows * kWaves;  // (row, wave) totals, row-major = item order
+  static_assert(kParts == 4 * 64, "one wave scans the totals, four per lane");
+  __shared__ uint64_t part[kParts + 1];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint64_t carry = 0;
+  for (int64_t c = 0; c < n; c += kScanSmallChunk) {
+    uint32_t v[kScanSmallRows];
+    Acc inc[kScanSmallRows];
+#pragma unroll
+    for (int r = 0; r < kScanSmallRows; ++r) {
+      const int64_t e = c + int64_t(r) * kScanSmallBlock + threadIdx.x;
+      v[r] = e < n ? in[e] : 0u;
+      inc[r] = v[r];
+    }
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      Acc y[kScanSmallRows];
+#pragma unroll
+      for (int r = 0; r < kScanSmallRows; ++r) y[r] = __shfl_up(inc[r], o);
+#pragma unroll
+      for (int r = 0; r < kScanSmallRows; ++r)
+        if (lane >= o) inc[r] += y[r];
+    }
+    if (lane == 63) {
+#pragma unroll
+      for (int r = 0; r < kScanSmallRows; ++r) part[r * kWaves + w] = inc[r];
+    }
+    __syncthreads();
+    if (w == 0) {
+      uint64_t t[4], sum = 0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        t[k] = sum;
+        sum += part[lane * 4 + k];
+      }
+      const uint64_t x = wave_inclusive(sum);
+      const uint64_t base = carry + x - sum;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) part[lane * 4 + k] = base + t[k];
+      if (lane == 63) part[kParts] = carry + x;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kScanSmallRows; ++r) {
+      const int64_t e = c + int64_t(r) * kScanSmallBlock + threadIdx.x;
+      if (e < n) out[e] = part[r * kWaves + w] + uint64_t(inc[r] - v[r]);
+    }
+    carry = part[kParts];
+    __syncthreads();  // part is rewritten by the next chunk
+  }
+  if (grand != nullptr && threadIdx.x == 0) *grand = carry;
+}
+
 // Stable LSD sort of (key, value) u32 pairs on the low 8 * passes bits of the
 // keys, ping-ponging between (k0, v0) and (k1, v1); an even number of passes
 // leaves the result in (k0, v0), an odd number in (k1, v1).  Workspace: hist
@@ -184,15 +257,22 @@ __global__ __launch_bounds__(kScanBlock) void scan_add_kernel(uint64_t* __restri
 inline int radix_sort_passes(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, int64_t n, int passes,
                              uint32_t* hist, uint64_t* hist_off, uint64_t* part, hipStream_t s) {
   if (n <= 1) return P2C_OK;
+  if (n > int64_t(UINT32_MAX)) return set_error(P2C_ECAPACITY, "radix sort: more than 2^32 - 1 keys");
   const int64_t nblk = (n + kSortTile - 1) / kSortTile;
   for (int pass = 0; pass < passes; ++pass) {
     const int shift = pass * 8;
     hipLaunchKernelGGL(radix_count_kernel, dim3(unsigned(nblk)), dim3(kScanBlock), 0, s, k0, n, shift, hist, nblk);
     const int64_t m = 256 * nblk;
-    const int64_t sb = (m + 1023) / 1024;
-    hipLaunchKernelGGL(scan_local_kernel, dim3(unsigned(sb)), dim3(kScanBlock), 0, s, hist, m, hist_off, part);
-    hipLaunchKernelGGL(scan_part_kernel, dim3(1), dim3(kScanBlock), 0, s, part, sb, part + sb);
-    hipLaunchKernelGGL(scan_add_kernel, dim3(unsigned(sb)), dim3(kScanBlock), 0, s, hist_off, m, part);
+    if (m <= kScanSmallMax) {
+      // every prefix of a digit histogram is at most n < 2^32
+      hipLaunchKernelGGL(scan_small_kernel<uint32_t>, dim3(1), dim3(kScanSmallBlock), 0, s, hist, m, hist_off,
+                         nullptr);
+    } else {
+      const int64_t sb = (m + 1023) / 1024;
+      hipLaunchKernelGGL(scan_local_kernel, dim3(unsigned(sb)), dim3(kScanBlock), 0, s, hist, m, hist_off, part);
+      hipLaunchKernelGGL(scan_part_kernel, dim3(1), dim3(kScanBlock), 0, s, part, sb, part + sb);
+      hipLaunchKernelGGL(scan_add_kernel, dim3(unsigned(sb)), dim3(kScanBlock), 0, s, hist_off, m, part);
+    }
     hipLaunchKernelGGL(radix_scatter_kernel, dim3(unsigned(nblk)), dim3(kScanBlock), 0, s, k0, v0, n, shift, hist_off,
                        nblk, k1, v1);
     std::swap(k0, k1);
@@ -212,6 +292,11 @@ inline int radix_sort(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, in
 // n / 1024 + 2 u64.
 inline int exclusive_scan_u32(const uint32_t* in, int64_t n, uint64_t* out, uint64_t* part, hipStream_t s) {
   if (n <= 0) return P2C_OK;
+  if (n <= kScanSmallMax) {
+    hipLaunchKernelGGL(scan_small_kernel<uint64_t>, dim3(1), dim3(kScanSmallBlock), 0, s, in, n, out, out + n);
+    P2C_HIP_CHECK(hipGetLastError());
+    return P2C_OK;
+  }
   const int64_t sb = (n + 1023) / 1024;
   hipLaunchKernelGGL(scan_local_kernel, dim3(unsigned(sb)), dim3(kScanBlock), 0, s, in, n, out, part);
   hipLaunchKernelGGL(scan_part_kernel, dim3(1), dim3(kScanBlock), 0, s, part, sb, out + n);

@@ -136,3 +136,24 @@ def test_pybind_compact_reports_first_error_in_buffer_order(gpu):
     b.hits = hits
     with pytest.raises(ValueError, match="rt_hit_count overflow during compaction"):
         cpp.compact_raw_candidates(s, b)
+
+
+@pytest.mark.parametrize("n", [1, 5, 1023, 16384, 16385, 40000, 65536, 65537, 300001])
+def test_exclusive_scan_both_forms(gpu, n):
+    # Up to 65,536 items the scan runs in one workgroup, above in three
+    # phases; both write the u64 exclusive prefix and the total at out[n].
+    import torch
+    from p2cccd import _native as nat  # p2c_exclusive_scan_u32 is declared by p2cccd.proxy
+
+    rng = np.random.default_rng(n)
+    vals = rng.integers(0, 1 << 32, size=n, dtype=np.uint64).astype(np.uint32)
+    vals[rng.random(n) < 0.3] = 0
+    src = torch.from_numpy(vals.view(np.int32)).to("cuda")
+    out = torch.full((n + 1,), -1, dtype=torch.int64, device="cuda")
+    part = torch.empty(n // 1024 + 2, dtype=torch.int64, device="cuda")
+    nat.call("p2c_exclusive_scan_u32", src.data_ptr(), n, out.data_ptr(), part.data_ptr(),
+             nat.stream_handle(torch.cuda.current_stream()))
+    torch.cuda.synchronize()
+    want = np.zeros(n + 1, dtype=np.uint64)
+    np.cumsum(vals.astype(np.uint64), out=want[1:])
+    assert np.array_equal(out.cpu().numpy().view(np.uint64), want)
