@@ -56,12 +56,17 @@ typedef struct p2c_prim {
   p2c_traj v[4];
 } p2c_prim;
 
-/* CertificateEngineConfig (certificate_engine.h:56-60). */
+/* CertificateEngineConfig (certificate_engine.h:56-60).  `flags` (not in
+ * the reference) selects the walk: 0 = the CPU engine's
+ * (certificate_engine.cpp:300-386), P2C_CERT_CUDA_TWIN = the CUDA batch
+ * twin's (certificate_cuda.cu:286-346), which returns UNDECIDED on the first
+ * interval in pre-order that may not be split instead of searching on. */
+#define P2C_CERT_CUDA_TWIN 1u
 typedef struct p2c_cert_config {
   double eps_time;
   double eps_space;
   uint32_t max_subdivision_depth;
-  uint32_t reserved;
+  uint32_t flags;
 } p2c_cert_config;
 
 /* PrimitiveIntervalResult (certificate_engine.h:69-81): 64 bytes. */
@@ -90,9 +95,11 @@ int p2c_stream_synchronize(void* stream);
 
 /* ---- reference-parity interval certificate ------------------------------
  * Replaces EvaluatePointTriangleBatchCuda / EvaluateEdgeEdgeBatchCuda
- * (certificate_cuda.h:11-20) with CertificateEngine CPU semantics
- * (certificate_engine.cpp:300-386): depth-first left-then-right subdivision,
- * first collision in that order wins, min margin over separated leaves.
+ * (certificate_cuda.h:11-20).  With config->flags 0, CertificateEngine CPU
+ * semantics (certificate_engine.cpp:300-386): depth-first left-then-right
+ * subdivision, first collision in that order wins, min margin over separated
+ * leaves.  With P2C_CERT_CUDA_TWIN, the CUDA twin's: the same walk, ended
+ * by the first collision or by the first interval that may not be split.
  * `prims` and `out` are device pointers of n elements.  A primitive that
  * fails validation gets status P2C_STATUS_INVALID and reason_code 3.
  * `invalid_count` (device uint32, may be NULL) is incremented per such one. */

@@ -56,6 +56,8 @@ def lib() -> ctypes.CDLL:
         _lib = ctypes.CDLL(str(LIB))
         _lib.orc_ref_eval_batch.argtypes = [_i32, _vp, _i64, _f64, _f64, _f64, _f64, _i32, _vp]
         _lib.orc_ref_eval_batch.restype = _i32
+        _lib.orc_ref_eval_twin_batch.argtypes = [_i32, _vp, _i64, _f64, _f64, _f64, _f64, _i32, _vp]
+        _lib.orc_ref_eval_twin_batch.restype = _i32
         _lib.orc_pt_distance.argtypes = [_vp, _f64]
         _lib.orc_pt_distance.restype = _f64
         _lib.orc_ee_distance.argtypes = [_vp, _f64]
@@ -116,6 +118,20 @@ def ref_cert(family: int, prims: np.ndarray, t0=0.0, t1=1.0, eps_time=1.0e-4, ep
     out = np.zeros(prims.shape[0], RESULT)
     rc = lib().orc_ref_eval_batch(family, prims.ctypes.data, prims.shape[0], t0, t1, eps_time, eps_space,
                                   max_depth, out.ctypes.data)
+    if rc != 0:
+        raise ValueError(f"oracle rejected input (code {rc})")
+    return out
+
+
+def ref_cert_twin(family: int, prims: np.ndarray, t0=0.0, t1=1.0, eps_time=1.0e-4, eps_space=1.0e-6,
+                  max_depth=32) -> np.ndarray:
+    """Restated walk of the reference's CUDA batch twin (oracle/ref_cert.c,
+    certificate_cuda.cu:286-346): UNDECIDED at the first interval that may not
+    be split."""
+    prims = np.ascontiguousarray(prims, dtype=PRIM)
+    out = np.zeros(prims.shape[0], RESULT)
+    rc = lib().orc_ref_eval_twin_batch(family, prims.ctypes.data, prims.shape[0], t0, t1, eps_time, eps_space,
+                                       max_depth, out.ctypes.data)
     if rc != 0:
         raise ValueError(f"oracle rejected input (code {rc})")
     return out

@@ -62,6 +62,10 @@ int orc_ref_eval(int family, const orc_prim* prim, double t0, double t1, double 
                  double eps_space, int max_depth, orc_result* out);
 int orc_ref_eval_batch(int family, const orc_prim* prims, int64_t n, double t0, double t1,
                        double eps_time, double eps_space, int max_depth, orc_result* out);
+/* The CUDA batch twin's walk (certificate_cuda.cu:286-346): UNDECIDED on the
+ * first interval that may not be split, in place of searching on. */
+int orc_ref_eval_twin_batch(int family, const orc_prim* prims, int64_t n, double t0, double t1,
+                            double eps_time, double eps_space, int max_depth, orc_result* out);
 double orc_pt_distance(const orc_prim* prim, double t);
 double orc_ee_distance(const orc_prim* prim, double t);
 

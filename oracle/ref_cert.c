@@ -1,6 +1,8 @@
 /*
  * Oracle (test infrastructure only): CPU restatement of the reference's
- * interval certificate, src/cpp/certificate/certificate_engine.cpp.
+ * interval certificate, src/cpp/certificate/certificate_engine.cpp, and of
+ * its CUDA batch twin's walk, src/cpp/certificate/certificate_cuda.cu:286-346
+ * (orc_ref_eval_twin).
  *
  * Written as a plain recursive C program, mirroring the reference's control
  * flow (the shipped HIP kernel uses a different, stackless traversal), so that
@@ -272,6 +274,71 @@ int orc_ref_eval(int family, const orc_prim* prim, double t0, double t1, double 
   Ctx c = {family, prim, eps_time, eps_space, max_depth, 0};
   *out = eval_rec(&c, t0, t1, 0);
   out->nodes = c.nodes;
+  return 0;
+}
+
+/* EvaluateIntervalIterative of the CUDA twin (certificate_cuda.cu:286-346):
+ * an explicit stack of at most 128 intervals, right half pushed under the
+ * left.  The first popped interval that touches (collision) or that may not
+ * be split (depth limit, eps_time, or a stack that two more entries would
+ * fill) ends the walk; where the CPU engine records an undecided leaf and
+ * keeps searching, this walk returns UNDECIDED on that leaf's interval.
+ * Separated intervals contribute max(0, lb - eps_space) to a running
+ * minimum that starts at 1e300.  `nodes` counts popped intervals.  The
+ * distance and motion-radius functions are the CPU engine's (the twin's are
+ * the same formulas; nvcc's default FMA contraction may change their last
+ * bits there). */
+static orc_result eval_twin(Ctx* c, double T0, double T1) {
+  double s0[128], s1[128];
+  int sd[128];
+  int size = 0;
+  s0[size] = T0; s1[size] = T1; sd[size] = 0; ++size;
+  double safe = 1.0e300;
+  while (size > 0) {
+    --size;
+    const double t0 = s0[size], t1 = s1[size];
+    const int depth = sd[size];
+    c->nodes++;
+    const double mid = 0.5 * (t0 + t1);
+    const double d0 = dist(c, t0), dm = dist(c, mid), d1 = dist(c, t1);
+    if (d0 <= c->eps_space || dm <= c->eps_space || d1 <= c->eps_space) {
+      orc_result r = make_result(c, 0, t0, t1);
+      r.toi_upper = d0 <= c->eps_space ? t0 : (dm <= c->eps_space ? mid : t1);
+      r.witness_family = c->family;
+      r.witness_id_a = c->pr->id_a;
+      r.witness_id_b = c->pr->id_b;
+      return r;
+    }
+    const double lb = dm - motion_radius(c->family, c->pr, t0, t1);
+    if (lb > c->eps_space) {
+      safe = fmin(safe, fmax(0.0, lb - c->eps_space));
+      continue;
+    }
+    if (depth >= c->max_depth || t1 - t0 <= c->eps_time || size + 2 >= 128) {
+      orc_result r = make_result(c, 2, t0, t1);
+      r.reason_code = 2;
+      r.next_refinement_mode = 1;
+      return r;
+    }
+    s0[size] = mid; s1[size] = t1; sd[size] = depth + 1; ++size;
+    s0[size] = t0; s1[size] = mid; sd[size] = depth + 1; ++size;
+  }
+  orc_result r = make_result(c, 1, T0, T1);
+  r.safe_margin_lb = safe < 1.0e299 ? safe : 0.0;
+  return r;
+}
+
+int orc_ref_eval_twin_batch(int family, const orc_prim* prims, int64_t n, double t0, double t1,
+                            double eps_time, double eps_space, int max_depth, orc_result* out) {
+  for (int64_t i = 0; i < n; ++i) {
+    /* same validation as the CPU engine (the twin's host wrapper validates first) */
+    orc_result probe;
+    const int rc = orc_ref_eval(family, &prims[i], t0, t1, eps_time, eps_space, 0, &probe);
+    if (rc != 0) return rc;
+    Ctx c = {family, &prims[i], eps_time, eps_space, max_depth, 0};
+    out[i] = eval_twin(&c, t0, t1);
+    out[i].nodes = c.nodes;
+  }
   return 0;
 }
 

@@ -77,7 +77,7 @@ p2c_cert_config ToC(const CertificateEngineConfig& c) {
   o.eps_time = c.eps_time;
   o.eps_space = c.eps_space;
   o.max_subdivision_depth = c.max_subdivision_depth;
-  o.reserved = 0;
+  o.flags = 0;
   return o;
 }
 
@@ -362,7 +362,8 @@ Status RunBatch(int family, const std::vector<Prim>& prims, double t0, double t1
   in.reserve(prims.size());
   for (const Prim& p : prims) in.push_back(Pack(p));
   std::vector<p2c_prim_result> out(prims.size());
-  const p2c_cert_config cfg = ToC(config);
+  p2c_cert_config cfg = ToC(config);
+  cfg.flags = P2C_CERT_CUDA_TWIN;  // the CUDA twin's walk (certificate_cuda.cu:286-346)
   if (auto s = FromRc(p2c_cert_ref_batch_host(family, in.data(), static_cast<std::int64_t>(in.size()),
                                                t0, t1, &cfg, out.data()));
       !s.ok)

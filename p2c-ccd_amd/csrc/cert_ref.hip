@@ -31,9 +31,14 @@ constexpr int kStackCap = 1024;      // pending nodes of the wave pass (LDS)
 constexpr uint32_t kLightBudget = P2C_REF_LIGHT_BUDGET;  // nodes a lane walks before the wave pass takes over
 constexpr uint8_t kHeavyMark = 0xFE;   // result status of a primitive left to the wave pass
 
+// twin: the CUDA batch twin's walk (certificate_cuda.cu:286-346), which
+// returns UNDECIDED on the first interval in pre-order that may not be split;
+// otherwise the CPU engine's (certificate_engine.cpp:300-386), which notes
+// such an interval and searches on for a collision.
 struct RefCfg {
   double t0, t1, eps_time, eps_space;
   uint32_t max_depth;
+  uint32_t twin;
 };
 
 enum : int { kNodeExpand = 0, kNodeCollision = 1, kNodeSeparated = 2, kNodeUndecided = 3 };
@@ -75,11 +80,12 @@ enum : uint32_t { kWalkDone = 0, kWalkCollision = 1, kWalkOverBudget = 2 };
 struct RefWalk {
   uint32_t state;
   uint32_t nodes;
-  uint64_t path;  // the collision node: position, interval, contact time
+  uint64_t path;  // the collision node (or, twin, the unsplittable one): position, interval, contact time
   uint32_t depth;
   double a, b, toi;
   double min_margin;
   bool have_margin, undecided;
+  bool limited;  // twin only: the walk ended on an interval that may not be split
 };
 
 // Pre-order walk (left half first) of the subtree rooted at (path, depth),
@@ -97,7 +103,7 @@ __device__ RefWalk walk_subtree(const RefPrim& q, const double vn[4], const RefC
   w.depth = 0;
   w.a = w.b = w.toi = 0.0;
   w.min_margin = 0.0;
-  w.have_margin = w.undecided = false;
+  w.have_margin = w.undecided = w.limited = false;
   const uint32_t depth0 = depth;
   bool db_known = true;
   for (;;) {
@@ -111,8 +117,9 @@ __device__ RefWalk walk_subtree(const RefPrim& q, const double vn[4], const RefC
     ++w.nodes;
     double dm, toi = 0.0, margin = 0.0;
     const int o = eval_node<kFamily>(q, vn, c, a, b, depth, da, db, &dm, &toi, &margin);
-    if (o == kNodeCollision) {
+    if (o == kNodeCollision || (c.twin && o == kNodeUndecided)) {
       w.state = kWalkCollision;
+      w.limited = o == kNodeUndecided;
       w.path = path;
       w.depth = depth;
       w.a = a;
@@ -143,11 +150,15 @@ __device__ RefWalk walk_subtree(const RefPrim& q, const double vn[4], const RefC
 }
 
 template <int kFamily>
-__device__ void write_result(const RefCfg& c, int64_t id_a, int64_t id_b, bool collision, double a, double b,
-                             double toi, bool undecided, double min_margin, uint32_t nodes,
+__device__ void write_result(const RefCfg& c, int64_t id_a, int64_t id_b, bool collision, bool limited, double a,
+                             double b, double toi, bool undecided, double min_margin, uint32_t nodes,
                              p2c_prim_result* out) {
   p2c_prim_result r;
-  if (collision) {
+  if (collision && limited) {  // twin: UNDECIDED on the unsplittable interval (certificate_cuda.cu:328-335)
+    init_result(&r, P2C_STATUS_UNDECIDED, kFamily, a, b);
+    r.reason_code = 2;
+    r.next_refinement_mode = 1;
+  } else if (collision) {
     init_result(&r, P2C_STATUS_COLLISION, kFamily, a, b);
     r.toi_upper = toi;
     r.witness_family = (uint8_t)kFamily;
@@ -202,7 +213,7 @@ __global__ __launch_bounds__(kRefBlock) void cert_ref_kernel(const p2c_prim* __r
       out[i] = r;
       continue;
     }
-    write_result<kFamily>(c, p.id_a, p.id_b, w.state == kWalkCollision, w.a, w.b, w.toi, w.undecided,
+    write_result<kFamily>(c, p.id_a, p.id_b, w.state == kWalkCollision, w.limited, w.a, w.b, w.toi, w.undecided,
                           w.min_margin, w.nodes, &out[i]);
   }
 }
@@ -259,7 +270,7 @@ __global__ __launch_bounds__(kHeavyBlock) void cert_ref_heavy_kernel(const p2c_p
       const double d0 = ref_distance<kFamily>(q, c.t0);
       const double d1 = ref_distance<kFamily>(q, c.t1);
 
-      bool have_best = false;
+      bool have_best = false, best_limited = false;
       uint64_t best_path = 0;
       uint32_t best_depth = 0;
       double best_a = 0.0, best_b = 0.0, best_toi = 0.0;
@@ -303,6 +314,7 @@ __global__ __launch_bounds__(kHeavyBlock) void cert_ref_heavy_kernel(const p2c_p
             if (hit && pass == 0) {
               // the earliest collision so far: everything still pending follows it
               have_best = true;
+              best_limited = __shfl(w.limited ? 1 : 0, 0) != 0;
               best_path = __shfl(w.path, 0);
               best_depth = __shfl(w.depth, 0);
               best_a = __shfl(w.a, 0);
@@ -337,12 +349,14 @@ __global__ __launch_bounds__(kHeavyBlock) void cert_ref_heavy_kernel(const p2c_p
           int keep = take;
           int base = size - take;
           if (pass == 0) {
-            const uint64_t hits = __ballot(act && o == kNodeCollision);
+            // twin: an interval that may not be split ends the walk like a collision
+            const uint64_t hits = __ballot(act && (o == kNodeCollision || (c.twin && o == kNodeUndecided)));
             if (hits) {
               const int j = __builtin_ctzll(hits);
               keep = j;
               base = 0;
               have_best = true;
+              best_limited = __shfl(o == kNodeUndecided ? 1 : 0, j) != 0;
               best_path = __shfl(path, j);
               best_depth = __shfl(depth, j);
               best_a = __shfl(a, j);
@@ -393,7 +407,8 @@ __global__ __launch_bounds__(kHeavyBlock) void cert_ref_heavy_kernel(const p2c_p
       for (int m = 32; m >= 1; m >>= 1) count += __shfl_xor(count, m);
       if (have_best) {
         if (lane == 0)
-          write_result<kFamily>(c, p.id_a, p.id_b, true, best_a, best_b, best_toi, false, 0.0, count, &out[i]);
+          write_result<kFamily>(c, p.id_a, p.id_b, true, best_limited, best_a, best_b, best_toi, false, 0.0, count,
+                                &out[i]);
       } else {
         double mm = have_margin ? margin_min : __builtin_inf();
         for (int m = 32; m >= 1; m >>= 1) {
@@ -403,7 +418,7 @@ __global__ __launch_bounds__(kHeavyBlock) void cert_ref_heavy_kernel(const p2c_p
         const bool und = __ballot(undecided) != 0;
         const bool any_margin = __ballot(have_margin) != 0;
         if (lane == 0)
-          write_result<kFamily>(c, p.id_a, p.id_b, false, 0.0, 0.0, 0.0, und, any_margin ? mm : 0.0, count,
+          write_result<kFamily>(c, p.id_a, p.id_b, false, false, 0.0, 0.0, 0.0, und, any_margin ? mm : 0.0, count,
                                 &out[i]);
       }
     }
@@ -474,7 +489,7 @@ int p2c_cert_ref_batch(int family, const p2c_prim* prims, int64_t n, double t0, 
   if (n == 0) return P2C_OK;
   if (prims == nullptr || out == nullptr)
     return p2c::set_error(P2C_EINVAL, "primitive and result buffers are required");
-  const p2c::RefCfg c{t0, t1, cfg->eps_time, cfg->eps_space, depth};
+  const p2c::RefCfg c{t0, t1, cfg->eps_time, cfg->eps_space, depth, (cfg->flags & P2C_CERT_CUDA_TWIN) ? 1u : 0u};
   const unsigned grid = p2c::grid_for(n, p2c::kRefBlock);
   const unsigned heavy_grid = p2c::grid_for(n, p2c::kHeavyChunk);
   hipStream_t s = p2c::as_stream(stream);

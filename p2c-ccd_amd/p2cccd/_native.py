@@ -20,7 +20,7 @@ LIB_PATH = _PKG_ROOT / "lib" / "libp2c_hip.so"
 TRAJ_DTYPE = np.dtype([("feature_id", "<i8"), ("position_t0", "<f8", (3,)), ("position_t1", "<f8", (3,))])
 PRIM_DTYPE = np.dtype([("id_a", "<i8"), ("id_b", "<i8"), ("v", TRAJ_DTYPE, (4,))])
 CERT_CONFIG_DTYPE = np.dtype(
-    [("eps_time", "<f8"), ("eps_space", "<f8"), ("max_subdivision_depth", "<u4"), ("reserved", "<u4")]
+    [("eps_time", "<f8"), ("eps_space", "<f8"), ("max_subdivision_depth", "<u4"), ("flags", "<u4")]
 )
 PRIM_RESULT_DTYPE = np.dtype(
     [
@@ -115,9 +115,12 @@ def device_available() -> bool:
     return bool(lib.p2c_device_available())
 
 
-def cert_config(eps_time=1.0e-4, eps_space=1.0e-6, max_subdivision_depth=32) -> np.ndarray:
+def cert_config(eps_time=1.0e-4, eps_space=1.0e-6, max_subdivision_depth=32, cuda_twin=False) -> np.ndarray:
+    """p2c_cert_config; cuda_twin selects the CUDA batch twin's walk
+    (certificate_cuda.cu:286-346) over the CPU engine's."""
     c = np.zeros(1, CERT_CONFIG_DTYPE)
     c["eps_time"], c["eps_space"], c["max_subdivision_depth"] = eps_time, eps_space, max_subdivision_depth
+    c["flags"] = 1 if cuda_twin else 0
     return c
 
 

@@ -214,3 +214,55 @@ def _engine_vs_reference(m):
         assert int(got.status) == int(ref[0]) and got.toi_upper == ref[1] and got.safe_margin_lb == ref[2]
         assert int(got.reason_code) == int(ref[7]) and int(got.next_refinement_mode) == int(ref[8])
         assert got.covered_feature_mask == int(ref[6])
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(max_depth=6), dict(eps_time=1e-8, max_depth=40)])
+def test_cuda_twin_walk_matches_oracle(gpu, kw):
+    # P2C_CERT_CUDA_TWIN: the walk of the reference's CUDA batch twin
+    # (certificate_cuda.cu:286-346), UNDECIDED on the first unsplittable
+    # interval; light and wave passes, node counts included.
+    import torch
+
+    et, d = kw.get("eps_time", 1e-4), kw.get("max_depth", 32)
+    qs = workloads.make_query_set(3000, 3000, seed=99)
+    for fam, x in ((1, qs.vf), (2, qs.ee)):
+        cfg = _native.cert_config(et, 1e-6, d, cuda_twin=True)
+        res, bad = _device_batch(torch, fam, _native.prims_from_positions(x), cfg=cfg)
+        ref = oracle.ref_cert_twin(fam, oracle.prims_from_positions(x), eps_time=et, max_depth=d)
+        assert bad == 0
+        _same(res, ref)
+
+
+def test_module_batch_is_the_cuda_twin(gpu):
+    # evaluate_*_batch_cuda follows the CUDA twin, so on a pair where an
+    # unsplittable interval precedes the first contact it answers UNDECIDED
+    # and the CPU/CUDA cross-check reports the reference's status mismatch.
+    m = pytest.importorskip("p2cccd_cpp")
+    qs = workloads.make_query_set(1500, 1500, seed=99)
+    x = qs.vf
+    op = oracle.prims_from_positions(x)
+    cpu, tw = oracle.ref_cert(1, op), oracle.ref_cert_twin(1, op)
+    idx = np.nonzero((cpu["status"] == 0) & (tw["status"] == 2))[0][:4]
+    assert idx.size > 0
+
+    def v(fid, a, b):
+        t = m.LinearVertexTrajectory()
+        t.feature_id, t.position_t0, t.position_t1 = fid, list(a), list(b)
+        return t
+
+    prims = []
+    for i in idx:
+        p = m.PointTriangleIntervalPrimitive()
+        p.point_id, p.triangle_id = int(i), int(i) + 100000
+        p.point = v(0, x[i, 0], x[i, 4])
+        p.triangle_v0, p.triangle_v1, p.triangle_v2 = (v(k, x[i, k], x[i, 4 + k]) for k in (1, 2, 3))
+        prims.append(p)
+    cfg = m.CertificateEngineConfig()
+    cfg.eps_time, cfg.eps_space, cfg.max_subdivision_depth = 1e-4, 1e-6, 32
+    r = m.evaluate_point_triangle_batch_cuda(prims, 0.0, 1.0, cfg)
+    for k, i in enumerate(idx):
+        assert r[k].status == m.CertificateStatus.UNDECIDED
+        assert (r[k].interval_t0, r[k].interval_t1) == (tw["interval_t0"][i], tw["interval_t1"][i])
+        assert r[k].toi_upper == tw["toi_upper"][i]
+    with pytest.raises(Exception, match="status mismatch"):
+        m.cross_check_cpu_cuda_exact(prims, [], 0.0, 1.0, cfg, 1e-9)

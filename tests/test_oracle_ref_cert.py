@@ -109,3 +109,55 @@ def _kat_ee(z0, z1):
     x[2], x[6] = [0, -1, z0], [0, -1, z1]
     x[3], x[7] = [0, 1, z0], [0, 1, z1]
     return x
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="compiled reference not present")
+@pytest.mark.parametrize("kw", [dict(), dict(max_depth=6), dict(eps_time=1e-2), dict(t0=0.1, t1=0.9, max_depth=6)])
+def test_cuda_twin_walk_against_compiled_engine(kw):
+    # The CUDA twin (certificate_cuda.cu:286-346) cannot be compiled here; its
+    # restatement is pinned through what ties it to the compiled CPU engine:
+    # the two walks visit the same intervals in the same order and differ
+    # only in what an interval that may not be split does (the twin stops on
+    # it, the engine notes it and searches on).
+    qs = workloads.make_query_set(1500, 1500, seed=99)
+    T0, T1 = kw.get("t0", 0.0), kw.get("t1", 1.0)
+    depth_cap, eps_time = kw.get("max_depth", 32), kw.get("eps_time", 1e-4)
+    turned_total = 0
+    for fam, x in ((1, qs.vf), (2, qs.ee)):
+        p = oracle.prims_from_positions(x)
+        cpu = oracle.ref_cert_compiled(fam, p, **kw)
+        own = oracle.ref_cert(fam, p, **kw)  # the restated engine, for node counts
+        tw = oracle.ref_cert_twin(fam, p, **kw)
+        # separation: no interval touched or stopped either walk; every field equal
+        sep = cpu["status"] == 1
+        assert (tw["status"][sep] == 1).all()
+        for f in FIELDS:
+            assert np.array_equal(tw[f][sep], cpu[f][sep]), (fam, f)
+        assert np.array_equal(tw["nodes"][sep], own["nodes"][sep])
+        # a collision is the twin's too unless an unsplittable interval came first
+        col = cpu["status"] == 0
+        kept = col & (tw["status"] == 0)
+        for f in FIELDS:
+            assert np.array_equal(tw[f][kept], cpu[f][kept]), (fam, f)
+        assert np.array_equal(tw["nodes"][kept], own["nodes"][kept])
+        turned = col & (tw["status"] != 0)
+        assert (tw["status"][turned] == 2).all()
+        # that interval lies left of the collision (disjoint nodes: pre-order is left to right)
+        assert (tw["interval_t1"][turned] <= cpu["interval_t0"][turned]).all()
+        # the twin is never undecided where the engine separates; its undecided
+        # results carry the engine's reason and refinement mode
+        und = tw["status"] == 2
+        assert not (und & sep).any()
+        assert (tw["reason_code"][und] == 2).all() and (tw["next_refinement_mode"][und] == 1).all()
+        for i in np.nonzero(und)[0]:
+            a, b = tw["interval_t0"][i], tw["interval_t1"][i]
+            # the compiled engine on that interval alone, with no split allowed,
+            # finds neither a contact nor a separation
+            leaf = oracle.ref_cert_compiled(fam, p[i:i + 1], t0=a, t1=b, eps_time=eps_time, max_depth=0)[0]
+            assert leaf["status"] == 2, (fam, i)
+            # and it may not be split: at the depth cap, or no wider than eps_time
+            depth = int(round(np.log2((T1 - T0) / (b - a))))
+            assert depth >= depth_cap or b - a <= eps_time, (fam, i, depth)
+        turned_total += int(turned.sum())
+    if not kw or "eps_time" in kw:
+        assert turned_total > 0  # the workload holds cases where the two walks disagree
