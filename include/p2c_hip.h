@@ -60,8 +60,14 @@ typedef struct p2c_prim {
  * the reference) selects the walk: 0 = the CPU engine's
  * (certificate_engine.cpp:300-386), P2C_CERT_CUDA_TWIN = the CUDA batch
  * twin's (certificate_cuda.cu:286-346), which returns UNDECIDED on the first
- * interval in pre-order that may not be split instead of searching on. */
+ * interval in pre-order that may not be split instead of searching on,
+ * P2C_CERT_PYTHON_TWIN = the Python twin's (pure_exact_cpu.py:98-244): EE
+ * guard abs(denom) > 1e-14, motion radius 0.5 (t1 - t0) (largest displacement
+ * of each side, summed), contact at the first smallest sample; its TOI is
+ * that sample, t1 otherwise, and pad[0] of each result holds the deepest
+ * node visited. */
 #define P2C_CERT_CUDA_TWIN 1u
+#define P2C_CERT_PYTHON_TWIN 2u
 typedef struct p2c_cert_config {
   double eps_time;
   double eps_space;

@@ -58,6 +58,8 @@ def lib() -> ctypes.CDLL:
         _lib.orc_ref_eval_batch.restype = _i32
         _lib.orc_ref_eval_twin_batch.argtypes = [_i32, _vp, _i64, _f64, _f64, _f64, _f64, _i32, _vp]
         _lib.orc_ref_eval_twin_batch.restype = _i32
+        _lib.orc_ref_eval_py_batch.argtypes = [_i32, _vp, _i64, _f64, _f64, _i32, _vp]
+        _lib.orc_ref_eval_py_batch.restype = _i32
         _lib.orc_pt_distance.argtypes = [_vp, _f64]
         _lib.orc_pt_distance.restype = _f64
         _lib.orc_ee_distance.argtypes = [_vp, _f64]
@@ -132,6 +134,18 @@ def ref_cert_twin(family: int, prims: np.ndarray, t0=0.0, t1=1.0, eps_time=1.0e-
     out = np.zeros(prims.shape[0], RESULT)
     rc = lib().orc_ref_eval_twin_batch(family, prims.ctypes.data, prims.shape[0], t0, t1, eps_time, eps_space,
                                        max_depth, out.ctypes.data)
+    if rc != 0:
+        raise ValueError(f"oracle rejected input (code {rc})")
+    return out
+
+
+def ref_cert_py(family: int, prims: np.ndarray, eps_time=1.0e-5, eps_space=1.0e-8, max_depth=32) -> np.ndarray:
+    """Restated Python twin (oracle/ref_cert.c, pure_exact_cpu.py:98-308) over
+    [0, 1]; `nodes` = exact evals / 3, `reason_code` = deepest depth."""
+    prims = np.ascontiguousarray(prims, dtype=PRIM)
+    out = np.zeros(prims.shape[0], RESULT)
+    rc = lib().orc_ref_eval_py_batch(family, prims.ctypes.data, prims.shape[0], eps_time, eps_space, max_depth,
+                                     out.ctypes.data)
     if rc != 0:
         raise ValueError(f"oracle rejected input (code {rc})")
     return out

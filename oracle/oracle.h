@@ -66,6 +66,11 @@ int orc_ref_eval_batch(int family, const orc_prim* prims, int64_t n, double t0, 
  * first interval that may not be split, in place of searching on. */
 int orc_ref_eval_twin_batch(int family, const orc_prim* prims, int64_t n, double t0, double t1,
                             double eps_time, double eps_space, int max_depth, orc_result* out);
+/* The Python twin (pure_exact_cpu.py:98-308) over [0, 1]: status, toi_upper,
+ * safe_margin_lb, nodes (exact evals / 3) and, in reason_code, the deepest
+ * depth reached. */
+int orc_ref_eval_py_batch(int family, const orc_prim* prims, int64_t n, double eps_time, double eps_space,
+                          int max_depth, orc_result* out);
 double orc_pt_distance(const orc_prim* prim, double t);
 double orc_ee_distance(const orc_prim* prim, double t);
 

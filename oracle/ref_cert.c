@@ -65,6 +65,11 @@ static double dist_ps(V3 p, V3 a, V3 b) {
   return dist_pp(p, v_add(a, v_scale(ab, t)));
 }
 
+/* Denominator guard of the segment-segment distance: 0 = the C++ engine's
+ * `denom != 0.0`, 1 = the Python twin's `abs(denom) > 1e-14`
+ * (pure_exact_cpu.py:126); set only around a Python-twin walk. */
+static int g_py_guard = 0;
+
 /* DistanceSegmentSegment, :92-131 */
 static double dist_ss(V3 p1, V3 q1, V3 p2, V3 q2) {
   const V3 d1 = v_sub(q1, p1);
@@ -84,7 +89,7 @@ static double dist_ss(V3 p1, V3 q1, V3 p2, V3 q2) {
     } else {
       const double b = v_dot(d1, d2);
       const double denom = a * e - b * b;
-      if (denom != 0.0) s = sclamp((b * f - c * e) / denom, 0.0, 1.0);
+      if (g_py_guard ? fabs(denom) > 1.0e-14 : denom != 0.0) s = sclamp((b * f - c * e) / denom, 0.0, 1.0);
       t = (b * s + f) / e;
       if (t < 0.0) {
         t = 0.0;
@@ -338,6 +343,79 @@ int orc_ref_eval_twin_batch(int family, const orc_prim* prims, int64_t n, double
     Ctx c = {family, &prims[i], eps_time, eps_space, max_depth, 0};
     out[i] = eval_twin(&c, t0, t1);
     out[i].nodes = c.nodes;
+  }
+  return 0;
+}
+
+/* The Python twin, _evaluate_recursive (pure_exact_cpu.py:195-244) with the
+ * query set-up of evaluate_external_ccd_query (:247-308): recursive, left
+ * half first; contact when the smallest of the three samples (the first of
+ * equals, as Python's min) is within eps_space, its time the TOI;
+ * separation and undecided report t1.  The motion radius is
+ * 0.5 * max(0, t1 - t0) * D with D the point's displacement plus the
+ * largest triangle-vertex displacement (PT) or the sum of each edge's
+ * largest (EE), computed once.  Returns the status; fills toi, margin,
+ * evals (3 per node) and the deepest depth reached. */
+typedef struct {
+  int status;
+  double toi, margin;
+  int evals, depth;
+} PyOut;
+
+static PyOut eval_py(const Ctx* c, double D, double t0, double t1, int depth) {
+  const double mid = 0.5 * (t0 + t1);
+  const double st[3] = {t0, mid, t1};
+  const double sd[3] = {dist(c, t0), dist(c, mid), dist(c, t1)};
+  int k = 0;
+  for (int i = 1; i < 3; ++i)
+    if (sd[i] < sd[k]) k = i;
+  PyOut r = {0, 0.0, 0.0, 3, depth};
+  if (sd[k] <= c->eps_space) { r.toi = st[k]; return r; }
+  const double lb = sd[1] - 0.5 * smax(0.0, t1 - t0) * D;
+  if (lb > c->eps_space) { r.status = 1; r.toi = t1; r.margin = smax(0.0, lb - c->eps_space); return r; }
+  if (depth >= c->max_depth || t1 - t0 <= c->eps_time) { r.status = 2; r.toi = t1; return r; }
+  const PyOut left = eval_py(c, D, t0, mid, depth + 1);
+  r.evals += left.evals;
+  if (left.status == 0) { r.toi = left.toi; r.depth = left.depth > depth ? left.depth : depth; return r; }
+  const PyOut right = eval_py(c, D, mid, t1, depth + 1);
+  r.evals += right.evals;
+  r.depth = left.depth > right.depth ? left.depth : right.depth;
+  if (right.status == 0) { r.toi = right.toi; return r; }
+  r.toi = t1;
+  if (left.status == 1 && right.status == 1) { r.status = 1; r.margin = smin(left.margin, right.margin); return r; }
+  r.status = 2;
+  return r;
+}
+
+static double vnorm(const orc_traj* tr) { return v_norm(v_sub(v_load(tr->p1), v_load(tr->p0))); }
+
+/* out: status, toi_upper, safe_margin_lb, nodes = evals / 3, reason_code =
+ * deepest depth (the Python result's max_depth) */
+int orc_ref_eval_py_batch(int family, const orc_prim* prims, int64_t n, double eps_time, double eps_space,
+                          int max_depth, orc_result* out) {
+  for (int64_t i = 0; i < n; ++i) {
+    if (!isfinite(eps_time) || eps_time <= 0.0 || !isfinite(eps_space) || eps_space <= 0.0 || max_depth < 0)
+      return -1;
+    const orc_prim* pr = &prims[i];
+    double D;
+    if (family == 1) {
+      double m = vnorm(&pr->v[1]);
+      if (m < vnorm(&pr->v[2])) m = vnorm(&pr->v[2]);
+      if (m < vnorm(&pr->v[3])) m = vnorm(&pr->v[3]);
+      D = vnorm(&pr->v[0]) + m;
+    } else {
+      D = smax(vnorm(&pr->v[0]), vnorm(&pr->v[1])) + smax(vnorm(&pr->v[2]), vnorm(&pr->v[3]));
+    }
+    Ctx c = {family, pr, eps_time, eps_space, max_depth, 0};
+    g_py_guard = 1;
+    const PyOut r = eval_py(&c, D, 0.0, 1.0, 0);
+    g_py_guard = 0;
+    memset(&out[i], 0, sizeof(out[i]));
+    out[i].status = r.status;
+    out[i].toi_upper = r.toi;
+    out[i].safe_margin_lb = r.margin;
+    out[i].nodes = r.evals / 3;
+    out[i].reason_code = (uint16_t)r.depth;
   }
   return 0;
 }

@@ -35,10 +35,14 @@ constexpr uint8_t kHeavyMark = 0xFE;   // result status of a primitive left to t
 // returns UNDECIDED on the first interval in pre-order that may not be split;
 // otherwise the CPU engine's (certificate_engine.cpp:300-386), which notes
 // such an interval and searches on for a collision.
+// py: the Python twin's rules (pure_exact_cpu.py:98-244): EE denominator
+// guard abs(denom) > 1e-14, motion radius 0.5 * (t1 - t0) * (summed largest
+// displacements), contact at the first smallest of the three samples.
 struct RefCfg {
   double t0, t1, eps_time, eps_space;
   uint32_t max_depth;
   uint32_t twin;
+  uint32_t py;
 };
 
 enum : int { kNodeExpand = 0, kNodeCollision = 1, kNodeSeparated = 2, kNodeUndecided = 3 };
@@ -62,11 +66,23 @@ template <int kFamily>
 __device__ inline int eval_node(const RefPrim& q, const double vn[4], const RefCfg& c, double a, double b,
                                 uint32_t depth, double da, double db, double* dm, double* toi, double* margin) {
   const double mid = 0.5 * (a + b);
-  *dm = ref_distance<kFamily>(q, mid);
-  if (da <= c.eps_space) { *toi = a; return kNodeCollision; }
-  if (*dm <= c.eps_space) { *toi = mid; return kNodeCollision; }
-  if (db <= c.eps_space) { *toi = b; return kNodeCollision; }
-  const double lb = *dm - ref_motion_radius_n<kFamily>(vn, a, b);
+  *dm = ref_distance<kFamily>(q, mid, c.py);
+  double lb;
+  if (c.py) {
+    // min over (t0, mid, t1) by distance, the first of equals (Python min)
+    double tm = a, dmin = da;
+    if (*dm < dmin) { tm = mid; dmin = *dm; }
+    if (db < dmin) { tm = b; dmin = db; }
+    if (dmin <= c.eps_space) { *toi = tm; return kNodeCollision; }
+    const double total = kFamily == P2C_FAMILY_POINT_TRIANGLE ? vn[0] + std_max(std_max(vn[1], vn[2]), vn[3])
+                                                              : std_max(vn[0], vn[1]) + std_max(vn[2], vn[3]);
+    lb = *dm - 0.5 * std_max(0.0, b - a) * total;
+  } else {
+    if (da <= c.eps_space) { *toi = a; return kNodeCollision; }
+    if (*dm <= c.eps_space) { *toi = mid; return kNodeCollision; }
+    if (db <= c.eps_space) { *toi = b; return kNodeCollision; }
+    lb = *dm - ref_motion_radius_n<kFamily>(vn, a, b);
+  }
   if (lb > c.eps_space) {
     *margin = std_max(0.0, lb - c.eps_space);
     return kNodeSeparated;
@@ -86,6 +102,7 @@ struct RefWalk {
   double min_margin;
   bool have_margin, undecided;
   bool limited;  // twin only: the walk ended on an interval that may not be split
+  uint32_t max_depth;  // deepest node visited
 };
 
 // Pre-order walk (left half first) of the subtree rooted at (path, depth),
@@ -104,6 +121,7 @@ __device__ RefWalk walk_subtree(const RefPrim& q, const double vn[4], const RefC
   w.a = w.b = w.toi = 0.0;
   w.min_margin = 0.0;
   w.have_margin = w.undecided = w.limited = false;
+  w.max_depth = depth;
   const uint32_t depth0 = depth;
   bool db_known = true;
   for (;;) {
@@ -113,8 +131,9 @@ __device__ RefWalk walk_subtree(const RefPrim& q, const double vn[4], const RefC
     }
     double a, b;
     node_interval(c, path, depth, &a, &b);
-    if (!db_known) db = ref_distance<kFamily>(q, b);
+    if (!db_known) db = ref_distance<kFamily>(q, b, c.py);
     ++w.nodes;
+    w.max_depth = depth > w.max_depth ? depth : w.max_depth;
     double dm, toi = 0.0, margin = 0.0;
     const int o = eval_node<kFamily>(q, vn, c, a, b, depth, da, db, &dm, &toi, &margin);
     if (o == kNodeCollision || (c.twin && o == kNodeUndecided)) {
@@ -152,7 +171,7 @@ __device__ RefWalk walk_subtree(const RefPrim& q, const double vn[4], const RefC
 template <int kFamily>
 __device__ void write_result(const RefCfg& c, int64_t id_a, int64_t id_b, bool collision, bool limited, double a,
                              double b, double toi, bool undecided, double min_margin, uint32_t nodes,
-                             p2c_prim_result* out) {
+                             uint32_t max_depth, p2c_prim_result* out) {
   p2c_prim_result r;
   if (collision && limited) {  // twin: UNDECIDED on the unsplittable interval (certificate_cuda.cu:328-335)
     init_result(&r, P2C_STATUS_UNDECIDED, kFamily, a, b);
@@ -173,6 +192,7 @@ __device__ void write_result(const RefCfg& c, int64_t id_a, int64_t id_b, bool c
     r.safe_margin_lb = min_margin;
   }
   r.nodes = nodes;
+  r.pad[0] = (uint8_t)max_depth;  // deepest node visited (the Python twin's max_depth)
   *out = r;
 }
 
@@ -204,8 +224,8 @@ __global__ __launch_bounds__(kRefBlock) void cert_ref_kernel(const p2c_prim* __r
     const RefPrim q = load_prim(p);
     double vn[4];
     vertex_norms(q, vn);
-    const double d0 = ref_distance<kFamily>(q, c.t0);
-    const double d1 = ref_distance<kFamily>(q, c.t1);
+    const double d0 = ref_distance<kFamily>(q, c.t0, c.py);
+    const double d1 = ref_distance<kFamily>(q, c.t1, c.py);
     const RefWalk w = walk_subtree<kFamily>(q, vn, c, 0, 0, d0, d1, budget);
     if (w.state == kWalkOverBudget) {
       init_result(&r, P2C_STATUS_UNDECIDED, kFamily, c.t0, c.t1);
@@ -214,7 +234,7 @@ __global__ __launch_bounds__(kRefBlock) void cert_ref_kernel(const p2c_prim* __r
       continue;
     }
     write_result<kFamily>(c, p.id_a, p.id_b, w.state == kWalkCollision, w.limited, w.a, w.b, w.toi, w.undecided,
-                          w.min_margin, w.nodes, &out[i]);
+                          w.min_margin, w.nodes, w.max_depth, &out[i]);
   }
 }
 
@@ -267,14 +287,15 @@ __global__ __launch_bounds__(kHeavyBlock) void cert_ref_heavy_kernel(const p2c_p
       const RefPrim q = load_prim(p);
       double vn[4];
       vertex_norms(q, vn);
-      const double d0 = ref_distance<kFamily>(q, c.t0);
-      const double d1 = ref_distance<kFamily>(q, c.t1);
+      const double d0 = ref_distance<kFamily>(q, c.t0, c.py);
+      const double d1 = ref_distance<kFamily>(q, c.t1, c.py);
 
       bool have_best = false, best_limited = false;
       uint64_t best_path = 0;
       uint32_t best_depth = 0;
       double best_a = 0.0, best_b = 0.0, best_toi = 0.0;
       uint32_t count = 0;  // per lane
+      uint32_t maxd = 0;   // per lane: deepest node counted
       double margin_min = 0.0;
       bool have_margin = false, undecided = false;
 
@@ -282,6 +303,7 @@ __global__ __launch_bounds__(kHeavyBlock) void cert_ref_heavy_kernel(const p2c_p
         if (pass == 1) {
           if (!have_best) break;
           count = 0;
+          maxd = 0;
         }
         __syncthreads();
         if (lane == 0) {
@@ -303,6 +325,7 @@ __global__ __launch_bounds__(kHeavyBlock) void cert_ref_heavy_kernel(const p2c_p
             const bool hit = __shfl(w.state == kWalkCollision ? 1 : 0, 0) != 0;
             if (lane == 0) {
               count += w.nodes;
+              maxd = w.max_depth > maxd ? w.max_depth : maxd;
               if (pass == 0 && !hit) {
                 if (w.have_margin) {
                   margin_min = have_margin ? std_min(margin_min, w.min_margin) : w.min_margin;
@@ -372,7 +395,10 @@ __global__ __launch_bounds__(kHeavyBlock) void cert_ref_heavy_kernel(const p2c_p
               }
             }
           }
-          if (act && lane <= keep) count += 1;  // pass 1's count is used only without a collision
+          if (act && lane <= keep) {  // pass 1's count is used only without a collision
+            count += 1;
+            maxd = depth > maxd ? depth : maxd;
+          }
           const bool kept = act && lane < keep && o == kNodeExpand;
           const uint64_t right = path | (1ull << depth);
           bool push_l = kept, push_r = kept;
@@ -404,11 +430,15 @@ __global__ __launch_bounds__(kHeavyBlock) void cert_ref_heavy_kernel(const p2c_p
       }
 
       // wave reductions
-      for (int m = 32; m >= 1; m >>= 1) count += __shfl_xor(count, m);
+      for (int m = 32; m >= 1; m >>= 1) {
+        count += __shfl_xor(count, m);
+        const uint32_t od = __shfl_xor(maxd, m);
+        maxd = od > maxd ? od : maxd;
+      }
       if (have_best) {
         if (lane == 0)
           write_result<kFamily>(c, p.id_a, p.id_b, true, best_limited, best_a, best_b, best_toi, false, 0.0, count,
-                                &out[i]);
+                                maxd, &out[i]);
       } else {
         double mm = have_margin ? margin_min : __builtin_inf();
         for (int m = 32; m >= 1; m >>= 1) {
@@ -419,7 +449,7 @@ __global__ __launch_bounds__(kHeavyBlock) void cert_ref_heavy_kernel(const p2c_p
         const bool any_margin = __ballot(have_margin) != 0;
         if (lane == 0)
           write_result<kFamily>(c, p.id_a, p.id_b, false, false, 0.0, 0.0, 0.0, und, any_margin ? mm : 0.0, count,
-                                &out[i]);
+                                maxd, &out[i]);
       }
     }
   }
@@ -489,7 +519,10 @@ int p2c_cert_ref_batch(int family, const p2c_prim* prims, int64_t n, double t0, 
   if (n == 0) return P2C_OK;
   if (prims == nullptr || out == nullptr)
     return p2c::set_error(P2C_EINVAL, "primitive and result buffers are required");
-  const p2c::RefCfg c{t0, t1, cfg->eps_time, cfg->eps_space, depth, (cfg->flags & P2C_CERT_CUDA_TWIN) ? 1u : 0u};
+  if ((cfg->flags & P2C_CERT_CUDA_TWIN) && (cfg->flags & P2C_CERT_PYTHON_TWIN))
+    return p2c::set_error(P2C_EINVAL, "choose one certificate walk: CUDA twin or Python twin");
+  const p2c::RefCfg c{t0, t1, cfg->eps_time, cfg->eps_space, depth, (cfg->flags & P2C_CERT_CUDA_TWIN) ? 1u : 0u,
+                      (cfg->flags & P2C_CERT_PYTHON_TWIN) ? 1u : 0u};
   const unsigned grid = p2c::grid_for(n, p2c::kRefBlock);
   const unsigned heavy_grid = p2c::grid_for(n, p2c::kHeavyChunk);
   hipStream_t s = p2c::as_stream(stream);

@@ -68,8 +68,9 @@ P2C_HD double dist_point_segment(D3 p, D3 a, D3 b) {
 }
 
 // Clamped segment-segment closest distance with the reference's `denom != 0`
-// guard and re-clamping order.
-P2C_HD double dist_segment_segment(D3 p1, D3 q1, D3 p2, D3 q2) {
+// guard and re-clamping order; `py` takes the Python twin's guard
+// `abs(denom) > 1e-14` (pure_exact_cpu.py:126) instead.
+P2C_HD double dist_segment_segment(D3 p1, D3 q1, D3 p2, D3 q2, bool py = false) {
   const D3 d1 = sub3(q1, p1);
   const D3 d2 = sub3(q2, p2);
   const D3 r = sub3(p1, p2);
@@ -87,7 +88,7 @@ P2C_HD double dist_segment_segment(D3 p1, D3 q1, D3 p2, D3 q2) {
     } else {
       const double b = dot3(d1, d2);
       const double denom = a * e - b * b;
-      if (denom != 0.0) s = std_clamp((b * f - c * e) / denom, 0.0, 1.0);
+      if (py ? (denom < 0.0 ? -denom : denom) > 1.0e-14 : denom != 0.0) s = std_clamp((b * f - c * e) / denom, 0.0, 1.0);
       t = (b * s + f) / e;
       if (t < 0.0) {
         t = 0.0;
@@ -146,13 +147,13 @@ P2C_HD double dist_point_triangle(D3 p, D3 a, D3 b, D3 c) {
 }
 
 template <int kFamily>
-P2C_HD double ref_distance(const RefPrim& q, double t) {
+P2C_HD double ref_distance(const RefPrim& q, double t, bool py = false) {
   const D3 x0 = lerp_at(q.p0[0], q.p1[0], t);
   const D3 x1 = lerp_at(q.p0[1], q.p1[1], t);
   const D3 x2 = lerp_at(q.p0[2], q.p1[2], t);
   const D3 x3 = lerp_at(q.p0[3], q.p1[3], t);
   if (kFamily == P2C_FAMILY_POINT_TRIANGLE) return dist_point_triangle(x0, x1, x2, x3);
-  return dist_segment_segment(x0, x1, x2, x3);
+  return dist_segment_segment(x0, x1, x2, x3, py);
 }
 
 // 0.5 * max(0, t1 - t0) * |p1 - p0|, per vertex.

@@ -266,3 +266,56 @@ def test_module_batch_is_the_cuda_twin(gpu):
         assert r[k].toi_upper == tw["toi_upper"][i]
     with pytest.raises(Exception, match="status mismatch"):
         m.cross_check_cpu_cuda_exact(prims, [], 0.0, 1.0, cfg, 1e-9)
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(eps_time=1e-4, max_depth=12), dict(eps_time=1e-9, max_depth=40)])
+def test_python_twin_walk_matches_oracle(gpu, kw):
+    # P2C_CERT_PYTHON_TWIN: the reference's Python twin (pure_exact_cpu.py:
+    # 98-308) against its restatement: status, TOI, margin, evaluations and
+    # deepest depth, through the light and wave passes.
+    import torch
+
+    et, es, d = kw.get("eps_time", 1e-5), 1e-8, kw.get("max_depth", 32)
+    qs = workloads.make_query_set(3000, 3000, seed=41)
+    for fam, x in ((1, qs.vf), (2, qs.ee)):
+        cfg = _native.cert_config(et, es, d)
+        cfg["flags"] = 2
+        res, bad = _device_batch(torch, fam, _native.prims_from_positions(x), cfg=cfg)
+        ref = oracle.ref_cert_py(fam, oracle.prims_from_positions(x), eps_time=et, eps_space=es, max_depth=d)
+        assert bad == 0
+        _same(res, ref, ("status", "toi_upper", "safe_margin_lb", "nodes"))
+        assert np.array_equal(res["pad"][:, 0].astype(np.int64), ref["reason_code"].astype(np.int64))
+
+
+def test_external_ccd_query_module(gpu):
+    # p2cccd.pure_exact mirrors evaluate_external_ccd_query: the reference's
+    # checks (test_pure_exact_cpu_baseline.py:57-70, :101-108) and the closed
+    # forms of the restatement's tests, through one batched launch.
+    from types import SimpleNamespace as Q
+
+    from p2cccd import pure_exact as pe
+
+    tri = ((0.0, 0.0, 0.0), (1.0, 0.0, 0.0), (0.0, 1.0, 0.0))
+
+    def vf(z0, z1, qid):
+        return Q(query_id=qid, family="vf", vertices_t0=((0.25, 0.25, z0),) + tri,
+                 vertices_t1=((0.25, 0.25, z1),) + tri, ground_truth_collides=None)
+
+    def ee(z0, z1, qid):
+        return Q(query_id=qid, family="ee", vertices_t0=((-1.0, 0.0, 0.0), (1.0, 0.0, 0.0), (0.0, -1.0, z0), (0.0, 1.0, z0)),
+                 vertices_t1=((-1.0, 0.0, 0.0), (1.0, 0.0, 0.0), (0.0, -1.0, z1), (0.0, 1.0, z1)),
+                 ground_truth_collides=True)
+
+    cfg = pe.PureExactCPUConfig(eps_time=1.0e-6, eps_space=1.0e-8, max_subdivision_depth=28)
+    r = pe.evaluate_external_ccd_queries([vf(1.0, -1.0, 1), ee(1.0, -1.0, 2), vf(2.0, 2.0, 3), vf(1.0, -3.0, 4),
+                                          vf(5e-9, -5e-9, 5)], cfg)
+    assert [x.query_id for x in r] == [1, 2, 3, 4, 5]
+    assert r[0].predicted_collision and r[0].toi_upper <= 0.5 and r[0].family == "point_triangle"
+    assert r[1].predicted_collision and r[1].toi_upper <= 0.5 and r[1].family == "edge_edge"
+    assert not r[2].predicted_collision and r[2].status == "separation"
+    assert (r[0].toi_upper, r[0].exact_evals, r[0].max_depth) == (0.5, 3, 0)
+    assert (r[3].status, r[3].toi_upper, r[3].exact_evals, r[3].max_depth) == ("collision", 0.25, 6, 1)
+    assert (r[2].toi_upper, r[2].safe_margin_lb, r[2].exact_evals) == (1.0, 2.0 - 1.0e-8, 3)
+    assert r[4].toi_upper == 0.5
+    with pytest.raises(ValueError, match="eps_time"):
+        pe.evaluate_external_ccd_query(vf(1.0, -1.0, 1), pe.PureExactCPUConfig(eps_time=0.0))

@@ -161,3 +161,70 @@ def test_cuda_twin_walk_against_compiled_engine(kw):
         turned_total += int(turned.sum())
     if not kw or "eps_time" in kw:
         assert turned_total > 0  # the workload holds cases where the two walks disagree
+
+
+def _pt_z(z0, z1):
+    # the reference's PT fixture (src/tests/python/test_pure_exact_cpu_baseline.py:26-37)
+    x = np.zeros((1, 8, 3))
+    x[0, 0], x[0, 4] = (0.25, 0.25, z0), (0.25, 0.25, z1)
+    for k, v in ((1, (0, 0, 0)), (2, (1, 0, 0)), (3, (0, 1, 0))):
+        x[0, k] = x[0, 4 + k] = v
+    return oracle.prims_from_positions(x)
+
+
+def _ee_z(z0, z1):
+    # the reference's EE fixture (:40-51)
+    x = np.zeros((1, 8, 3))
+    x[0, 0] = x[0, 4] = (-1, 0, 0)
+    x[0, 1] = x[0, 5] = (1, 0, 0)
+    x[0, 2], x[0, 6] = (0, -1, z0), (0, -1, z1)
+    x[0, 3], x[0, 7] = (0, 1, z0), (0, 1, z1)
+    return oracle.prims_from_positions(x)
+
+
+def test_python_twin_restatement_kats():
+    # The reference's own checks (test_pure_exact_cpu_baseline.py:57-70),
+    # then closed forms on the same fixtures, where the distance is |z(t)|
+    # and z(t) = (1 - t) z0 + t z1 is exact at the sampled times.
+    kw = dict(eps_time=1.0e-6, eps_space=1.0e-8, max_depth=28)
+    pt = oracle.ref_cert_py(1, _pt_z(1.0, -1.0), **kw)[0]
+    ee = oracle.ref_cert_py(2, _ee_z(1.0, -1.0), **kw)[0]
+    sep = oracle.ref_cert_py(1, _pt_z(2.0, 2.0), **kw)[0]
+    assert pt["status"] == 0 and pt["toi_upper"] <= 0.5
+    assert ee["status"] == 0 and ee["toi_upper"] <= 0.5
+    assert sep["status"] == 1
+    # z 1 -> -1: the root's midpoint touches; 3 evaluations, depth 0
+    assert (pt["toi_upper"], pt["nodes"], pt["reason_code"]) == (0.5, 1, 0)
+    # z 1 -> -3: root [0,1] gives |z| = 1, 1, 3 and bound 1 - 0.5 * 4 < 0; its
+    # left half touches at t = 0.25: 6 evaluations, depth 1
+    r = oracle.ref_cert_py(1, _pt_z(1.0, -3.0), **kw)[0]
+    assert (r["status"], r["toi_upper"], r["nodes"], r["reason_code"]) == (0, 0.25, 2, 1)
+    # static at z = 2: one node, margin 2 - eps_space, TOI t1
+    assert (sep["toi_upper"], sep["safe_margin_lb"], sep["nodes"]) == (1.0, 2.0 - 1.0e-8, 1)
+    # contact time is the smallest sample, not the first within eps_space:
+    # z 5e-9 -> -5e-9 touches at every sample, smallest (0) at the midpoint;
+    # the C++ engine reports the first, t = 0
+    r = oracle.ref_cert_py(1, _pt_z(5e-9, -5e-9), **kw)[0]
+    assert r["status"] == 0 and r["toi_upper"] == 0.5
+    e = oracle.ref_cert(1, _pt_z(5e-9, -5e-9), eps_time=1e-6, eps_space=1e-8, max_depth=28)[0]
+    assert e["status"] == 0 and e["toi_upper"] == 0.0
+    # an undecided walk reports t1 and margin 0, and its depth is the cap
+    r = oracle.ref_cert_py(1, _pt_z(1.0, -1.0 + 1e-3), eps_time=1e-6, eps_space=1e-8, max_depth=2)[0]
+    assert r["status"] in (0, 2)
+    if r["status"] == 2:
+        assert (r["toi_upper"], r["safe_margin_lb"], r["reason_code"]) == (1.0, 0.0, 2)
+
+
+def test_python_twin_agrees_with_engine_decisions():
+    # Same subdivision and tests as the C++ engine, up to the contact-time
+    # rule and the rounding of the motion radius: the decisions of the two
+    # agree on nearly every pair of the plain workload, and where both touch,
+    # the twin's contact time lies in the engine's contact interval or later.
+    qs = workloads.make_query_set(1500, 1500, seed=99)
+    for fam, x in ((1, qs.vf), (2, qs.ee)):
+        p = oracle.prims_from_positions(x)
+        e = oracle.ref_cert(fam, p, eps_time=1e-5, eps_space=1e-8)
+        y = oracle.ref_cert_py(fam, p, eps_time=1e-5, eps_space=1e-8)
+        assert np.mean(e["status"] == y["status"]) > 0.98
+        both = (e["status"] == 0) & (y["status"] == 0)
+        assert (y["toi_upper"][both] >= e["interval_t0"][both]).all()
