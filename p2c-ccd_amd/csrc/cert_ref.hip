@@ -31,19 +31,22 @@ constexpr int kStackCap = 1024;      // pending nodes of the wave pass (LDS)
 constexpr uint32_t kLightBudget = P2C_REF_LIGHT_BUDGET;  // nodes a lane walks before the wave pass takes over
 constexpr uint8_t kHeavyMark = 0xFE;   // result status of a primitive left to the wave pass
 
-// twin: the CUDA batch twin's walk (certificate_cuda.cu:286-346), which
-// returns UNDECIDED on the first interval in pre-order that may not be split;
-// otherwise the CPU engine's (certificate_engine.cpp:300-386), which notes
-// such an interval and searches on for a collision.
-// py: the Python twin's rules (pure_exact_cpu.py:98-244): EE denominator
-// guard abs(denom) > 1e-14, motion radius 0.5 * (t1 - t0) * (summed largest
-// displacements), contact at the first smallest of the three samples.
 struct RefCfg {
   double t0, t1, eps_time, eps_space;
   uint32_t max_depth;
-  uint32_t twin;
-  uint32_t py;
 };
+
+// The walk, a template parameter of the kernels (kMode):
+//   kModeEngine    the CPU engine's (certificate_engine.cpp:300-386): an
+//                  interval that may not be split is noted and the search
+//                  goes on for a collision;
+//   kModeCudaTwin  the CUDA batch twin's (certificate_cuda.cu:286-346):
+//                  UNDECIDED on the first such interval in pre-order;
+//   kModePython    the Python twin's rules (pure_exact_cpu.py:98-244): EE
+//                  denominator guard abs(denom) > 1e-14, motion radius
+//                  0.5 * (t1 - t0) * (summed largest displacements), contact
+//                  at the first smallest of the three samples.
+enum : int { kModeEngine = 0, kModeCudaTwin = 1, kModePython = 2 };
 
 enum : int { kNodeExpand = 0, kNodeCollision = 1, kNodeSeparated = 2, kNodeUndecided = 3 };
 
@@ -62,13 +65,13 @@ __device__ inline void node_interval(const RefCfg& c, uint64_t path, uint32_t de
 // One node of certificate_engine.cpp:300-386 given the distances at its two
 // ends (da at a, db at b): the endpoint/midpoint contact tests in the
 // reference's order, then the Lipschitz separation test, then the leaf rule.
-template <int kFamily>
+template <int kFamily, int kMode>
 __device__ inline int eval_node(const RefPrim& q, const double vn[4], const RefCfg& c, double a, double b,
                                 uint32_t depth, double da, double db, double* dm, double* toi, double* margin) {
   const double mid = 0.5 * (a + b);
-  *dm = ref_distance<kFamily>(q, mid, c.py);
+  *dm = ref_distance<kFamily>(q, mid, kMode == kModePython);
   double lb;
-  if (c.py) {
+  if (kMode == kModePython) {
     // min over (t0, mid, t1) by distance, the first of equals (Python min)
     double tm = a, dmin = da;
     if (*dm < dmin) { tm = mid; dmin = *dm; }
@@ -110,7 +113,7 @@ struct RefWalk {
 // distances at the root's two ends.  A left child reuses its parent's start
 // and midpoint distances; the right sibling after a finished left subtree
 // starts where that subtree's last node ended, so only its end is new.
-template <int kFamily>
+template <int kFamily, int kMode>
 __device__ RefWalk walk_subtree(const RefPrim& q, const double vn[4], const RefCfg& c, uint64_t path,
                                 uint32_t depth, double da, double db, uint32_t budget) {
   RefWalk w;
@@ -131,12 +134,12 @@ __device__ RefWalk walk_subtree(const RefPrim& q, const double vn[4], const RefC
     }
     double a, b;
     node_interval(c, path, depth, &a, &b);
-    if (!db_known) db = ref_distance<kFamily>(q, b, c.py);
+    if (!db_known) db = ref_distance<kFamily>(q, b, kMode == kModePython);
     ++w.nodes;
-    w.max_depth = depth > w.max_depth ? depth : w.max_depth;
+    if (kMode == kModePython) w.max_depth = depth > w.max_depth ? depth : w.max_depth;  // reported by that walk only
     double dm, toi = 0.0, margin = 0.0;
-    const int o = eval_node<kFamily>(q, vn, c, a, b, depth, da, db, &dm, &toi, &margin);
-    if (o == kNodeCollision || (c.twin && o == kNodeUndecided)) {
+    const int o = eval_node<kFamily, kMode>(q, vn, c, a, b, depth, da, db, &dm, &toi, &margin);
+    if (o == kNodeCollision || (kMode == kModeCudaTwin && o == kNodeUndecided)) {
       w.state = kWalkCollision;
       w.limited = o == kNodeUndecided;
       w.path = path;
@@ -203,7 +206,7 @@ __device__ inline void vertex_norms(const RefPrim& q, double vn[4]) {
 
 // Light pass: one lane per primitive, at most kLightBudget nodes; a primitive
 // that needs more is marked for the wave pass.
-template <int kFamily>
+template <int kFamily, int kMode>
 __global__ __launch_bounds__(kRefBlock) void cert_ref_kernel(const p2c_prim* __restrict__ prims, int64_t n,
                                                              RefCfg c, uint32_t budget,
                                                              p2c_prim_result* __restrict__ out,
@@ -224,9 +227,9 @@ __global__ __launch_bounds__(kRefBlock) void cert_ref_kernel(const p2c_prim* __r
     const RefPrim q = load_prim(p);
     double vn[4];
     vertex_norms(q, vn);
-    const double d0 = ref_distance<kFamily>(q, c.t0, c.py);
-    const double d1 = ref_distance<kFamily>(q, c.t1, c.py);
-    const RefWalk w = walk_subtree<kFamily>(q, vn, c, 0, 0, d0, d1, budget);
+    const double d0 = ref_distance<kFamily>(q, c.t0, kMode == kModePython);
+    const double d1 = ref_distance<kFamily>(q, c.t1, kMode == kModePython);
+    const RefWalk w = walk_subtree<kFamily, kMode>(q, vn, c, 0, 0, d0, d1, budget);
     if (w.state == kWalkOverBudget) {
       init_result(&r, P2C_STATUS_UNDECIDED, kFamily, c.t0, c.t1);
       r.status = kHeavyMark;
@@ -270,7 +273,7 @@ __device__ inline uint64_t lanes_below(int lane) { return lane == 0 ? 0ull : (~0
 // after that collision.  A full stack falls back to one lane walking the
 // leftmost pending subtree in sequence.  Result equal to the sequential walk's,
 // node count included.
-template <int kFamily>
+template <int kFamily, int kMode>
 __global__ __launch_bounds__(kHeavyBlock) void cert_ref_heavy_kernel(const p2c_prim* __restrict__ prims,
                                                                      int64_t n, RefCfg c,
                                                                      p2c_prim_result* __restrict__ out) {
@@ -287,8 +290,8 @@ __global__ __launch_bounds__(kHeavyBlock) void cert_ref_heavy_kernel(const p2c_p
       const RefPrim q = load_prim(p);
       double vn[4];
       vertex_norms(q, vn);
-      const double d0 = ref_distance<kFamily>(q, c.t0, c.py);
-      const double d1 = ref_distance<kFamily>(q, c.t1, c.py);
+      const double d0 = ref_distance<kFamily>(q, c.t0, kMode == kModePython);
+      const double d1 = ref_distance<kFamily>(q, c.t1, kMode == kModePython);
 
       bool have_best = false, best_limited = false;
       uint64_t best_path = 0;
@@ -320,12 +323,12 @@ __global__ __launch_bounds__(kHeavyBlock) void cert_ref_heavy_kernel(const p2c_p
             // subtree to its end (or to its first collision)
             RefWalk w = {};
             if (lane == 0)
-              w = walk_subtree<kFamily>(q, vn, c, st.path[size - 1], st.depth[size - 1], st.da[size - 1],
+              w = walk_subtree<kFamily, kMode>(q, vn, c, st.path[size - 1], st.depth[size - 1], st.da[size - 1],
                                         st.db[size - 1], 0xFFFFFFFFu);
             const bool hit = __shfl(w.state == kWalkCollision ? 1 : 0, 0) != 0;
             if (lane == 0) {
               count += w.nodes;
-              maxd = w.max_depth > maxd ? w.max_depth : maxd;
+              if (kMode == kModePython) maxd = w.max_depth > maxd ? w.max_depth : maxd;
               if (pass == 0 && !hit) {
                 if (w.have_margin) {
                   margin_min = have_margin ? std_min(margin_min, w.min_margin) : w.min_margin;
@@ -367,13 +370,13 @@ __global__ __launch_bounds__(kHeavyBlock) void cert_ref_heavy_kernel(const p2c_p
           double a = 0.0, b = 0.0, dm = 0.0, toi = 0.0, margin = 0.0;
           if (act) {
             node_interval(c, path, depth, &a, &b);
-            o = eval_node<kFamily>(q, vn, c, a, b, depth, da, db, &dm, &toi, &margin);
+            o = eval_node<kFamily, kMode>(q, vn, c, a, b, depth, da, db, &dm, &toi, &margin);
           }
           int keep = take;
           int base = size - take;
           if (pass == 0) {
             // twin: an interval that may not be split ends the walk like a collision
-            const uint64_t hits = __ballot(act && (o == kNodeCollision || (c.twin && o == kNodeUndecided)));
+            const uint64_t hits = __ballot(act && (o == kNodeCollision || (kMode == kModeCudaTwin && o == kNodeUndecided)));
             if (hits) {
               const int j = __builtin_ctzll(hits);
               keep = j;
@@ -397,7 +400,7 @@ __global__ __launch_bounds__(kHeavyBlock) void cert_ref_heavy_kernel(const p2c_p
           }
           if (act && lane <= keep) {  // pass 1's count is used only without a collision
             count += 1;
-            maxd = depth > maxd ? depth : maxd;
+            if (kMode == kModePython) maxd = depth > maxd ? depth : maxd;
           }
           const bool kept = act && lane < keep && o == kNodeExpand;
           const uint64_t right = path | (1ull << depth);
@@ -521,22 +524,32 @@ int p2c_cert_ref_batch(int family, const p2c_prim* prims, int64_t n, double t0, 
     return p2c::set_error(P2C_EINVAL, "primitive and result buffers are required");
   if ((cfg->flags & P2C_CERT_CUDA_TWIN) && (cfg->flags & P2C_CERT_PYTHON_TWIN))
     return p2c::set_error(P2C_EINVAL, "choose one certificate walk: CUDA twin or Python twin");
-  const p2c::RefCfg c{t0, t1, cfg->eps_time, cfg->eps_space, depth, (cfg->flags & P2C_CERT_CUDA_TWIN) ? 1u : 0u,
-                      (cfg->flags & P2C_CERT_PYTHON_TWIN) ? 1u : 0u};
+  const p2c::RefCfg c{t0, t1, cfg->eps_time, cfg->eps_space, depth};
+  const int mode = (cfg->flags & P2C_CERT_CUDA_TWIN)     ? p2c::kModeCudaTwin
+                   : (cfg->flags & P2C_CERT_PYTHON_TWIN) ? p2c::kModePython
+                                                         : p2c::kModeEngine;
   const unsigned grid = p2c::grid_for(n, p2c::kRefBlock);
   const unsigned heavy_grid = p2c::grid_for(n, p2c::kHeavyChunk);
   hipStream_t s = p2c::as_stream(stream);
-  if (family == P2C_FAMILY_POINT_TRIANGLE) {
-    hipLaunchKernelGGL(p2c::cert_ref_kernel<P2C_FAMILY_POINT_TRIANGLE>, dim3(grid), dim3(p2c::kRefBlock), 0, s,
-                       prims, n, c, p2c::kLightBudget, out, invalid_count);
-    hipLaunchKernelGGL(p2c::cert_ref_heavy_kernel<P2C_FAMILY_POINT_TRIANGLE>, dim3(heavy_grid),
-                       dim3(p2c::kHeavyBlock), 0, s, prims, n, c, out);
+  const bool pt = family == P2C_FAMILY_POINT_TRIANGLE;
+#define P2C_REF_LAUNCH(F, M)                                                                                     \
+  do {                                                                                                           \
+    hipLaunchKernelGGL((p2c::cert_ref_kernel<F, M>), dim3(grid), dim3(p2c::kRefBlock), 0, s, prims, n, c,        \
+                       p2c::kLightBudget, out, invalid_count);                                                   \
+    hipLaunchKernelGGL((p2c::cert_ref_heavy_kernel<F, M>), dim3(heavy_grid), dim3(p2c::kHeavyBlock), 0, s, prims, \
+                       n, c, out);                                                                               \
+  } while (0)
+  if (mode == p2c::kModeEngine) {
+    if (pt) P2C_REF_LAUNCH(P2C_FAMILY_POINT_TRIANGLE, p2c::kModeEngine);
+    else P2C_REF_LAUNCH(P2C_FAMILY_EDGE_EDGE, p2c::kModeEngine);
+  } else if (mode == p2c::kModeCudaTwin) {
+    if (pt) P2C_REF_LAUNCH(P2C_FAMILY_POINT_TRIANGLE, p2c::kModeCudaTwin);
+    else P2C_REF_LAUNCH(P2C_FAMILY_EDGE_EDGE, p2c::kModeCudaTwin);
   } else {
-    hipLaunchKernelGGL(p2c::cert_ref_kernel<P2C_FAMILY_EDGE_EDGE>, dim3(grid), dim3(p2c::kRefBlock), 0, s,
-                       prims, n, c, p2c::kLightBudget, out, invalid_count);
-    hipLaunchKernelGGL(p2c::cert_ref_heavy_kernel<P2C_FAMILY_EDGE_EDGE>, dim3(heavy_grid),
-                       dim3(p2c::kHeavyBlock), 0, s, prims, n, c, out);
+    if (pt) P2C_REF_LAUNCH(P2C_FAMILY_POINT_TRIANGLE, p2c::kModePython);
+    else P2C_REF_LAUNCH(P2C_FAMILY_EDGE_EDGE, p2c::kModePython);
   }
+#undef P2C_REF_LAUNCH
   P2C_HIP_CHECK(hipGetLastError());
   return P2C_OK;
 }
