@@ -336,7 +336,10 @@ static orc_result eval_twin(Ctx* c, double T0, double T1) {
 int orc_ref_eval_twin_batch(int family, const orc_prim* prims, int64_t n, double t0, double t1,
                             double eps_time, double eps_space, int max_depth, orc_result* out) {
   for (int64_t i = 0; i < n; ++i) {
-    /* same validation as the CPU engine (the twin's host wrapper validates first) */
+    /* The reference's RunCudaBatch (certificate_cuda.cu:456-552) validates
+       neither config, interval nor primitives; the library refuses them as
+       the CPU engine does, a deliberate difference, and this restatement
+       validates the same way so both agree on what is refused. */
     orc_result probe;
     const int rc = orc_ref_eval(family, &prims[i], t0, t1, eps_time, eps_space, 0, &probe);
     if (rc != 0) return rc;

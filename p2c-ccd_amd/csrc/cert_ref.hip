@@ -5,10 +5,12 @@
 // answer and node count as the sequential walk.
 //
 // Replaces the reference's CUDA batch (src/cpp/certificate/certificate_cuda.cu:
-// 286-392, 456-552) but follows the CPU engine's semantics
-// (certificate_engine.cpp:300-386): the reference CUDA twin stops at the first
-// undecided leaf, which can change a collision into "undecided"; this kernel
-// keeps walking as the CPU engine does, so its answers equal the CPU's.
+// 286-392, 456-552).  The walk is a compile-time mode chosen by
+// p2c_cert_config.flags: the CPU engine's (certificate_engine.cpp:300-386,
+// the default: an interval that may not be split is noted and the search goes
+// on), the CUDA twin's (certificate_cuda.cu:286-346: UNDECIDED on the first
+// such interval) or the Python twin's (pure_exact_cpu.py:98-244).  The same
+// walk functions run on the host for p2c_cert_ref_eval_cpu.
 #include <cmath>
 #include <vector>
 
@@ -52,7 +54,7 @@ enum : int { kNodeExpand = 0, kNodeCollision = 1, kNodeSeparated = 2, kNodeUndec
 
 // The node's interval, rebuilt from the root by replaying the midpoints the
 // recursion computes (bit k of `path` = right half taken at depth k).
-__device__ inline void node_interval(const RefCfg& c, uint64_t path, uint32_t depth, double* a, double* b) {
+P2C_HD void node_interval(const RefCfg& c, uint64_t path, uint32_t depth, double* a, double* b) {
   double x = c.t0, y = c.t1;
   for (uint32_t k = 0; k < depth; ++k) {
     const double m = 0.5 * (x + y);
@@ -66,7 +68,7 @@ __device__ inline void node_interval(const RefCfg& c, uint64_t path, uint32_t de
 // ends (da at a, db at b): the endpoint/midpoint contact tests in the
 // reference's order, then the Lipschitz separation test, then the leaf rule.
 template <int kFamily, int kMode>
-__device__ inline int eval_node(const RefPrim& q, const double vn[4], const RefCfg& c, double a, double b,
+P2C_HD int eval_node(const RefPrim& q, const double vn[4], const RefCfg& c, double a, double b,
                                 uint32_t depth, double da, double db, double* dm, double* toi, double* margin) {
   const double mid = 0.5 * (a + b);
   *dm = ref_distance<kFamily>(q, mid, kMode == kModePython);
@@ -114,7 +116,7 @@ struct RefWalk {
 // and midpoint distances; the right sibling after a finished left subtree
 // starts where that subtree's last node ended, so only its end is new.
 template <int kFamily, int kMode>
-__device__ RefWalk walk_subtree(const RefPrim& q, const double vn[4], const RefCfg& c, uint64_t path,
+P2C_HD RefWalk walk_subtree(const RefPrim& q, const double vn[4], const RefCfg& c, uint64_t path,
                                 uint32_t depth, double da, double db, uint32_t budget) {
   RefWalk w;
   w.state = kWalkDone;
@@ -172,7 +174,7 @@ __device__ RefWalk walk_subtree(const RefPrim& q, const double vn[4], const RefC
 }
 
 template <int kFamily>
-__device__ void write_result(const RefCfg& c, int64_t id_a, int64_t id_b, bool collision, bool limited, double a,
+P2C_HD void write_result(const RefCfg& c, int64_t id_a, int64_t id_b, bool collision, bool limited, double a,
                              double b, double toi, bool undecided, double min_margin, uint32_t nodes,
                              uint32_t max_depth, p2c_prim_result* out) {
   p2c_prim_result r;
@@ -199,7 +201,7 @@ __device__ void write_result(const RefCfg& c, int64_t id_a, int64_t id_b, bool c
   *out = r;
 }
 
-__device__ inline void vertex_norms(const RefPrim& q, double vn[4]) {
+P2C_HD void vertex_norms(const RefPrim& q, double vn[4]) {
 #pragma unroll
   for (int k = 0; k < 4; ++k) vn[k] = norm3(sub3(q.p1[k], q.p0[k]));
 }
@@ -469,12 +471,29 @@ __global__ __launch_bounds__(kRefBlock) void cert_ref_distance_kernel(
   }
 }
 
-// ValidateConfig / ValidateInterval (certificate_engine.cpp:224-242), plus the
-// depth bound this stackless walk supports.
-int check_args(int family, double t0, double t1, const p2c_cert_config* cfg, uint32_t* eff_depth) {
+// ValidateConfig / ValidateInterval (certificate_engine.cpp:224-242), the
+// walk flags, and the depth bound of this walk: a node's path is one 64-bit
+// word, so no node may lie deeper than 63 levels.  An interval stops
+// splitting at max_subdivision_depth or once its width b - a is at most
+// eps_time.  With u the spacing of doubles just above max(t0, t1), a split of
+// [a, b] leaves children no wider than (b - a) / 2 + u / 2 (the rounded
+// midpoint), so the widths after k splits are at most (t1 - t0) / 2^k + u and
+// the computed width at most that plus u / 2; when eps_time >= 4u every
+// interval is therefore done by depth d + 1, d the first depth at which the
+// nominal width (t1 - t0) / 2^d reaches eps_time.  Below 4u an interval can
+// stop shrinking (a midpoint that rounds to an end reproduces its parent) and
+// only max_subdivision_depth ends the walk.  Either way the deepest reachable
+// node must be at most 63 deep, or the call is refused; *walk_depth is then
+// min(max_subdivision_depth, 63), which changes no answer.
+int check_args(int family, double t0, double t1, const p2c_cert_config* cfg, uint32_t* walk_depth,
+               int* mode) {
   if (family != P2C_FAMILY_POINT_TRIANGLE && family != P2C_FAMILY_EDGE_EDGE)
     return set_error(P2C_EINVAL, "feature family must be point-triangle (1) or edge-edge (2)");
   if (cfg == nullptr) return set_error(P2C_EINVAL, "CertificateEngineConfig is required");
+  if (cfg->flags & ~(P2C_CERT_CUDA_TWIN | P2C_CERT_PYTHON_TWIN))
+    return set_error(P2C_EINVAL, "unknown certificate flags");
+  if ((cfg->flags & P2C_CERT_CUDA_TWIN) && (cfg->flags & P2C_CERT_PYTHON_TWIN))
+    return set_error(P2C_EINVAL, "choose one certificate walk: CUDA twin or Python twin");
   if (!std::isfinite(cfg->eps_time) || cfg->eps_time <= 0.0)
     return set_error(P2C_EINVAL, "CertificateEngineConfig.eps_time must be finite and positive");
   if (!std::isfinite(cfg->eps_space) || cfg->eps_space <= 0.0)
@@ -483,21 +502,39 @@ int check_args(int family, double t0, double t1, const p2c_cert_config* cfg, uin
     return set_error(P2C_EINVAL, "certificate interval endpoints must be finite");
   if (t0 < 0.0 || t1 > 1.0 || t0 > t1)
     return set_error(P2C_EINVAL, "certificate interval must satisfy 0 <= t0 <= t1 <= 1");
-  // Depth at which the nominal width (t1 - t0) / 2^d first reaches eps_time;
-  // two levels of slack cover midpoint rounding.
-  uint32_t d = 0;
-  double w = t1 - t0;
-  while (w > cfg->eps_time && d < 80) {
-    w *= 0.5;
-    ++d;
+  const double top = std::fmax(std::fabs(t0), std::fabs(t1));
+  const double spacing = std::nextafter(top, HUGE_VAL) - top;
+  uint32_t reach = cfg->max_subdivision_depth;
+  if (cfg->eps_time >= 4.0 * spacing) {
+    uint32_t d = 0;
+    double w = t1 - t0;
+    while (w > cfg->eps_time && d < 80) {
+      w *= 0.5;
+      ++d;
+    }
+    if (d + 2 < reach) reach = d + 2;
   }
-  uint32_t eff = cfg->max_subdivision_depth;
-  if (d + 2 < eff) eff = d + 2;
-  if (eff > 63)
+  if (reach > 63)
     return set_error(P2C_EUNSUPPORTED,
                      "subdivision deeper than 63 levels is not supported by the HIP certificate");
-  *eff_depth = cfg->max_subdivision_depth;
+  *walk_depth = cfg->max_subdivision_depth < 63u ? cfg->max_subdivision_depth : 63u;
+  *mode = (cfg->flags & P2C_CERT_CUDA_TWIN)     ? kModeCudaTwin
+          : (cfg->flags & P2C_CERT_PYTHON_TWIN) ? kModePython
+                                                : kModeEngine;
   return P2C_OK;
+}
+
+// One primitive's certificate on the host: the light pass's walk with no
+// node budget, so the same functions and operation order as the kernels.
+template <int kFamily, int kMode>
+void host_certify(const RefPrim& q, const p2c_prim& p, const RefCfg& c, p2c_prim_result* out) {
+  double vn[4];
+  vertex_norms(q, vn);
+  const double d0 = ref_distance<kFamily>(q, c.t0, kMode == kModePython);
+  const double d1 = ref_distance<kFamily>(q, c.t1, kMode == kModePython);
+  const RefWalk w = walk_subtree<kFamily, kMode>(q, vn, c, 0, 0, d0, d1, 0xFFFFFFFFu);
+  write_result<kFamily>(c, p.id_a, p.id_b, w.state == kWalkCollision, w.limited, w.a, w.b, w.toi, w.undecided,
+                        w.min_margin, w.nodes, w.max_depth, out);
 }
 
 const char* invalid_message(int family, const p2c_prim& p) {
@@ -517,17 +554,13 @@ int p2c_cert_ref_batch(int family, const p2c_prim* prims, int64_t n, double t0, 
                        const p2c_cert_config* cfg, p2c_prim_result* out,
                        uint32_t* invalid_count, void* stream) {
   uint32_t depth = 0;
-  if (int rc = p2c::check_args(family, t0, t1, cfg, &depth)) return rc;
+  int mode = 0;
+  if (int rc = p2c::check_args(family, t0, t1, cfg, &depth, &mode)) return rc;
   if (n < 0) return p2c::set_error(P2C_EINVAL, "primitive count must be non-negative");
   if (n == 0) return P2C_OK;
   if (prims == nullptr || out == nullptr)
     return p2c::set_error(P2C_EINVAL, "primitive and result buffers are required");
-  if ((cfg->flags & P2C_CERT_CUDA_TWIN) && (cfg->flags & P2C_CERT_PYTHON_TWIN))
-    return p2c::set_error(P2C_EINVAL, "choose one certificate walk: CUDA twin or Python twin");
   const p2c::RefCfg c{t0, t1, cfg->eps_time, cfg->eps_space, depth};
-  const int mode = (cfg->flags & P2C_CERT_CUDA_TWIN)     ? p2c::kModeCudaTwin
-                   : (cfg->flags & P2C_CERT_PYTHON_TWIN) ? p2c::kModePython
-                                                         : p2c::kModeEngine;
   const unsigned grid = p2c::grid_for(n, p2c::kRefBlock);
   const unsigned heavy_grid = p2c::grid_for(n, p2c::kHeavyChunk);
   hipStream_t s = p2c::as_stream(stream);
@@ -574,7 +607,8 @@ int p2c_cert_ref_distance(int family, const p2c_prim* prims, const double* t, in
 int p2c_cert_ref_batch_host(int family, const p2c_prim* prims, int64_t n, double t0, double t1,
                             const p2c_cert_config* cfg, p2c_prim_result* out) {
   uint32_t depth = 0;
-  if (int rc = p2c::check_args(family, t0, t1, cfg, &depth)) return rc;
+  int mode = 0;
+  if (int rc = p2c::check_args(family, t0, t1, cfg, &depth, &mode)) return rc;
   if (n < 0) return p2c::set_error(P2C_EINVAL, "primitive count must be non-negative");
   if (n == 0) return P2C_OK;
   // Host-side validation first: the reference's CPU path rejects the batch with
@@ -609,17 +643,24 @@ int p2c_cert_ref_batch_host(int family, const p2c_prim* prims, int64_t n, double
 int p2c_cert_ref_eval_cpu(int family, const p2c_prim* prim, double t0, double t1,
                           const p2c_cert_config* cfg, p2c_prim_result* out) {
   uint32_t depth = 0;
-  if (int rc = p2c::check_args(family, t0, t1, cfg, &depth)) return rc;
+  int mode = 0;
+  if (int rc = p2c::check_args(family, t0, t1, cfg, &depth, &mode)) return rc;
   if (prim == nullptr || out == nullptr)
     return p2c::set_error(P2C_EINVAL, "primitive and result pointers are required");
   if (!p2c::prim_valid(*prim)) return p2c::set_error(P2C_EINVAL, p2c::invalid_message(family, *prim));
+  const p2c::RefCfg c{t0, t1, cfg->eps_time, cfg->eps_space, depth};
   const p2c::RefPrim q = p2c::load_prim(*prim);
-  if (family == P2C_FAMILY_POINT_TRIANGLE)
-    p2c::ref_certify<P2C_FAMILY_POINT_TRIANGLE>(q, prim->id_a, prim->id_b, t0, t1, cfg->eps_time,
-                                                cfg->eps_space, depth, out);
-  else
-    p2c::ref_certify<P2C_FAMILY_EDGE_EDGE>(q, prim->id_a, prim->id_b, t0, t1, cfg->eps_time,
-                                           cfg->eps_space, depth, out);
+  if (family == P2C_FAMILY_POINT_TRIANGLE) {
+    if (mode == p2c::kModeEngine) p2c::host_certify<P2C_FAMILY_POINT_TRIANGLE, p2c::kModeEngine>(q, *prim, c, out);
+    else if (mode == p2c::kModeCudaTwin)
+      p2c::host_certify<P2C_FAMILY_POINT_TRIANGLE, p2c::kModeCudaTwin>(q, *prim, c, out);
+    else p2c::host_certify<P2C_FAMILY_POINT_TRIANGLE, p2c::kModePython>(q, *prim, c, out);
+  } else {
+    if (mode == p2c::kModeEngine) p2c::host_certify<P2C_FAMILY_EDGE_EDGE, p2c::kModeEngine>(q, *prim, c, out);
+    else if (mode == p2c::kModeCudaTwin)
+      p2c::host_certify<P2C_FAMILY_EDGE_EDGE, p2c::kModeCudaTwin>(q, *prim, c, out);
+    else p2c::host_certify<P2C_FAMILY_EDGE_EDGE, p2c::kModePython>(q, *prim, c, out);
+  }
   return P2C_OK;
 }
 

@@ -226,89 +226,4 @@ P2C_HD void init_result(p2c_prim_result* r, int status, uint32_t family, double 
   r->nodes = 0;
 }
 
-// The interval certificate over [T0, T1] as a stackless depth-first walk.
-//
-// A node is (depth, path): bit k of `path` is the branch taken at depth k
-// (0 = left half, 1 = right half).  Its interval is rebuilt from the root by
-// replaying the same midpoints the recursion computes, so no per-thread stack
-// is kept.  Pre-order, left before right, stop at the first collision: the
-// same visiting order and the same answer as the recursive definition
-// (certificate_engine.cpp:300-386).  Separation leaves contribute min margin;
-// any undecided leaf without a collision makes the root undecided.
-// Requires max depth <= 62 (checked by the caller).
-template <int kFamily>
-P2C_HD void ref_certify(const RefPrim& q, int64_t id_a, int64_t id_b, double T0, double T1,
-                        double eps_time, double eps_space, uint32_t max_depth,
-                        p2c_prim_result* out) {
-  uint64_t path = 0;
-  uint32_t depth = 0;
-  double min_margin = 0.0;
-  bool have_margin = false;
-  bool undecided = false;
-  uint32_t nodes = 0;
-  for (;;) {
-    double a = T0, b = T1;
-    for (uint32_t k = 0; k < depth; ++k) {
-      const double m = 0.5 * (a + b);
-      if ((path >> k) & 1ull) a = m; else b = m;
-    }
-    ++nodes;
-    const double mid = 0.5 * (a + b);
-    const double d0 = ref_distance<kFamily>(q, a);
-    const double dm = ref_distance<kFamily>(q, mid);
-    const double d1 = ref_distance<kFamily>(q, b);
-    double toi = -1.0;
-    if (d0 <= eps_space) toi = a;
-    else if (dm <= eps_space) toi = mid;
-    else if (d1 <= eps_space) toi = b;
-    if (toi >= 0.0) {
-      init_result(out, P2C_STATUS_COLLISION, kFamily, a, b);
-      out->toi_upper = toi;
-      out->witness_family = (uint8_t)kFamily;
-      out->witness_id_a = id_a;
-      out->witness_id_b = id_b;
-      out->nodes = nodes;
-      return;
-    }
-    const double lb = dm - ref_motion_radius<kFamily>(q, a, b);
-    bool leaf = true;
-    if (lb > eps_space) {
-      const double margin = std_max(0.0, lb - eps_space);
-      if (depth == 0) {
-        init_result(out, P2C_STATUS_SEPARATION, kFamily, a, b);
-        out->safe_margin_lb = margin;
-        out->nodes = nodes;
-        return;
-      }
-      min_margin = have_margin ? std_min(min_margin, margin) : margin;
-      have_margin = true;
-    } else if (depth >= max_depth || b - a <= eps_time) {
-      undecided = true;
-      if (depth == 0) break;
-    } else {
-      leaf = false;
-    }
-    if (!leaf) {
-      ++depth;  // descend into the left half (its path bit is already 0)
-      continue;
-    }
-    // Advance to the next node in pre-order: climb while we are a right child.
-    while (depth > 0 && ((path >> (depth - 1)) & 1ull)) {
-      path &= ~(1ull << (depth - 1));
-      --depth;
-    }
-    if (depth == 0) break;
-    path |= 1ull << (depth - 1);  // right sibling
-  }
-  if (undecided) {
-    init_result(out, P2C_STATUS_UNDECIDED, kFamily, T0, T1);
-    out->reason_code = 2;            // kCertificateReasonMaxSubdivisionDepth
-    out->next_refinement_mode = 1;   // kBisectInterval
-  } else {
-    init_result(out, P2C_STATUS_SEPARATION, kFamily, T0, T1);
-    out->safe_margin_lb = min_margin;
-  }
-  out->nodes = nodes;
-}
-
 }  // namespace p2c

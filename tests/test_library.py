@@ -60,6 +60,93 @@ def test_c_abi_rejects_bad_config_with_reference_text():
         _native.cert_ref_eval_cpu(2, bad, 0.0, 1.0, _native.cert_config())
 
 
+def _sliding_graze(h_ulps: int, u: float) -> np.ndarray:
+    """A point descending onto the plane of a triangle that slides under it,
+    ending h_ulps doubles above eps_space = 1e-6 at t = 1: the motion radius
+    keeps the last interval [t, 1] undecided down to one double spacing, where
+    eps_time 1e-17 can no longer end the walk and only the depth cap does."""
+    h = 1e-6
+    for _ in range(h_ulps):
+        h = np.nextafter(h, 1.0)
+    x = np.zeros((1, 8, 3))
+    tri = np.array([(0.0, 0.0, 0.0), (1.0, 0.0, 0.0), (0.0, 1.0, 0.0)])
+    x[0, 0], x[0, 4] = (0.25, 0.25, h + 0.1), (0.25, 0.25, h)
+    x[0, 1:4], x[0, 5:8] = tri, tri - (u, 0.0, 0.0)
+    return x
+
+
+def test_walk_depth_guard_refuses_walks_past_63_levels():
+    # eps_time below the spacing of doubles near t = 1: an interval of one
+    # spacing splits into a copy of itself, so max_subdivision_depth alone
+    # bounds the walk; past 63 levels the path word would overflow.
+    x = _sliding_graze(1, 0.1)
+    p = _native.prims_from_positions(x)
+    for depth in (64, 65, 200):
+        cfg = _native.cert_config(1e-17, 1e-6, depth)
+        with pytest.raises(_native.NativeError, match="deeper than 63 levels") as e:
+            _native.cert_ref_eval_cpu(1, p, 0.0, 1.0, cfg)
+        assert e.value.code == -4
+        out = np.zeros(1, _native.PRIM_RESULT_DTYPE)
+        with pytest.raises(_native.NativeError, match="deeper than 63 levels"):  # refused before any device call
+            _native.call("p2c_cert_ref_batch_host", 1, _native.ptr(p), 1, 0.0, 1.0, _native.ptr(cfg),
+                         _native.ptr(out))
+    # a large cap with eps_time well above the spacing: widths end the walk
+    # long before 63 levels, so it is accepted and agrees with the oracle
+    qs = workloads.make_query_set(60, 60, seed=5)
+    for fam, xs in ((1, qs.vf), (2, qs.ee)):
+        mine = _native.cert_ref_eval_cpu(fam, _native.prims_from_positions(xs), 0.0, 1.0,
+                                         _native.cert_config(1e-6, 1e-6, 500))
+        ref = oracle.ref_cert(fam, oracle.prims_from_positions(xs), eps_time=1e-6, max_depth=500)
+        assert np.array_equal(mine["status"], ref["status"]) and np.array_equal(mine["nodes"], ref["nodes"])
+
+
+@pytest.mark.parametrize("u,nodes", [(0.1, 127), (0.3, 231)])
+def test_walk_reaches_the_63_level_cap_like_the_reference(u, nodes):
+    # 63 levels is the deepest walk allowed: the copy chain ends there,
+    # UNDECIDED, as the reference engine ends it (2 * 63 + 1 nodes for u 0.1).
+    x = _sliding_graze(1, u)
+    op = oracle.prims_from_positions(x)
+    for flags, ref in ((0, oracle.ref_cert(1, op, eps_time=1e-17, max_depth=63)),
+                       (1, oracle.ref_cert_twin(1, op, eps_time=1e-17, max_depth=63))):
+        cfg = _native.cert_config(1e-17, 1e-6, 63)
+        cfg["flags"] = flags
+        mine = _native.cert_ref_eval_cpu(1, _native.prims_from_positions(x), 0.0, 1.0, cfg)
+        assert mine["status"][0] == 2 and mine["nodes"][0] == ref["nodes"][0]
+        for f in ("interval_t0", "interval_t1", "toi_upper", "reason_code"):
+            assert mine[f][0] == ref[f][0], (flags, f)
+    assert oracle.ref_cert(1, op, eps_time=1e-17, max_depth=63)["nodes"][0] == nodes
+    if oracle.ref_available():
+        r = oracle.ref_cert_compiled(1, op, eps_time=1e-17, max_depth=63)
+        assert r["status"][0] == 2 and r["toi_upper"][0] == 1.0
+
+
+def test_host_certificate_follows_every_walk_flag():
+    # flags select the walk on the host entry as on the kernels: the CUDA
+    # twin's (UNDECIDED on the first unsplittable interval) and the Python
+    # twin's, each against its oracle restatement.
+    qs = workloads.make_query_set(500, 500, seed=2024)
+    for fam, x in ((1, qs.vf), (2, qs.ee)):
+        p, op = _native.prims_from_positions(x), oracle.prims_from_positions(x)
+        mine = _native.cert_ref_eval_cpu(fam, p, 0.0, 1.0, _native.cert_config(cuda_twin=True))
+        ref = oracle.ref_cert_twin(fam, op)
+        for f in ("status", "toi_upper", "safe_margin_lb", "interval_t0", "interval_t1", "nodes"):
+            assert np.array_equal(mine[f].astype(np.float64), ref[f].astype(np.float64)), (fam, f)
+        cfg = _native.cert_config(1e-5, 1e-8, 32)
+        cfg["flags"] = 2
+        mine = _native.cert_ref_eval_cpu(fam, p, 0.0, 1.0, cfg)
+        ref = oracle.ref_cert_py(fam, op)
+        for f in ("status", "toi_upper", "safe_margin_lb", "nodes"):
+            assert np.array_equal(mine[f].astype(np.float64), ref[f].astype(np.float64)), (fam, f)
+        assert np.array_equal(mine["pad"][:, 0].astype(np.int64), ref["reason_code"].astype(np.int64))
+    p = _native.prims_from_positions(qs.vf[:1])
+    for flags, msg in ((3, "choose one certificate walk"), (4, "unknown certificate flags"),
+                       (0x80000000, "unknown certificate flags")):
+        cfg = _native.cert_config()
+        cfg["flags"] = flags
+        with pytest.raises(_native.NativeError, match=msg):
+            _native.cert_ref_eval_cpu(1, p, 0.0, 1.0, cfg)
+
+
 # ---- p2cccd_cpp surface (host rules only) ---------------------------------
 cpp = pytest.importorskip("p2cccd_cpp")
 
