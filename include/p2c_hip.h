@@ -246,6 +246,23 @@ int p2c_tuv_replay(const void* pairs, const uint32_t* replay_list, const uint32_
                    uint32_t replay_cap, double tolerance, int64_t max_itr, void* heaps, int64_t heap_cap,
                    int64_t slots, p2c_tuv_result* out, void* stream);
 
+/* Allocation-free certificate.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  It certifies n
+ * pairs set up in `pairs` (a caller buffer of p2c_tuv_pairs_bytes(n) bytes)
+ * exactly as p2c_tuv_stream (light_budget 1024, refill_at 8, handover_at 8,
+ * 3 blocks of 256 per CU) + p2c_tuv_wave + p2c_tuv_replay (heap_cap 65,536,
+ * 8 slots) do, writing one result per pair to `out`, but keeps every scratch
+ * buffer (counters, saved walks, heavy and replay lists, replay heaps) in one
+ * static device array of about 30 MB per GPU, held until the process ends;
+ * it calls no hipMalloc.  n is at most p2c_tuv_static_max_pairs() (262,144);
+ * larger n is refused with P2C_EINVAL.  All calls on one GPU share that
+ * array: calls that overlap on one GPU (two streams, two threads) race and
+ * must be serialised by the caller. */
+size_t p2c_tuv_static_max_pairs(void);
+int p2c_tuv_certify_static(const void* pairs, int64_t n, double tolerance, int64_t max_itr, p2c_tuv_result* out,
+                           void* stream);
+
 /* ---- swept-AABB LBVH broad phase ----------------------------------------
  * Replaces the OptiX candidate emitter (optix_candidate_tracer.cpp:652-889,
  * optix_candidate_program.cu:66-130) with the candidate-set semantics of the

@@ -17,6 +17,7 @@
 // reference (src/tools/tight_inclusion_full_query_benchmark.cpp:375-397;
 // src/tools/scene_object_envelope_native_ti_walltime.cpp:704-754).
 #include <cmath>
+#include <mutex>
 
 #include "cert_tuv_math.h"
 #include "p2c_hip.h"
@@ -1064,6 +1065,67 @@ int check_tuv(double ms, double tol, double t_max, int64_t max_itr) {
   return P2C_OK;
 }
 
+// ---- allocation-free certificate ----------------------------------------
+// The stream -> wave -> replay sequence for callers that may not allocate:
+// every scratch buffer it needs lives in one file-scope device array.
+constexpr int64_t kStaticMaxPairs = int64_t(1) << 18;  // pairs per call
+constexpr int64_t kStaticHeapCap = int64_t(1) << 16;   // replay heap entries per slot
+constexpr int64_t kStaticSlots = 8;                     // pairs replayed concurrently
+constexpr int64_t kStaticLightBudget = 1024;
+constexpr int kStaticRefillAt = 8, kStaticHandoverAt = 8, kStaticBlocksPerCu = 3;
+constexpr size_t kScratchCountersOff = 0;
+constexpr size_t kScratchStatesOff = 256;
+constexpr size_t kScratchHeavyOff = kScratchStatesOff + sizeof(WalkState) * size_t(kStaticMaxPairs);
+constexpr size_t kScratchReplayOff = kScratchHeavyOff + sizeof(uint32_t) * size_t(kStaticMaxPairs);
+constexpr size_t kScratchHeapsOff = kScratchReplayOff + sizeof(uint32_t) * size_t(kStaticMaxPairs);
+constexpr size_t kScratchBytes = kScratchHeapsOff + sizeof(HeapBox) * size_t(kStaticHeapCap * kStaticSlots);
+static_assert(kScratchHeavyOff % 256 == 0 && kScratchReplayOff % 256 == 0 && kScratchHeapsOff % 256 == 0,
+              "scratch sections stay 256-byte aligned");
+
+__global__ void zero_words_kernel(uint64_t* p, int n) {
+  const int i = int(threadIdx.x);
+  if (i < n) p[i] = 0;
+}
+
+}  // namespace
+}  // namespace p2c
+
+// One copy per device, held from module load until the process ends.  Host
+// code never passes this name to a kernel: the device address comes from
+// hipGetSymbolAddress, once per device (scratch_on_current_device).
+static __device__ __align__(256) unsigned char scratch[p2c::kScratchBytes];
+
+namespace p2c {
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices];
+int g_scratch_grid[kMaxDevices];
+
+// The current device's copy of `scratch` and the certificate grid for it,
+// looked up on the first call on that device with the device current.
+int scratch_on_current_device(unsigned char** ptr, int* grid_blocks) {
+  int dev = 0;
+  P2C_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return set_error(P2C_EUNSUPPORTED, "device ordinal beyond 63");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    if (p == nullptr) return set_error(P2C_EHIP, "hipGetSymbolAddress(scratch) returned no address");
+    int cus = 0;
+    P2C_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    if (cus <= 0) return set_error(P2C_EHIP, "device reports no compute units");
+    g_scratch_grid[dev] = cus * kStaticBlocksPerCu;
+    g_scratch_ptr[dev] = p;
+  }
+  *ptr = static_cast<unsigned char*>(g_scratch_ptr[dev]);
+  *grid_blocks = g_scratch_grid[dev];
+  return P2C_OK;
+}
+
 }  // namespace
 }  // namespace p2c
 
@@ -1266,6 +1328,39 @@ int p2c_tuv_replay(const void* pairs, const uint32_t* replay_list, const uint32_
                      replay_cap, tolerance, max_itr, static_cast<p2c::HeapBox*>(heaps), heap_cap, slots, out);
   P2C_HIP_CHECK(hipGetLastError());
   return P2C_OK;
+}
+
+size_t p2c_tuv_static_max_pairs(void) { return size_t(p2c::kStaticMaxPairs); }
+
+int p2c_tuv_certify_static(const void* pairs, int64_t n, double tolerance, int64_t max_itr, p2c_tuv_result* out,
+                           void* stream) {
+  if (int rc = p2c::check_tuv(0.0, tolerance, 1.0, max_itr)) return rc;
+  if (n < 0) return p2c::set_error(P2C_EINVAL, "pair count must be non-negative");
+  if (n > p2c::kStaticMaxPairs)
+    return p2c::set_error(P2C_EINVAL, "p2c_tuv_certify_static takes at most 262144 pairs per call");
+  if (n == 0) return P2C_OK;
+  if (pairs == nullptr || out == nullptr) return p2c::set_error(P2C_EINVAL, "pair and result buffers are required");
+  unsigned char* base = nullptr;
+  int grid = 0;
+  if (int rc = p2c::scratch_on_current_device(&base, &grid)) return rc;
+  void* counters = base + p2c::kScratchCountersOff;
+  void* states = base + p2c::kScratchStatesOff;
+  auto* heavy = reinterpret_cast<uint32_t*>(base + p2c::kScratchHeavyOff);
+  auto* replay = reinterpret_cast<uint32_t*>(base + p2c::kScratchReplayOff);
+  void* heaps = base + p2c::kScratchHeapsOff;
+  const uint32_t cap = uint32_t(p2c::kStaticMaxPairs);
+  // the previous call's counters are still there: clear them with a kernel
+  // (no memset node, so the call may also be captured in a graph)
+  hipLaunchKernelGGL(p2c::zero_words_kernel, dim3(1), dim3(64), 0, p2c::as_stream(stream),
+                     static_cast<uint64_t*>(counters), 8);
+  P2C_HIP_CHECK(hipGetLastError());
+  if (int rc = p2c_tuv_stream(pairs, n, nullptr, tolerance, max_itr, p2c::kStaticLightBudget, p2c::kStaticRefillAt,
+                              p2c::kStaticHandoverAt, out, counters, states, heavy, cap, grid, stream))
+    return rc;
+  if (int rc = p2c_tuv_wave(pairs, counters, states, heavy, cap, tolerance, max_itr, out, replay, grid, stream))
+    return rc;
+  return p2c_tuv_replay(pairs, replay, static_cast<const uint32_t*>(counters) + 2, cap, tolerance, max_itr, heaps,
+                        p2c::kStaticHeapCap, p2c::kStaticSlots, out, stream);
 }
 
 }  // extern "C"

@@ -47,6 +47,8 @@ nat._sig("p2c_tuv_rounds", [_vp, _i64, _vp, _f64, _i64, _vp, _i32, _vp, _vp, _vp
 nat.lib.p2c_tuv_state_bytes.restype = nat.ctypes.c_size_t
 nat._sig("p2c_tuv_replay", [_vp, _vp, _vp, _u32, _f64, _i64, _vp, _i64, _i64, _vp, _vp])
 nat.lib.p2c_tuv_pair_bytes.restype = nat.ctypes.c_size_t
+nat._sig("p2c_tuv_certify_static", [_vp, _i64, _f64, _i64, _vp, _vp])
+nat.lib.p2c_tuv_static_max_pairs.restype = nat.ctypes.c_size_t
 nat.lib.p2c_tuv_pairs_bytes.argtypes = [nat._i64]
 nat.lib.p2c_tuv_pairs_bytes.restype = nat.ctypes.c_size_t
 assert nat.lib.p2c_tuv_pair_bytes() == PAIR_BYTES
