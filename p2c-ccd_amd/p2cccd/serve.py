@@ -416,7 +416,7 @@ def make_scene(sid: int, triangles: int, seed: int) -> scenes.Scene:
         v2 = v.copy() + np.array([2.02, 0.0, 0.0])
         kind = "spheres"
     else:
-        n = max(int(np.sqrt(per / 2.0)) + 1, 3)
+        n = max(int(np.ceil(np.sqrt(per / 2.0))) + 1, 3)
         v, f = _grid_mesh(n, 2.0)
         v2 = v.copy() + np.array([0.0, 0.0, 0.02])
         kind = "grids"

@@ -104,7 +104,14 @@ def collect(n_variants: int = 50, base_seed: int = workloads.BASE_SEED + 1000):
 def train(X: torch.Tensor, y: torch.Tensor, epochs: int = 300, lr: float = 2e-3, seed: int = 0):
     torch.manual_seed(seed)
     mu = X.mean(0)
-    sd = X.std(0).clamp(min=1e-6)
+    sd = X.std(0)
+    # a feature that is constant over the training cells (the cell-count
+    # column of ones, for one) normalises to 0 and carries nothing; dividing
+    # by a floored std instead would fold into a huge first-layer weight and
+    # an opposite huge bias, whose FP32 cancellation at inference would cost
+    # more than the features themselves
+    const = sd <= 1e-6 * X.abs().mean(0).clamp(min=1.0)
+    sd = torch.where(const, torch.ones_like(sd), sd)
     Xn = (X - mu) / sd
     model = stpf.StpfPriority().to(X.device)
     opt = torch.optim.Adam(model.parameters(), lr=lr)
@@ -119,11 +126,15 @@ def train(X: torch.Tensor, y: torch.Tensor, epochs: int = 300, lr: float = 2e-3,
             loss = lossf(model.logits(Xn[j]), y[j])
             loss.backward()
             opt.step()
-    # fold the input normalisation into the first layer so the kernel takes raw features
+    # fold the input normalisation into the first layer (in FP64) so the kernel takes raw features;
+    # constant features get zero weight, which is what their normalised value 0 contributed
     with torch.no_grad():
-        W = model.l1.weight / sd
-        model.l1.bias -= (model.l1.weight * (mu / sd)).sum(1)
-        model.l1.weight.copy_(W)
+        w = model.l1.weight.double()
+        w[:, const] = 0.0
+        W = w / sd.double()
+        b = model.l1.bias.double() - (w * (mu.double() / sd.double())).sum(1)
+        model.l1.weight.copy_(W.float())
+        model.l1.bias.copy_(b.float())
     return model, float(loss.detach())
 
 
