@@ -394,6 +394,45 @@ int p2c_proxy_compact_host(const p2c_proxy_set* host_set, const p2c_raw_hit* raw
                            p2c_candidate** cand, int64_t* n_cand);
 void p2c_host_free(void* p);
 
+/* ---- external query batches ------------------------------------------------
+ * Replaces GenerateCandidatesForExternalBatch's proxy trace and compaction
+ * (src/cpp/rt_candidate/external_batch_candidate.cpp:231-339).  One record
+ * per query of the batch (ExternalBatchQuery, external_batch_candidate.h:
+ * 20-31), with what the host resolves in the reference's sequential order:
+ * the runtime query id, the x shift of the query's slab in the batched proxy
+ * scene, and the box pair (1, 2 when the query has none).  224 bytes. */
+typedef struct p2c_external_query {
+  double vertices_t0[4][3];
+  double vertices_t1[4][3];
+  double x_offset;
+  uint64_t runtime_query_id;
+  uint32_t source_query_index;
+  uint32_t family; /* 0 vertex-face (points {0} vs {1,2,3}), 1 edge-edge ({0,1} vs {2,3}) */
+  uint32_t object_a_id;
+  uint32_t object_b_id;
+} p2c_external_query;
+
+size_t p2c_external_batch_workspace_bytes(int64_t n);
+/* Candidates of n queries on the device: query order[k] (device u32, the
+ * queries by ascending runtime id) is the k-th visited; a query whose two
+ * shifted swept boxes overlap (inclusive FP64 test) yields one candidate,
+ * written in visiting order with candidate_id = runtime_query_id * 1000003 +
+ * ordinal + 1 (the reference's MakeCandidateId over its sorted list).  status
+ * (device u64[4]): [0] candidate count; [1] the first query (input index)
+ * whose shifted boxes are not finite, [2] the first query whose merged motion
+ * bound is not a finite float, [3] the first candidate position whose id
+ * wraps to 0, each UINT64_MAX when there is none (ValidateProxyScene,
+ * ValidateRawHit and ValidateCandidateRecord failures, in that precedence).
+ * Candidates at or beyond `cap` are dropped. */
+int p2c_external_batch_candidates(const p2c_external_query* queries, const uint32_t* order, int64_t n,
+                                  void* workspace, size_t workspace_bytes, p2c_candidate* out, int64_t cap,
+                                  uint64_t* status, void* stream);
+/* Host-buffer form (the reference's host vectors): copies in, runs, copies
+ * status[4] and the candidates back (malloc'ed, free with p2c_host_free);
+ * trace_ms (optional) is the device time. */
+int p2c_external_batch_host(const p2c_external_query* queries, const uint32_t* order, int64_t n,
+                            p2c_candidate** cand, int64_t* n_cand, uint64_t* status, double* trace_ms);
+
 /* ---- scheduling scorers ---------------------------------------------------
  * Scores only order certificate work; exact certificates decide every pair.
  *

@@ -453,3 +453,135 @@ def ref_dense_group(inputs, cfg: dict):
     if rc != 0:
         raise ValueError(err.value.decode())
     return su, sd
+
+
+# ---- external query batches and proxy-scene construction ---------------------
+EXT_QUERY = np.dtype({"names": ["source_query_id", "source_query_index", "family", "v0", "v1", "has_ground_truth",
+                                "ground_truth_collides", "has_box_pair", "box_pair"],
+                      "formats": ["<u8", "<u4", "<u4", ("<f8", (4, 3)), ("<f8", (4, 3)), "u1", "u1", "u1",
+                                  ("<u4", (2,))],
+                      "offsets": [0, 8, 12, 16, 112, 208, 209, 210, 212], "itemsize": 224})
+EXT_STATS = np.dtype([("primitive_count", "<u8"), ("raw_hit_count", "<u8"), ("compact_candidate_count", "<u8"),
+                      ("candidate_recall", "<f8")])
+BUILD_OBJECT = np.dtype({"names": ["object_id", "proxy_type", "slabs_per_motion_segment", "n_patches", "n_segments",
+                                   "patch0", "segment0", "eps_proxy"],
+                         "formats": ["<u4"] * 7 + ["<f8"], "offsets": [0, 4, 8, 12, 16, 20, 24, 32], "itemsize": 40})
+BUILD_PATCH = np.dtype({"names": ["patch_id", "local_center", "radius"], "formats": ["<u4", ("<f8", (3,)), "<f8"],
+                        "offsets": [0, 8, 32], "itemsize": 40})
+BUILD_SEGMENT = np.dtype([("t0", "<f8"), ("t1", "<f8"), ("translation0", "<f8", (3,)), ("rotation0", "<f8", (4,)),
+                          ("translation1", "<f8", (3,)), ("rotation1", "<f8", (4,))])
+PRIM_DOUBLES = 36  # one built proxy primitive, see ref_external_shim.cpp
+
+
+def ref_external_batch(queries: np.ndarray):
+    """GenerateCandidatesForExternalBatch of the compiled reference:
+    (candidates CANDIDATE[k], runtime ids u64[n], stats EXT_STATS) or ValueError."""
+    L = ref_lib()
+    q = np.ascontiguousarray(queries, dtype=EXT_QUERY)
+    n = len(q)
+    cand = np.zeros(max(n, 1), CANDIDATE)
+    rid = np.zeros(max(n, 1), np.uint64)
+    st = np.zeros(1, EXT_STATS)
+    nc = ctypes.c_int64()
+    err = ctypes.create_string_buffer(256)
+    vp = ctypes.c_void_p
+    rc = L.ref_external_batch(ctypes.c_int64(n), vp(q.ctypes.data), ctypes.c_int64(len(cand)), vp(cand.ctypes.data),
+                              ctypes.byref(nc), vp(rid.ctypes.data), vp(st.ctypes.data), err, 256)
+    if rc != 0:
+        raise ValueError(err.value.decode())
+    return cand[: nc.value], rid[:n], st[0]
+
+
+def ref_build_proxy_scene(query_id: int, objects: np.ndarray, patches: np.ndarray, segments: np.ndarray):
+    """BuildProxyScene of the compiled reference: float64[n, 36] (the field
+    order of ref_external_shim.cpp) or ValueError."""
+    L = ref_lib()
+    o = np.ascontiguousarray(objects, dtype=BUILD_OBJECT)
+    p = np.ascontiguousarray(patches, dtype=BUILD_PATCH)
+    s = np.ascontiguousarray(segments, dtype=BUILD_SEGMENT)
+    cap = 4096
+    out = np.zeros((cap, PRIM_DOUBLES))
+    n = ctypes.c_int64()
+    err = ctypes.create_string_buffer(256)
+    vp = ctypes.c_void_p
+    rc = L.ref_build_proxy_scene(ctypes.c_uint64(query_id), ctypes.c_int(len(o)), vp(o.ctypes.data),
+                                 vp(p.ctypes.data), vp(s.ctypes.data), ctypes.c_int64(cap), vp(out.ctypes.data),
+                                 ctypes.byref(n), err, 256)
+    if rc != 0:
+        raise ValueError(err.value.decode())
+    return out[: n.value]
+
+
+def external_batch(queries: np.ndarray):
+    """Restatement of GenerateCandidatesForExternalBatch
+    (external_batch_candidate.cpp:128-339) taken literally: the batched proxy
+    scene (two swept-AABB proxies per query, one slab [2q, 2q + 1] each,
+    shifted along x by the running extent), every proxy pair i < j tested as
+    GenerateRawCandidatesCpu does (candidate_buffer.cpp:144-181), then the
+    translation, the sort by (query_id, patch_a_id, patch_b_id) and
+    MakeCandidateId.  O(n^2): small batches only.  Returns (candidates as a
+    list of dicts, runtime ids)."""
+    with np.errstate(over="ignore"):  # overflow is one of the reference's error cases
+        return _external_batch(np.asarray(queries, dtype=EXT_QUERY))
+
+
+def _external_batch(q: np.ndarray):
+    n = len(q)
+    seen, taken, rids = set(), set(), []
+    cursor = 0.0
+    prox = []  # (query index, object, slab, t0, t1, lo[3], hi[3])
+    for i in range(n):
+        if not np.isfinite(q["v0"][i]).all():
+            raise ValueError("external batch query vertices_t0 must be finite")
+        if not np.isfinite(q["v1"][i]).all():
+            raise ValueError("external batch query vertices_t1 must be finite")
+        sid = int(q["source_query_id"][i])
+        if sid in seen:
+            raise ValueError("external batch source_query_id values must be unique")
+        seen.add(sid)
+        rid = sid if sid > 0 else i + 1
+        while rid in taken:
+            rid += 1
+        taken.add(rid)
+        rids.append(rid)
+        sides = ([0], [1, 2, 3]) if q["family"][i] == 0 else ([0, 1], [2, 3])
+        boxes = []
+        for idx in sides:
+            pts = np.concatenate([q["v0"][i][idx], q["v1"][i][idx]])
+            boxes.append((pts.min(0), pts.max(0)))
+        lo_x = min(boxes[0][0][0], boxes[1][0][0])
+        hi_x = max(boxes[0][1][0], boxes[1][1][0])
+        off = cursor - lo_x
+        t0 = float(i) * 2.0
+        for obj, (lo, hi) in zip((1, 2), boxes):
+            lo, hi = lo.copy(), hi.copy()
+            lo[0] += off
+            hi[0] += off
+            if not (np.isfinite(lo).all() and np.isfinite(hi).all()):
+                raise ValueError("proxy bounds contain a non-finite value")
+            prox.append((i, obj, i, t0, t0 + 1.0, lo, hi))
+        cursor += (hi_x - lo_x) + 1.0e-2
+    hits = []
+    for a in range(len(prox)):
+        for b in range(a + 1, len(prox)):
+            A, B = prox[a], prox[b]
+            if A[1] == B[1] or A[2] != B[2] or not (A[3] < B[4] and B[3] < A[4]):
+                continue
+            if (A[6] < B[5]).any() or (B[6] < A[5]).any():
+                continue
+            rad = [0.5 * max(P[6] - P[5]) for P in (A, B)]
+            r = np.float32(max(rad))
+            if not np.isfinite(r) or r < 0:
+                raise ValueError("raw candidate motion_bound must be finite and non-negative")
+            hits.append((A[0], r))
+    out = []
+    for i, r in hits:
+        out.append(dict(query_id=rids[i], slab_id=i,
+                        object_a_id=int(q["box_pair"][i][0]) if q["has_box_pair"][i] else 1,
+                        patch_a_id=0 if q["family"][i] == 0 else 1,
+                        object_b_id=int(q["box_pair"][i][1]) if q["has_box_pair"][i] else 2,
+                        patch_b_id=(int(q["source_query_index"][i]) + 1) & 0xFFFFFFFF, motion_bound=r))
+    out.sort(key=lambda c: (c["query_id"], c["patch_a_id"], c["patch_b_id"]))
+    for k, c in enumerate(out):
+        c["candidate_id"] = (c["query_id"] * 1_000_003 + k + 1) & 0xFFFFFFFFFFFFFFFF
+    return out, np.array(rids, dtype=np.uint64)

@@ -146,6 +146,37 @@ int orc_proxy_generate(uint64_t query_id, const orc_proxy_set* set, int64_t raw_
                        int64_t* n_raw, int64_t cand_cap, orc_candidate* cand, int64_t* n_cand,
                        orc_density* density);
 
+/* external query batches and proxy-scene construction (reference shim
+   ref_external_shim.cpp; the restatement of the external batch rule is
+   oracle.external_batch in __init__.py) */
+typedef struct {
+  uint64_t source_query_id;
+  uint32_t source_query_index, family; /* 0 vf, 1 ee */
+  double v0[4][3], v1[4][3];
+  uint8_t has_ground_truth, ground_truth_collides, has_box_pair, pad;
+  uint32_t box_pair[2];
+} orc_ext_query;
+
+typedef struct {
+  uint64_t primitive_count, raw_hit_count, compact_candidate_count;
+  double candidate_recall;
+} orc_ext_stats;
+
+/* one rigid object of a proxy-scene build; patches and segments are indexed
+   into shared arrays */
+typedef struct {
+  uint32_t object_id, proxy_type, slabs_per_motion_segment, n_patches, n_segments, patch0, segment0;
+  double eps_proxy;
+} orc_build_object;
+typedef struct {
+  uint32_t patch_id;
+  double local_center[3], radius;
+} orc_build_patch;
+typedef struct {
+  double t0, t1, translation0[3], rotation0[4], translation1[3], rotation1[4];
+} orc_build_segment;
+#define ORC_PRIM_DOUBLES 36
+
 /* scorers (mlp.c) */
 uint64_t orc_mix64(uint64_t x);
 double orc_unit_hash(uint64_t x);

@@ -1,4 +1,5 @@
-// p2cccd_cpp: the reference's pybind11 module name and certificate surface
+// p2cccd_cpp (compiled as its core, p2cccd_cpp._p2cccd_cpp): the reference's
+// pybind11 module name and certificate surface
 // (src/cpp/bindings/py_module.cpp:1326-1357, 1358-1431, 1715-1796, 1831-1851,
 // 1958-2014), backed by the gfx950 library.  Errors raise ValueError with the
 // Status message, as the reference's ThrowIfError does (:38-42).
@@ -57,9 +58,56 @@ py::dict BindingStatus() {
   return d;
 }
 
+// An external query batch given as the reference's Python objects (anything
+// with .queries whose items carry query_id, source_query_index, family "vf" /
+// "ee", vertices_t0, vertices_t1, ground_truth_collides and box_pair), parsed
+// with the reference's rules and messages (py_module.cpp:118-181).
+std::array<std::array<double, 3>, 4> ParseVertices(const py::handle& vertices, const char* field) {
+  const py::sequence seq = py::cast<py::sequence>(vertices);
+  if (py::len(seq) != 4) throw py::value_error(std::string(field) + " must contain four vertices");
+  std::array<std::array<double, 3>, 4> out{};
+  for (std::size_t i = 0; i < 4; ++i) {
+    const py::sequence v = py::cast<py::sequence>(seq[i]);
+    if (py::len(v) != 3) throw py::value_error(std::string(field) + " vertices must be 3D");
+    for (std::size_t a = 0; a < 3; ++a) out[i][a] = py::cast<double>(v[a]);
+  }
+  return out;
+}
+
+std::vector<ExternalBatchQuery> ParseExternalBatch(const py::object& batch) {
+  const py::sequence items = py::cast<py::sequence>(batch.attr("queries"));
+  std::vector<ExternalBatchQuery> out;
+  out.reserve(py::len(items));
+  for (const py::handle& item : items) {
+    ExternalBatchQuery q;
+    q.source_query_id = py::cast<std::uint64_t>(item.attr("query_id"));
+    q.source_query_index = py::cast<std::uint32_t>(item.attr("source_query_index"));
+    const std::string fam = py::str(item.attr("family"));
+    if (fam == "vf") q.family = ExternalQueryFamily::kVertexFace;
+    else if (fam == "ee") q.family = ExternalQueryFamily::kEdgeEdge;
+    else throw py::value_error("unsupported external CCD query family: " + fam);
+    q.vertices_t0 = ParseVertices(item.attr("vertices_t0"), "vertices_t0");
+    q.vertices_t1 = ParseVertices(item.attr("vertices_t1"), "vertices_t1");
+    const py::object gt = item.attr("ground_truth_collides");
+    if (!gt.is_none()) {
+      q.has_ground_truth = true;
+      q.ground_truth_collides = py::cast<bool>(gt);
+    }
+    const py::object pair = item.attr("box_pair");
+    if (!pair.is_none()) {
+      const py::sequence bp = py::cast<py::sequence>(pair);
+      if (py::len(bp) != 2) throw py::value_error("box_pair must contain exactly two object ids");
+      q.has_box_pair = true;
+      q.box_pair = {py::cast<std::uint32_t>(bp[0]), py::cast<std::uint32_t>(bp[1])};
+    }
+    out.push_back(q);
+  }
+  return out;
+}
+
 }  // namespace
 
-PYBIND11_MODULE(p2cccd_cpp, m) {
+PYBIND11_MODULE(_p2cccd_cpp, m) {
   m.doc() = "P2CCCD certificate bindings backed by hand-written gfx950 (MI355X) kernels.";
 
   py::enum_<ProxyType>(m, "ProxyType")
@@ -422,6 +470,74 @@ PYBIND11_MODULE(p2cccd_cpp, m) {
       .def_readwrite("candidates", &CandidateGenerationResult::candidates)
       .def_readwrite("timing", &CandidateGenerationResult::timing)
       .def_readwrite("density", &CandidateGenerationResult::density);
+  py::class_<PoseSample>(m, "PoseSample")
+      .def(py::init<>())
+      .def_readwrite("translation", &PoseSample::translation)
+      .def_readwrite("rotation_xyzw", &PoseSample::rotation_xyzw);
+  py::class_<MotionSegment>(m, "MotionSegment")
+      .def(py::init<>())
+      .def_readwrite("t0", &MotionSegment::t0)
+      .def_readwrite("t1", &MotionSegment::t1)
+      .def_readwrite("pose_t0", &MotionSegment::pose_t0)
+      .def_readwrite("pose_t1", &MotionSegment::pose_t1);
+  py::class_<Patch>(m, "Patch")
+      .def(py::init<>())
+      .def_readwrite("patch_id", &Patch::patch_id)
+      .def_readwrite("triangle_ids", &Patch::triangle_ids)
+      .def_readwrite("triangle_count", &Patch::triangle_count)
+      .def_readwrite("area", &Patch::area)
+      .def_readwrite("local_center", &Patch::local_center)
+      .def_readwrite("radius", &Patch::radius);
+  py::class_<ProxyObjectBuildInput>(m, "ProxyObjectBuildInput")
+      .def(py::init<>())
+      .def_readwrite("object_id", &ProxyObjectBuildInput::object_id)
+      .def_readwrite("proxy_type", &ProxyObjectBuildInput::proxy_type)
+      .def_readwrite("patches", &ProxyObjectBuildInput::patches)
+      .def_readwrite("motion_segments", &ProxyObjectBuildInput::motion_segments)
+      .def_readwrite("slabs_per_motion_segment", &ProxyObjectBuildInput::slabs_per_motion_segment)
+      .def_readwrite("eps_proxy", &ProxyObjectBuildInput::eps_proxy);
+  py::class_<ProxySceneBuildInput>(m, "ProxySceneBuildInput")
+      .def(py::init<>())
+      .def_readwrite("query_id", &ProxySceneBuildInput::query_id)
+      .def_readwrite("objects", &ProxySceneBuildInput::objects);
+  py::class_<RuntimeQueryIdMapping>(m, "RuntimeQueryIdMapping")
+      .def(py::init<>())
+      .def_readwrite("source_query_id", &RuntimeQueryIdMapping::source_query_id)
+      .def_readwrite("runtime_query_id", &RuntimeQueryIdMapping::runtime_query_id);
+  py::class_<ExternalBatchCandidateResult>(m, "ExternalBatchCandidateResult")
+      .def(py::init<>())
+      .def_readwrite("backend_name", &ExternalBatchCandidateResult::backend_name)
+      .def_readwrite("timing", &ExternalBatchCandidateResult::timing)
+      .def_readwrite("primitive_count", &ExternalBatchCandidateResult::primitive_count)
+      .def_readwrite("raw_hit_count", &ExternalBatchCandidateResult::raw_hit_count)
+      .def_readwrite("compact_candidate_count", &ExternalBatchCandidateResult::compact_candidate_count)
+      .def_readwrite("candidate_recall", &ExternalBatchCandidateResult::candidate_recall)
+      .def_readwrite("candidates", &ExternalBatchCandidateResult::candidates)
+      .def_readwrite("runtime_query_ids", &ExternalBatchCandidateResult::runtime_query_ids);
+  m.def("build_proxy_scene",
+        [](const ProxySceneBuildInput& input) {
+          ProxyScene scene;
+          ThrowIfError(BuildProxyScene(input, &scene));
+          return scene;
+        },
+        py::arg("input"));
+  m.def("generate_candidates_for_external_batch",
+        [](const py::object& batch, const std::string& backend_name, bool allow_optix_cpu_fallback) {
+          CheckCandidateBackend(backend_name);
+          CandidateGeneratorConfig config;
+          config.backend = backend_name.rfind("optix", 0) == 0 ? CandidateBackend::kOptix : CandidateBackend::kCpuReference;
+          config.allow_optix_cpu_fallback = allow_optix_cpu_fallback;
+          const std::vector<ExternalBatchQuery> queries = ParseExternalBatch(batch);
+          ExternalBatchCandidateResult result;
+          Status s;
+          {
+            py::gil_scoped_release release;
+            s = GenerateCandidatesForExternalBatch(queries, config, &result);
+          }
+          ThrowIfError(s);
+          return result;
+        },
+        py::arg("batch"), py::arg("backend_name") = "cpu_reference", py::arg("allow_optix_cpu_fallback") = false);
   m.def("validate_proxy_scene",
         [](const ProxyScene& scene) {
           ThrowIfError(ValidateProxyScene(scene));

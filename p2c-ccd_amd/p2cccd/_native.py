@@ -13,6 +13,11 @@ from pathlib import Path
 
 import numpy as np
 
+try:  # one HIP runtime per process: bind the library to the one torch loads
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 _PKG_ROOT = Path(__file__).resolve().parent.parent
 LIB_PATH = _PKG_ROOT / "lib" / "libp2c_hip.so"
 
