@@ -13,6 +13,8 @@
 namespace py = pybind11;
 using namespace p2cccd;
 
+void BindPolicy(py::module_& m);  // py_policy.cpp
+
 namespace {
 
 void ThrowIfError(const Status& s) {
@@ -586,6 +588,7 @@ PYBIND11_MODULE(_p2cccd_cpp, m) {
           return out;
         },
         py::arg("scene"), py::arg("raw_buffer"));
+  BindPolicy(m);
   m.def("audit_log_rows_for_query",
         [](const std::vector<AuditLogRow>& rows, std::uint64_t query_id) {
           std::vector<AuditLogRow> out;

@@ -14,3 +14,4 @@ except ImportError:  # the bindings then load the system runtime
 
 from ._p2cccd_cpp import *  # noqa: E402,F401,F403
 from ._p2cccd_cpp import __doc__  # noqa: E402,F401
+from ._p2cccd_cpp import _dense_group_columns, _schedule_runtime_columns  # noqa: E402,F401  (p2cccd.policy)
