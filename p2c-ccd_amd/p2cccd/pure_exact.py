@@ -64,11 +64,18 @@ def _family(query) -> str:
     return f
 
 
-def evaluate_external_ccd_queries(queries: Sequence, config: PureExactCPUConfig | None = None,
-                                  device: str | torch.device = "cuda") -> list[PureExactQueryResult]:
-    """Certifies every query (one launch per family); results in query order."""
-    cfg = _validate_config(config or PureExactCPUConfig())
-    out: list[PureExactQueryResult | None] = [None] * len(queries)
+WALK_PYTHON_TWIN = 2  # P2C_CERT_PYTHON_TWIN
+WALK_CUDA_TWIN = 1    # P2C_CERT_CUDA_TWIN
+
+
+def certify_queries(queries: Sequence, config: PureExactCPUConfig, walk: int,
+                    device: str | torch.device = "cuda") -> np.ndarray:
+    """Raw certificate results (PRIM_RESULT_DTYPE, query order) of external
+    queries over [0, 1] with one of the reference's walks, one device launch
+    per family.  Primitive ids are 0 and 1, as the reference's twin path
+    numbers them (rt_exact.py:223-281)."""
+    cfg = _validate_config(config)
+    out = np.zeros(len(queries), nat.PRIM_RESULT_DTYPE)
     fams = [_family(q) for q in queries]
     for fam, code in (("vf", 1), ("ee", 2)):
         idx = [i for i, f in enumerate(fams) if f == fam]
@@ -80,25 +87,33 @@ def evaluate_external_ccd_queries(queries: Sequence, config: PureExactCPUConfig 
             x[r, 4:] = np.asarray(queries[i].vertices_t1, np.float64)
         if not np.isfinite(x).all():
             raise ValueError("external CCD query vertices must be finite")
-        ids = np.stack([np.arange(len(idx)), np.arange(len(idx))], 1)
-        prims = nat.prims_from_positions(x, ids=ids)
+        prims = nat.prims_from_positions(x, ids=np.tile([0, 1], (len(idx), 1)))
         c = nat.cert_config(cfg.eps_time, cfg.eps_space, cfg.max_subdivision_depth)
-        c["flags"] = 2  # P2C_CERT_PYTHON_TWIN
+        c["flags"] = walk
         d_in = torch.from_numpy(prims.view(np.uint8).copy()).to(device)
         d_out = torch.zeros(len(idx) * nat.PRIM_RESULT_DTYPE.itemsize, dtype=torch.uint8, device=device)
         nat.call("p2c_cert_ref_batch", code, d_in.data_ptr(), len(idx), 0.0, 1.0, nat.ptr(c), d_out.data_ptr(),
                  0, nat.stream_handle())
-        res = d_out.cpu().numpy().view(nat.PRIM_RESULT_DTYPE)
-        for r, i in enumerate(idx):
-            status = _STATUS[int(res["status"][r])]
-            q = queries[i]
-            out[i] = PureExactQueryResult(
-                query_id=q.query_id, family=_WITNESS[fam],
-                predicted_collision=status == "collision" or (status == "undecided"
-                                                               and cfg.conservative_undecided_as_collision),
-                ground_truth_collision=getattr(q, "ground_truth_collides", None), status=status,
-                toi_upper=float(res["toi_upper"][r]), safe_margin_lb=float(res["safe_margin_lb"][r]),
-                exact_evals=3 * int(res["nodes"][r]), max_depth=int(res["pad"][r][0]))
+        out[idx] = d_out.cpu().numpy().view(nat.PRIM_RESULT_DTYPE)
+    return out
+
+
+def evaluate_external_ccd_queries(queries: Sequence, config: PureExactCPUConfig | None = None,
+                                  device: str | torch.device = "cuda") -> list[PureExactQueryResult]:
+    """Certifies every query with the Python twin's walk (one launch per
+    family); results in query order."""
+    cfg = _validate_config(config or PureExactCPUConfig())
+    res = certify_queries(queries, cfg, WALK_PYTHON_TWIN, device)
+    out = []
+    for i, q in enumerate(queries):
+        status = _STATUS[int(res["status"][i])]
+        out.append(PureExactQueryResult(
+            query_id=q.query_id, family=_WITNESS[_family(q)],
+            predicted_collision=status == "collision" or (status == "undecided"
+                                                           and cfg.conservative_undecided_as_collision),
+            ground_truth_collision=getattr(q, "ground_truth_collides", None), status=status,
+            toi_upper=float(res["toi_upper"][i]), safe_margin_lb=float(res["safe_margin_lb"][i]),
+            exact_evals=3 * int(res["nodes"][i]), max_depth=int(res["pad"][i][0])))
     return out
 
 
