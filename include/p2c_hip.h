@@ -135,6 +135,144 @@ int p2c_cert_ref_eval_cpu(int family, const p2c_prim* prim, double interval_t0,
                           double interval_t1, const p2c_cert_config* config,
                           p2c_prim_result* out);
 
+/* ---- exact work queue ------------------------------------------------------
+ * Replaces ProcessExactWorkQueueCpu and GenerateConservativeRefinementWorkItems
+ * (certificate_engine.cpp:593-720, certificate_engine.h:83-120): every entry's
+ * certificate (CertificateEngine::Evaluate, :729-824), its audit rows and its
+ * refinement children, computed on the device.
+ *
+ * One queue entry: an ExactWorkItem (runtime_contracts.h:67-81), its query's
+ * CertificateEngineConfig (certificate_engine.h:56-60) and its primitives,
+ * [pt_begin, pt_begin + pt_count) of the point-triangle array and
+ * [ee_begin, ee_begin + ee_count) of the edge-edge array (ranges may overlap
+ * and be shared between entries).  120 bytes. */
+typedef struct p2c_queue_entry {
+  uint64_t work_item_id;
+  uint64_t parent_candidate_id;
+  uint64_t query_id;
+  uint32_t slab_id;
+  uint32_t patch_a_id;
+  uint32_t patch_b_id;
+  uint32_t feature_family_mask;
+  double interval_t0;
+  double interval_t1;
+  uint32_t topk_feature_ids_offset;
+  uint16_t depth;
+  uint8_t source;
+  uint8_t pad0;
+  float priority_score;
+  uint32_t max_subdivision_depth;
+  double eps_time;
+  double eps_space;
+  int64_t pt_begin;
+  int64_t pt_count;
+  int64_t ee_begin;
+  int64_t ee_count;
+} p2c_queue_entry;
+
+/* CertificateResult (runtime_contracts.h:83-99).  96 bytes. */
+typedef struct p2c_certificate {
+  uint64_t work_item_id;
+  uint64_t query_id;
+  double interval_t0;
+  double interval_t1;
+  double toi_upper;
+  double safe_margin_lb;
+  int64_t witness_id_a;
+  int64_t witness_id_b;
+  double eps_time;
+  double eps_space;
+  uint32_t covered_feature_mask;
+  uint16_t reason_code;
+  uint8_t status;
+  uint8_t witness_family;
+  uint8_t next_refinement_mode;
+  uint8_t pad[7];
+} p2c_certificate;
+
+/* AuditLogRow (runtime_contracts.h:101-114).  80 bytes. */
+typedef struct p2c_audit_row {
+  uint64_t event_id;
+  uint64_t query_id;
+  uint64_t candidate_id;
+  uint64_t work_item_id;
+  double interval_t0;
+  double interval_t1;
+  uint64_t timestamp_us;
+  double aux_value0;
+  double aux_value1;
+  uint16_t action;
+  uint16_t depth;
+  uint8_t stage;
+  uint8_t pad[3];
+} p2c_audit_row;
+
+/* ExactWorkQueueConfig (certificate_engine.h:83-87); walk_flags selects the
+ * walk as p2c_cert_config.flags does (0: the CPU engine's). */
+typedef struct p2c_queue_config {
+  uint64_t first_event_id;
+  uint64_t first_timestamp_us;
+  uint32_t emit_dequeue_events;
+  uint32_t walk_flags;
+} p2c_queue_config;
+
+/* ExactRefinementConfig (certificate_engine.h:95-99). */
+typedef struct p2c_refine_config {
+  uint64_t first_child_work_item_id;
+  double min_interval_width;
+  uint32_t max_child_depth;
+  uint32_t pad;
+} p2c_refine_config;
+
+/* status words of p2c_exact_queue_run */
+#define P2C_QUEUE_STATUS_WORDS 6
+/* Device workspace of a queue of n entries whose entries certify at most
+ * pt_pairs point-triangle and ee_pairs edge-edge (entry, primitive) pairs
+ * (the sums of pt_count and ee_count bound them). */
+size_t p2c_exact_queue_workspace_bytes(int64_t n, int64_t pt_pairs, int64_t ee_pairs);
+/* Certifies the n entries (device arrays; pts of n_pt and ees of n_ee
+ * primitives), in one pass of device kernels, in the reference's queue
+ * order semantics:
+ *   certs[i]  entry i's certificate: the PT primitives of an entry whose mask
+ *             has bit 0, then its EE primitives (bit 1), the first collision
+ *             winning, an invalid config or a reached invalid primitive
+ *             giving the invalid-input certificate (MakeInvalidInputCertificate);
+ *   rows      with emit_dequeue_events, rows[2i] the EXACT/DEQUEUED row and
+ *             rows[2i + 1] the CERTIFY row of entry i, event ids and
+ *             timestamps counting up from the config's first ones; without,
+ *             rows[i] the CERTIFY row;
+ *   children  (refine not NULL) the refinement children of every entry in
+ *             queue order, as queue entries with their parent's config and
+ *             primitive ranges: entry i's first child id is
+ *             first_child_work_item_id plus the number of children of entries
+ *             before it (GenerateConservativeRefinementWorkItems per entry
+ *             with a running first id).  At most child_cap are written.
+ * status (device u64[P2C_QUEUE_STATUS_WORDS]): [0] UINT64_MAX, or the first
+ * failure as entry * 64 + code in the reference's check order (decode with
+ * p2c_exact_queue_error); [1] the child count; [2] UINT64_MAX, or the
+ * position of an entry that repeats an earlier (query_id, work_item_id);
+ * [3] non-zero when the pair bounds were too small; [4], [5] the PT and EE
+ * pairs the children written would certify (the next pass's bounds).
+ * Asynchronous on `stream`. */
+int p2c_exact_queue_run(const p2c_queue_entry* entries, int64_t n, const p2c_prim* pts, int64_t n_pt,
+                        const p2c_prim* ees, int64_t n_ee, int64_t pt_pairs, int64_t ee_pairs,
+                        const p2c_queue_config* config, const p2c_refine_config* refine, void* workspace,
+                        size_t workspace_bytes, p2c_certificate* certs, p2c_audit_row* rows,
+                        p2c_queue_entry* children, int64_t child_cap, uint64_t* status, void* stream);
+/* Turns status words read back from p2c_exact_queue_run into the call's
+ * result: P2C_OK, or the error code with the reference's message for the
+ * first failure (ValidateExactWorkItem, ValidateAuditLogRow,
+ * ValidateCertificateResult, ValidateExactWorkQueueCoverage) in
+ * p2c_last_error(). */
+int p2c_exact_queue_error(const uint64_t* status_host);
+/* Host-buffer form (the reference's host vectors): allocates, copies in,
+ * runs, synchronises and copies out.  certs holds n, rows n or 2n, children
+ * child_cap entries; *n_children gets the child count (may be NULL). */
+int p2c_exact_queue_host(const p2c_queue_entry* entries, int64_t n, const p2c_prim* pts, int64_t n_pt,
+                         const p2c_prim* ees, int64_t n_ee, const p2c_queue_config* config,
+                         const p2c_refine_config* refine, p2c_certificate* certs, p2c_audit_row* rows,
+                         p2c_queue_entry* children, int64_t child_cap, int64_t* n_children);
+
 /* ---- (t,u,v) inclusion certificate --------------------------------------
  * Replaces the Tight-Inclusion call sites ticcd::vertexFaceCCD / edgeEdgeCCD
  * (tight_inclusion_full_query_benchmark.cpp:375-397, with ms, tolerance,

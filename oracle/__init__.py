@@ -585,3 +585,99 @@ def _external_batch(q: np.ndarray):
     for k, c in enumerate(out):
         c["candidate_id"] = (c["query_id"] * 1_000_003 + k + 1) & 0xFFFFFFFFFFFFFFFF
     return out, np.array(rids, dtype=np.uint64)
+
+
+# ---- exact work queue (oracle/ref_queue.c, ref_shim.cpp) ---------------------
+# Byte layouts of the library's p2c_queue_entry / p2c_certificate / p2c_audit_row.
+QUEUE_ENTRY = np.dtype([
+    ("work_item_id", "<u8"), ("parent_candidate_id", "<u8"), ("query_id", "<u8"), ("slab_id", "<u4"),
+    ("patch_a_id", "<u4"), ("patch_b_id", "<u4"), ("feature_family_mask", "<u4"), ("interval_t0", "<f8"),
+    ("interval_t1", "<f8"), ("topk_feature_ids_offset", "<u4"), ("depth", "<u2"), ("source", "u1"),
+    ("pad0", "u1"), ("priority_score", "<f4"), ("max_subdivision_depth", "<u4"), ("eps_time", "<f8"),
+    ("eps_space", "<f8"), ("pt_begin", "<i8"), ("pt_count", "<i8"), ("ee_begin", "<i8"), ("ee_count", "<i8"),
+])
+CERTIFICATE = np.dtype([
+    ("work_item_id", "<u8"), ("query_id", "<u8"), ("interval_t0", "<f8"), ("interval_t1", "<f8"),
+    ("toi_upper", "<f8"), ("safe_margin_lb", "<f8"), ("witness_id_a", "<i8"), ("witness_id_b", "<i8"),
+    ("eps_time", "<f8"), ("eps_space", "<f8"), ("covered_feature_mask", "<u4"), ("reason_code", "<u2"),
+    ("status", "u1"), ("witness_family", "u1"), ("next_refinement_mode", "u1"), ("pad", "u1", (7,)),
+])
+AUDIT_ROW = np.dtype([
+    ("event_id", "<u8"), ("query_id", "<u8"), ("candidate_id", "<u8"), ("work_item_id", "<u8"),
+    ("interval_t0", "<f8"), ("interval_t1", "<f8"), ("timestamp_us", "<u8"), ("aux_value0", "<f8"),
+    ("aux_value1", "<f8"), ("action", "<u2"), ("depth", "<u2"), ("stage", "u1"), ("pad", "u1", (3,)),
+])
+assert QUEUE_ENTRY.itemsize == 120 and CERTIFICATE.itemsize == 96 and AUDIT_ROW.itemsize == 80
+_u64 = ctypes.c_uint64
+
+
+def _queue_sigs(L, compiled: bool):
+    if getattr(L, "_queue_ready", False):
+        return L
+    if compiled:
+        L.ref_process_queue.argtypes = [_vp, _i64, _vp, _vp, _u64, _u64, _i32, _vp, _vp, ctypes.c_char_p, _i64]
+        L.ref_process_queue.restype = _i32
+        L.ref_refine_queue.argtypes = [_vp, _vp, _i64, _u64, ctypes.c_uint32, _f64, _vp, _i64, ctypes.c_char_p,
+                                       _i64]
+        L.ref_refine_queue.restype = _i64
+    else:
+        L.orc_queue_process.argtypes = [_vp, _i64, _vp, _vp, _u64, _u64, _i32, _vp, _vp,
+                                        ctypes.POINTER(ctypes.c_char_p)]
+        L.orc_queue_process.restype = _i32
+        L.orc_queue_refine.argtypes = [_vp, _vp, _i64, _u64, ctypes.c_uint32, _f64, _vp, _i64]
+        L.orc_queue_refine.restype = _i64
+    L._queue_ready = True
+    return L
+
+
+def _queue_args(entries, pts, ees):
+    entries = np.ascontiguousarray(entries, QUEUE_ENTRY)
+    pts = np.ascontiguousarray(pts, PRIM)
+    ees = np.ascontiguousarray(ees, PRIM)
+    return entries, pts, ees
+
+
+def queue_process(entries, pts, ees, first_event_id=1, first_timestamp_us=1, emit=True, compiled=False):
+    """ProcessExactWorkQueueCpu: (certificates, rows), or ValueError with the
+    reference's message.  compiled=True runs the reference's own code
+    (oracle/_ref), else the C restatement."""
+    entries, pts, ees = _queue_args(entries, pts, ees)
+    n = entries.shape[0]
+    certs = np.zeros(n, CERTIFICATE)
+    rows = np.zeros(n * (2 if emit else 1), AUDIT_ROW)
+    if compiled:
+        L = _queue_sigs(ref_lib(), True)
+        msg = ctypes.create_string_buffer(512)
+        rc = L.ref_process_queue(entries.ctypes.data, n, pts.ctypes.data, ees.ctypes.data, first_event_id,
+                                 first_timestamp_us, int(emit), certs.ctypes.data, rows.ctypes.data, msg, 512)
+        text = msg.value.decode()
+    else:
+        L = _queue_sigs(lib(), False)
+        m = ctypes.c_char_p()
+        rc = L.orc_queue_process(entries.ctypes.data, n, pts.ctypes.data, ees.ctypes.data, first_event_id,
+                                 first_timestamp_us, int(emit), certs.ctypes.data, rows.ctypes.data, ctypes.byref(m))
+        text = (m.value or b"").decode()
+    if rc != 0:
+        raise ValueError(text)
+    return certs, rows
+
+
+def queue_refine(entries, certs, first_child=1, max_child_depth=64, min_interval_width=1.0e-4, compiled=False):
+    """GenerateConservativeRefinementWorkItems per entry in queue order with a
+    running first child id; children as entries."""
+    entries = np.ascontiguousarray(entries, QUEUE_ENTRY)
+    certs = np.ascontiguousarray(certs, CERTIFICATE)
+    n = entries.shape[0]
+    out = np.zeros(2 * n, QUEUE_ENTRY)
+    if compiled:
+        L = _queue_sigs(ref_lib(), True)
+        msg = ctypes.create_string_buffer(512)
+        got = L.ref_refine_queue(entries.ctypes.data, certs.ctypes.data, n, first_child, max_child_depth,
+                                 min_interval_width, out.ctypes.data, 2 * n, msg, 512)
+        if got < 0:
+            raise ValueError(msg.value.decode())
+    else:
+        L = _queue_sigs(lib(), False)
+        got = L.orc_queue_refine(entries.ctypes.data, certs.ctypes.data, n, first_child, max_child_depth,
+                                 min_interval_width, out.ctypes.data, 2 * n)
+    return out[:got]

@@ -7,6 +7,7 @@
  *
  * Contents (each function cites the reference lines it restates):
  *   ref_cert.c    reference-parity interval certificate (CertificateEngine)
+ *   ref_queue.c   exact work queue: aggregation, audit rows, refinement children
  *   tuv_cert.c    (t,u,v) inclusion certificate, canonical best-first order
  *   broadphase.c  swept boxes + sort-and-sweep VF/EE candidate sets + brute force
  *   mlp.c         TinySTPF FP64 scorer and the 32-128-128-1 FP32 group scorer
@@ -15,6 +16,9 @@
  *   ref_cert  : golden vectors written by the compiled reference
  *               (oracle/_ref, tests/golden/ref_cert_*.npz) + the reference
  *               KATs (toi 0.25, margin > 1.9).
+ *   ref_queue : the compiled reference's ProcessExactWorkQueueCpu and
+ *               GenerateConservativeRefinementWorkItems (oracle/_ref) field
+ *               for field, and its golden vectors (tests/golden/queue_golden.npz).
  *   tuv_cert  : parity with Tight-Inclusion itself is UNPINNED (library not
  *               vendored); pinned by closed forms, exact-rational brute force
  *               (sympy) on small cases and invariants (zero FN, toi <= exact).
@@ -72,6 +76,82 @@ int orc_ref_eval_twin_batch(int family, const orc_prim* prims, int64_t n, double
 int orc_ref_eval_py_batch(int family, const orc_prim* prims, int64_t n, double eps_time, double eps_space,
                           int max_depth, orc_result* out);
 double orc_pt_distance(const orc_prim* prim, double t);
+
+/* Exact work queue (certificate_engine.cpp:593-720).  The entry, certificate
+ * and audit-row records have the byte layout of the library's C-ABI records
+ * so one test buffer feeds both.  An entry is an ExactWorkItem, its query's
+ * config and its primitive ranges into the PT and EE arrays. */
+typedef struct {
+  uint64_t work_item_id;
+  uint64_t parent_candidate_id;
+  uint64_t query_id;
+  uint32_t slab_id;
+  uint32_t patch_a_id;
+  uint32_t patch_b_id;
+  uint32_t feature_family_mask;
+  double interval_t0;
+  double interval_t1;
+  uint32_t topk_feature_ids_offset;
+  uint16_t depth;
+  uint8_t source;
+  uint8_t pad0;
+  float priority_score;
+  uint32_t max_subdivision_depth;
+  double eps_time;
+  double eps_space;
+  int64_t pt_begin;
+  int64_t pt_count;
+  int64_t ee_begin;
+  int64_t ee_count;
+} orc_queue_entry;
+
+typedef struct {
+  uint64_t work_item_id;
+  uint64_t query_id;
+  double interval_t0;
+  double interval_t1;
+  double toi_upper;
+  double safe_margin_lb;
+  int64_t witness_id_a;
+  int64_t witness_id_b;
+  double eps_time;
+  double eps_space;
+  uint32_t covered_feature_mask;
+  uint16_t reason_code;
+  uint8_t status;
+  uint8_t witness_family;
+  uint8_t next_refinement_mode;
+  uint8_t pad[7];
+} orc_certificate;
+
+typedef struct {
+  uint64_t event_id;
+  uint64_t query_id;
+  uint64_t candidate_id;
+  uint64_t work_item_id;
+  double interval_t0;
+  double interval_t1;
+  uint64_t timestamp_us;
+  double aux_value0;
+  double aux_value1;
+  uint16_t action;
+  uint16_t depth;
+  uint8_t stage;
+  uint8_t pad[3];
+} orc_audit_row;
+
+/* ProcessExactWorkQueueCpu restated: 0, or -1 with the reference's message
+ * for the first failure in *msg.  rows holds 2n (emit) or n rows. */
+int orc_queue_process(const orc_queue_entry* entries, int64_t n, const orc_prim* pts, const orc_prim* ees,
+                      uint64_t first_event_id, uint64_t first_timestamp_us, int emit, orc_certificate* certs,
+                      orc_audit_row* rows, const char** msg);
+/* GenerateConservativeRefinementWorkItems for each entry in queue order, the
+ * first child id of each being first_child plus the children before it;
+ * children keep their parent's config and ranges.  Returns the child count
+ * (children beyond cap are counted, not written). */
+int64_t orc_queue_refine(const orc_queue_entry* entries, const orc_certificate* certs, int64_t n,
+                         uint64_t first_child, uint32_t max_child_depth, double min_interval_width,
+                         orc_queue_entry* out, int64_t cap);
 double orc_ee_distance(const orc_prim* prim, double t);
 
 /* (t,u,v) inclusion certificate, canonical sequential order. */

@@ -1,9 +1,11 @@
 // Implementation of the reference C++ certificate interface over libp2c_hip.
 //
-// Engine aggregation, queue audit rows, coverage guard, refinement children
-// and validators restate src/cpp/certificate/certificate_engine.cpp:388-824
-// and src/cpp/common/validators.cpp:160-315 (same rules, same messages);
-// primitive certificates come from the gfx950 kernels.
+// Single-query aggregation, coverage guard, per-parent refinement children and
+// validators restate src/cpp/certificate/certificate_engine.cpp:388-824 and
+// src/cpp/common/validators.cpp:160-315 (same rules, same messages);
+// primitive certificates come from the gfx950 kernels, and the work queue
+// (certificates and audit rows of every entry) from the library's device
+// queue pass.
 #include "certificate_hip.h"
 
 #include <algorithm>
@@ -177,37 +179,6 @@ void ApplyUndecided(std::uint16_t reason, CertificateResult* r) {
   r->safe_margin_lb = 0.0;
   r->reason_code = reason == kCertificateReasonNone ? kCertificateReasonMaxSubdivisionDepth : reason;
   r->next_refinement_mode = ModeForReason(r->reason_code);
-}
-
-std::uint16_t AuditAction(const CertificateResult& r) {
-  switch (r.status) {
-    case CertificateStatus::kCollision:
-      return kExactAuditCollision;
-    case CertificateStatus::kSeparation:
-      return kExactAuditSeparation;
-    case CertificateStatus::kUndecided:
-      return r.reason_code == kCertificateReasonInvalidInput ? kExactAuditInvalidInput
-                                                             : kExactAuditUndecided;
-  }
-  return kExactAuditUndecided;
-}
-
-AuditLogRow MakeRow(std::uint64_t event, std::uint64_t ts, const ExactWorkItem& w, AuditStage stage,
-                    std::uint16_t action, double aux0, double aux1) {
-  AuditLogRow r;
-  r.event_id = event;
-  r.query_id = w.query_id;
-  r.candidate_id = w.parent_candidate_id;
-  r.work_item_id = w.work_item_id;
-  r.stage = stage;
-  r.action = action;
-  r.depth = w.depth;
-  r.interval_t0 = w.interval_t0;
-  r.interval_t1 = w.interval_t1;
-  r.timestamp_us = ts;
-  r.aux_value0 = aux0;
-  r.aux_value1 = aux1;
-  return r;
 }
 
 // A batch of primitives sharing (family, interval, config), certified in one
@@ -502,52 +473,93 @@ Status ValidateExactWorkQueueCoverage(const std::vector<ExactCertificateQuery>& 
   return Status::Ok();
 }
 
+// ProcessExactWorkQueueCpu (certificate_engine.cpp:593-661) on the device:
+// the queue is packed into p2c_queue_entry records and two primitive arrays,
+// and p2c_exact_queue_host computes every certificate and audit row in one
+// pass of kernels, failing with the message the reference's loop would give
+// first.
 Status ProcessExactWorkQueueCpu(const std::vector<ExactCertificateQuery>& queue,
                                 const ExactWorkQueueConfig& config, ExactWorkQueueResult* result) {
   if (result == nullptr) return Status::Error("exact work queue result output pointer is null");
-  if (config.first_event_id == 0) return Status::Error("ExactWorkQueueConfig.first_event_id must be non-zero");
-  if (config.first_timestamp_us == 0)
-    return Status::Error("ExactWorkQueueConfig.first_timestamp_us must be non-zero");
-  // The reference fails the whole queue at the first invalid work item.
-  for (const auto& q : queue)
-    if (auto s = ValidateExactWorkItem(q.work_item); !s.ok) return s;
-
-  std::vector<PrimSlot> slots;
-  std::vector<std::size_t> begins;
-  if (auto s = CertifyQueries(queue, &slots, &begins); !s.ok) return s;
-
-  ExactWorkQueueResult out;
-  out.certificates.reserve(queue.size());
-  out.audit_log.reserve(queue.size() * (config.emit_dequeue_events ? 2U : 1U));
-  std::uint64_t event = config.first_event_id;
-  std::uint64_t ts = config.first_timestamp_us;
-  for (std::size_t qi = 0; qi < queue.size(); ++qi) {
-    const ExactCertificateQuery& q = queue[qi];
-    if (config.emit_dequeue_events) {
-      AuditLogRow row = MakeRow(event++, ts++, q.work_item, AuditStage::kExact, kExactAuditDequeued,
-                                q.work_item.priority_score,
-                                static_cast<double>(q.work_item.feature_family_mask));
-      if (auto s = ValidateAuditLogRow(row); !s.ok) return s;
-      out.audit_log.push_back(row);
-    }
-    CertificateResult cert;
-    Status st = ValidateConfig(q.config);
-    if (st.ok) st = Aggregate(q, slots, begins[2 * qi], begins[2 * qi + 1], &cert);
-    if (!st.ok) {
-      cert = BaseResult(q.work_item, q.config);
-      ApplyUndecided(kCertificateReasonInvalidInput, &cert);
-    }
-    if (auto s = ValidateCertificateResult(cert); !s.ok) return s;
-    AuditLogRow row = MakeRow(event++, ts++, q.work_item, AuditStage::kCertify, AuditAction(cert),
-                              cert.status == CertificateStatus::kSeparation ? cert.safe_margin_lb
-                                                                            : cert.toi_upper,
-                              static_cast<double>(cert.next_refinement_mode));
-    if (auto s = ValidateAuditLogRow(row); !s.ok) return s;
-    out.certificates.push_back(cert);
-    out.audit_log.push_back(row);
-    ++out.processed_count;
+  std::vector<p2c_queue_entry> entries(queue.size());
+  std::vector<p2c_prim> pts, ees;
+  for (std::size_t i = 0; i < queue.size(); ++i) {
+    const ExactCertificateQuery& q = queue[i];
+    const ExactWorkItem& w = q.work_item;
+    p2c_queue_entry& e = entries[i];
+    e = p2c_queue_entry{};
+    e.work_item_id = w.work_item_id;
+    e.parent_candidate_id = w.parent_candidate_id;
+    e.query_id = w.query_id;
+    e.slab_id = w.slab_id;
+    e.patch_a_id = w.patch_a_id;
+    e.patch_b_id = w.patch_b_id;
+    e.feature_family_mask = w.feature_family_mask;
+    e.interval_t0 = w.interval_t0;
+    e.interval_t1 = w.interval_t1;
+    e.topk_feature_ids_offset = w.topk_feature_ids_offset;
+    e.depth = w.depth;
+    e.source = static_cast<std::uint8_t>(w.source);
+    e.priority_score = w.priority_score;
+    e.max_subdivision_depth = q.config.max_subdivision_depth;
+    e.eps_time = q.config.eps_time;
+    e.eps_space = q.config.eps_space;
+    e.pt_begin = static_cast<std::int64_t>(pts.size());
+    e.pt_count = static_cast<std::int64_t>(q.point_triangle_primitives.size());
+    e.ee_begin = static_cast<std::int64_t>(ees.size());
+    e.ee_count = static_cast<std::int64_t>(q.edge_edge_primitives.size());
+    for (const auto& p : q.point_triangle_primitives) pts.push_back(Pack(p));
+    for (const auto& p : q.edge_edge_primitives) ees.push_back(Pack(p));
   }
-  if (auto s = ValidateExactWorkQueueCoverage(queue, out); !s.ok) return s;
+  const p2c_queue_config qc{config.first_event_id, config.first_timestamp_us,
+                            config.emit_dequeue_events ? 1u : 0u, 0u};
+  std::vector<p2c_certificate> certs(queue.size());
+  std::vector<p2c_audit_row> rows(queue.size() * (config.emit_dequeue_events ? 2U : 1U));
+  if (auto s = FromRc(p2c_exact_queue_host(entries.data(), static_cast<std::int64_t>(entries.size()), pts.data(),
+                                           static_cast<std::int64_t>(pts.size()), ees.data(),
+                                           static_cast<std::int64_t>(ees.size()), &qc, nullptr, certs.data(),
+                                           rows.data(), nullptr, 0, nullptr));
+      !s.ok)
+    return s;
+  ExactWorkQueueResult out;
+  out.certificates.reserve(certs.size());
+  for (const p2c_certificate& c : certs) {
+    CertificateResult r;
+    r.work_item_id = c.work_item_id;
+    r.query_id = c.query_id;
+    r.status = static_cast<CertificateStatus>(c.status);
+    r.interval_t0 = c.interval_t0;
+    r.interval_t1 = c.interval_t1;
+    r.toi_upper = c.toi_upper;
+    r.safe_margin_lb = c.safe_margin_lb;
+    r.witness_family = c.witness_family;
+    r.witness_id_a = c.witness_id_a;
+    r.witness_id_b = c.witness_id_b;
+    r.covered_feature_mask = c.covered_feature_mask;
+    r.eps_time = c.eps_time;
+    r.eps_space = c.eps_space;
+    r.reason_code = c.reason_code;
+    r.next_refinement_mode = static_cast<CertificateRefinementMode>(c.next_refinement_mode);
+    out.certificates.push_back(r);
+  }
+  out.audit_log.reserve(rows.size());
+  for (const p2c_audit_row& a : rows) {
+    AuditLogRow r;
+    r.event_id = a.event_id;
+    r.query_id = a.query_id;
+    r.candidate_id = a.candidate_id;
+    r.work_item_id = a.work_item_id;
+    r.stage = static_cast<AuditStage>(a.stage);
+    r.action = a.action;
+    r.depth = a.depth;
+    r.interval_t0 = a.interval_t0;
+    r.interval_t1 = a.interval_t1;
+    r.timestamp_us = a.timestamp_us;
+    r.aux_value0 = a.aux_value0;
+    r.aux_value1 = a.aux_value1;
+    out.audit_log.push_back(r);
+  }
+  out.processed_count = queue.size();
   *result = std::move(out);
   return Status::Ok();
 }
