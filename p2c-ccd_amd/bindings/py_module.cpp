@@ -13,7 +13,8 @@
 namespace py = pybind11;
 using namespace p2cccd;
 
-void BindPolicy(py::module_& m);  // py_policy.cpp
+void BindPolicy(py::module_& m);    // py_policy.cpp
+void BindDatasets(py::module_& m);  // datasets_io.cpp
 
 namespace {
 
@@ -589,6 +590,7 @@ PYBIND11_MODULE(_p2cccd_cpp, m) {
         },
         py::arg("scene"), py::arg("raw_buffer"));
   BindPolicy(m);
+  BindDatasets(m);
   m.def("audit_log_rows_for_query",
         [](const std::vector<AuditLogRow>& rows, std::uint64_t query_id) {
           std::vector<AuditLogRow> out;
