@@ -408,7 +408,8 @@ int p2c_tuv_certify_static(const void* pairs, int64_t n, double tolerance, int64
  * 256-302 boxes, 528-550 object envelopes, 595-696 VF/EE sets): candidate
  * sets are identical as sets.  All pointers in p2c_bp_mesh are device
  * pointers; topology (faces, edges, objects) is prepared once per mesh as in
- * the reference's PrepareScene (:497-586).  At most 64 objects per scene. */
+ * the reference's PrepareScene (:497-586).  At most 32768 objects per scene
+ * (the object-pair matrix takes n_objects^2 bytes of the workspace). */
 typedef struct p2c_bp_mesh {
   const int32_t* faces;          /* (F, 3) */
   const int32_t* edges;          /* (E, 2), (min, max) */

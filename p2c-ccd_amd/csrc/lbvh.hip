@@ -37,7 +37,8 @@ namespace p2c {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kMaxObjects = 64;
+constexpr int kMaxObjects = 1 << 15;  // the allowed matrix holds kMaxObjects^2 bytes (1 GiB)
+constexpr int kLdsObjects = 64;        // envelopes reduced in LDS per workgroup up to this many objects
 
 struct Box6 {
   double mn[3];
@@ -86,7 +87,7 @@ __device__ inline bool overlap(const Box6& a, const double* mn, const double* mx
 
 // ---- workspace ----------------------------------------------------------
 struct Layout {
-  size_t roots, vbox, fbox, ebox, objkey, allowed, keys0, vals0, keys1, vals1, hist, fnodes, fparent, fflag, frcs,
+  size_t roots, vbox, fbox, ebox, objkey, bounds, allowed, keys0, vals0, keys1, vals1, hist, fnodes, fparent, fflag, frcs,
       fhi, flo, enodes, eparent, eflag, ercs, ehi, elo, qorder, eorder, counts, resume, offsets, hist_off, scan_part, err,
       eroots, total;
   int64_t sort_cap, hist_blocks;
@@ -110,7 +111,8 @@ Layout make_layout(int64_t nv, int64_t nf, int64_t ne, int32_t nobj) {
   L.vbox = take(sizeof(Box6) * nv);
   L.fbox = take(sizeof(Box6) * nf);
   L.ebox = take(sizeof(Box6) * ne);
-  L.objkey = take(sizeof(uint64_t) * 6 * nobj);
+  L.objkey = take(sizeof(uint64_t) * 6 * size_t(nobj));
+  L.bounds = take(sizeof(double) * 6);
   L.allowed = take(size_t(nobj) * nobj);
   L.keys0 = take(4 * sall);
   L.vals0 = take(4 * sall);
@@ -151,22 +153,28 @@ T* at(void* ws, size_t off) {
 
 // ---- 1. boxes and object envelopes -------------------------------------
 // Resets the object envelopes, the error counters (4 words) and the counts
-// of both trees' top-refit root lists; one workgroup of kMaxObjects * 6 lanes.
-__global__ void objkey_init_kernel(uint64_t* objkey, int32_t nobj, uint32_t* err, uint32_t* froots,
-                                   uint32_t* eroots) {
-  const int i = threadIdx.x;
-  if (i < nobj * 6) objkey[i] = (i % 6) < 3 ? ~0ull : 0ull;
-  if (i < 4) err[i] = 0u;
-  if (i == 4) froots[0] = 0u;
-  if (i == 5) eroots[0] = 0u;
+// of both trees' top-refit root lists.
+__global__ __launch_bounds__(kBlock) void objkey_init_kernel(uint64_t* objkey, int32_t nobj, uint32_t* err,
+                                                             uint32_t* froots, uint32_t* eroots) {
+  const int64_t i0 = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  for (int64_t i = i0; i < int64_t(nobj) * 6; i += int64_t(gridDim.x) * blockDim.x)
+    objkey[i] = (i % 6) < 3 ? ~0ull : 0ull;
+  if (i0 < 4) err[i0] = 0u;
+  if (i0 == 4) froots[0] = 0u;
+  if (i0 == 5) eroots[0] = 0u;
 }
 
 __global__ __launch_bounds__(kBlock) void vertex_box_kernel(const double* __restrict__ x0,
                                                             const double* __restrict__ x1, int64_t nv, double th,
                                                             const int32_t* __restrict__ vobj, int32_t nobj,
                                                             Box6* __restrict__ vbox, uint64_t* __restrict__ objkey) {
-  __shared__ unsigned long long lkey[kMaxObjects * 6];
-  for (int i = threadIdx.x; i < nobj * 6; i += blockDim.x) lkey[i] = (i % 6) < 3 ? ~0ull : 0ull;
+  // up to kLdsObjects objects: envelopes reduced in LDS, then one global
+  // atomic per word and workgroup; more objects: global atomics per vertex
+  __shared__ unsigned long long lkey[kLdsObjects * 6];
+  const bool lds = nobj <= kLdsObjects;
+  unsigned long long* key = lds ? lkey : reinterpret_cast<unsigned long long*>(objkey);
+  if (lds)
+    for (int i = threadIdx.x; i < nobj * 6; i += blockDim.x) lkey[i] = (i % 6) < 3 ? ~0ull : 0ull;
   __syncthreads();
   for (int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < nv; v += int64_t(gridDim.x) * blockDim.x) {
     Box6 b;
@@ -179,11 +187,12 @@ __global__ __launch_bounds__(kBlock) void vertex_box_kernel(const double* __rest
     if (o >= 0 && o < nobj) {
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        atomicMin(&lkey[o * 6 + k], (unsigned long long)f64_order_key(b.mn[k]));
-        atomicMax(&lkey[o * 6 + 3 + k], (unsigned long long)f64_order_key(b.mx[k]));
+        atomicMin(&key[int64_t(o) * 6 + k], (unsigned long long)f64_order_key(b.mn[k]));
+        atomicMax(&key[int64_t(o) * 6 + 3 + k], (unsigned long long)f64_order_key(b.mx[k]));
       }
     }
   }
+  if (!lds) return;
   __syncthreads();
   for (int i = threadIdx.x; i < nobj * 6; i += blockDim.x) {
     if ((i % 6) < 3) atomicMin((unsigned long long*)&objkey[i], lkey[i]);
@@ -199,21 +208,23 @@ __device__ inline void object_box(const uint64_t* objkey, int o, double* mn, dou
   }
 }
 
-__global__ void allowed_kernel(const uint64_t* __restrict__ objkey, int32_t nobj, uint8_t* __restrict__ allowed) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nobj * nobj) return;
-  const int a = i / nobj, b = i % nobj;
-  uint8_t ok = 1;
-  if (a != b) {
-    double amn[3], amx[3], bmn[3], bmx[3];
-    object_box(objkey, a, amn, amx);
-    object_box(objkey, b, bmn, bmx);
-    ok = (amn[0] <= bmx[0] && bmn[0] <= amx[0] && amn[1] <= bmx[1] && bmn[1] <= amx[1] && amn[2] <= bmx[2] &&
-          bmn[2] <= amx[2])
-             ? 1
-             : 0;
+__global__ __launch_bounds__(kBlock) void allowed_kernel(const uint64_t* __restrict__ objkey, int32_t nobj,
+                                                         uint8_t* __restrict__ allowed) {
+  const int64_t n2 = int64_t(nobj) * nobj;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n2; i += int64_t(gridDim.x) * blockDim.x) {
+    const int a = int(i / nobj), b = int(i % nobj);
+    uint8_t ok = 1;
+    if (a != b) {
+      double amn[3], amx[3], bmn[3], bmx[3];
+      object_box(objkey, a, amn, amx);
+      object_box(objkey, b, bmn, bmx);
+      ok = (amn[0] <= bmx[0] && bmn[0] <= amx[0] && amn[1] <= bmx[1] && bmn[1] <= amx[1] && amn[2] <= bmx[2] &&
+            bmn[2] <= amx[2])
+               ? 1
+               : 0;
+    }
+    allowed[i] = ok;
   }
-  allowed[i] = ok;
 }
 
 __device__ inline uint32_t spread10(uint32_t x) {
@@ -236,13 +247,19 @@ __device__ inline uint32_t morton_of(const Box6& b, const double* lo, const doub
   return (spread10(c[0]) << 2) | (spread10(c[1]) << 1) | spread10(c[2]);
 }
 
-__device__ inline void scene_bounds(const uint64_t* objkey, int32_t nobj, double* lo, double* inv) {
-  double hi[3];
+// The Morton frame: the union of the object envelopes (lo, 1 / extent per
+// axis), reduced once by one workgroup into bounds[6].  Its min/max over the
+// same values in any order is exact, so the frame does not depend on the
+// reduction order.
+__global__ __launch_bounds__(kBlock) void scene_bounds_kernel(const uint64_t* __restrict__ objkey, int32_t nobj,
+                                                              double* __restrict__ bounds) {
+  __shared__ double red[6][kBlock];
+  double lo[3], hi[3];
   for (int k = 0; k < 3; ++k) {
     lo[k] = __builtin_inf();
     hi[k] = -__builtin_inf();
   }
-  for (int o = 0; o < nobj; ++o) {
+  for (int o = threadIdx.x; o < nobj; o += blockDim.x) {
     double mn[3], mx[3];
     object_box(objkey, o, mn, mx);
     for (int k = 0; k < 3; ++k) {
@@ -251,9 +268,25 @@ __device__ inline void scene_bounds(const uint64_t* objkey, int32_t nobj, double
     }
   }
   for (int k = 0; k < 3; ++k) {
-    const double ext = hi[k] - lo[k];
-    inv[k] = ext > 0.0 ? 1.0 / ext : 0.0;
+    red[k][threadIdx.x] = lo[k];
+    red[3 + k][threadIdx.x] = hi[k];
   }
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if (int(threadIdx.x) < w)
+      for (int k = 0; k < 3; ++k) {
+        const double a = red[k][threadIdx.x + w], b = red[3 + k][threadIdx.x + w];
+        red[k][threadIdx.x] = a < red[k][threadIdx.x] ? a : red[k][threadIdx.x];
+        red[3 + k][threadIdx.x] = b > red[3 + k][threadIdx.x] ? b : red[3 + k][threadIdx.x];
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 3; ++k) {
+      const double ext = red[3 + k][0] - red[k][0];
+      bounds[k] = red[k][0];
+      bounds[3 + k] = ext > 0.0 ? 1.0 / ext : 0.0;
+    }
 }
 
 // Face boxes and keys (items [0, nf)), edge boxes and keys ([nf, nf + ne)),
@@ -268,19 +301,21 @@ struct PrimKeys {
   const int32_t* faces;
   const int32_t* edges;
   const Box6* vbox;
-  const uint64_t* objkey;
+  const double* bounds;  // lo[3], inv[3] of scene_bounds_kernel
   Box6* fbox;
   Box6* ebox;
   uint32_t* keys;
   uint32_t* vals;
   int64_t nf, ne, nv;
   double th;
-  int32_t nobj;
 };
 
 __global__ __launch_bounds__(kBlock) void prim_box_kernel(PrimKeys P) {
   double lo[3], inv[3];
-  scene_bounds(P.objkey, P.nobj, lo, inv);
+  for (int k = 0; k < 3; ++k) {
+    lo[k] = P.bounds[k];
+    inv[k] = P.bounds[3 + k];
+  }
   const int64_t total = P.nf + P.ne + P.nv;
   for (int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; g < total; g += int64_t(gridDim.x) * blockDim.x) {
     const uint32_t kind = g < P.nf ? 0u : (g < P.nf + P.ne ? 1u : 2u);
@@ -773,7 +808,7 @@ int check_mesh(const p2c_bp_mesh* m) {
   if (m->n_vertices < 0 || m->n_faces < 0 || m->n_edges < 0)
     return set_error(P2C_EINVAL, "mesh sizes must be non-negative");
   if (m->n_objects < 1 || m->n_objects > kMaxObjects)
-    return set_error(P2C_EUNSUPPORTED, "object count must be in [1, 64] per scene");
+    return set_error(P2C_EUNSUPPORTED, "object count must be in [1, 32768] per scene");
   if (m->n_faces > (int64_t(1) << 30) || m->n_edges > (int64_t(1) << 30) || m->n_vertices > (int64_t(1) << 30))
     return set_error(P2C_EUNSUPPORTED, "meshes above 2^30 primitives are not supported");
   if (m->hit_cache_slots < 0 || m->hit_cache_slots > 4096)
@@ -803,13 +838,15 @@ int p2c_bp_build(const p2c_bp_mesh* m, const double* x0, const double* x1, doubl
   const int32_t nobj = m->n_objects;
   uint64_t* objkey = at<uint64_t>(ws, L.objkey);
   Box6* vbox = at<Box6>(ws, L.vbox);
-  hipLaunchKernelGGL(objkey_init_kernel, dim3(1), dim3(kMaxObjects * 6), 0, s, objkey, nobj, at<uint32_t>(ws, L.err),
-                     at<uint32_t>(ws, L.roots), at<uint32_t>(ws, L.eroots));
+  hipLaunchKernelGGL(objkey_init_kernel, dim3(grid_for(int64_t(nobj) * 6, kBlock)), dim3(kBlock), 0, s, objkey, nobj,
+                     at<uint32_t>(ws, L.err), at<uint32_t>(ws, L.roots), at<uint32_t>(ws, L.eroots));
   if (nv > 0)
     hipLaunchKernelGGL(vertex_box_kernel, dim3(grid_for(nv, kBlock, 2048)), dim3(kBlock), 0, s, x0, x1, nv, thickness,
                        m->vertex_object, nobj, vbox, objkey);
-  hipLaunchKernelGGL(allowed_kernel, dim3(unsigned((nobj * nobj + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, objkey,
-                     nobj, at<uint8_t>(ws, L.allowed));
+  hipLaunchKernelGGL(allowed_kernel, dim3(grid_for(int64_t(nobj) * nobj, kBlock)), dim3(kBlock), 0, s, objkey, nobj,
+                     at<uint8_t>(ws, L.allowed));
+  double* bounds = at<double>(ws, L.bounds);
+  hipLaunchKernelGGL(scene_bounds_kernel, dim3(1), dim3(kBlock), 0, s, objkey, nobj, bounds);
   uint32_t* k0 = at<uint32_t>(ws, L.keys0);
   uint32_t* v0 = at<uint32_t>(ws, L.vals0);
   uint32_t* k1 = at<uint32_t>(ws, L.keys1);
@@ -829,8 +866,8 @@ int p2c_bp_build(const p2c_bp_mesh* m, const double* x0, const double* x1, doubl
   // boxes and tagged keys of every kind, then one sort: faces at 0, edges at nf, vertices at nf + ne
   const int64_t off_of[3] = {0, nf, nf + ne};
   if (nf + ne + nv > 0) {
-    const PrimKeys P{x0, x1, m->faces, m->edges, vbox, objkey, at<Box6>(ws, L.fbox), at<Box6>(ws, L.ebox), k0, v0,
-                     nf, ne, nv, thickness, nobj};
+    const PrimKeys P{x0, x1, m->faces, m->edges, vbox, bounds, at<Box6>(ws, L.fbox), at<Box6>(ws, L.ebox), k0, v0,
+                     nf, ne, nv, thickness};
     hipLaunchKernelGGL(prim_box_kernel, dim3(grid_for(nf + ne + nv, kBlock)), dim3(kBlock), 0, s, P);
     if (int rc = radix_sort(k0, v0, k1, v1, nf + ne + nv, hist, hist_off, part, s)) return rc;
   }

@@ -74,3 +74,63 @@ def test_hit_cache_size_does_not_change_the_output(gpu, slots):
     got = BroadPhase.for_scene(s, "cuda", cache_slots=slots).candidates(x0, x1, s.thickness)
     assert ref[0].shape[0] > 1000 and ref[1].shape[0] > 1000
     assert torch.equal(ref[0], got[0]) and torch.equal(ref[1], got[1])
+
+
+def _many_objects(n_obj, seed, spacing=0.3, level=0):
+    """n_obj small icospheres on a jittered grid, each moving on its own:
+    neighbours' envelopes overlap or not depending on the jitter."""
+    rng = np.random.default_rng(seed)
+    v, f = scenes.icosphere(level, 0.1)
+    side = int(np.ceil(n_obj ** (1 / 3)))
+    xs, fs = [], []
+    for k in range(n_obj):
+        c = np.array([k % side, (k // side) % side, k // (side * side)], float) * spacing
+        c += rng.uniform(-0.08, 0.08, 3)
+        xs.append(v + c)
+        fs.append(f + k * len(v))
+    x0 = np.concatenate(xs)
+    x1 = x0 + np.repeat(rng.normal(0.0, 0.04, (n_obj, 3)), len(v), axis=0)
+    faces = np.concatenate(fs).astype(np.int32)
+    return scenes.Scene(f"objects{n_obj}", x0, x1, scenes.prepare_topology(faces, len(x0)), 1e-3)
+
+
+@pytest.mark.parametrize("n_obj,seed", [(3, 1), (63, 2), (64, 3), (65, 4), (700, 5), (2000, 6)])
+def test_many_object_scenes_match_oracle(gpu, n_obj, seed):
+    s = _many_objects(n_obj, seed)
+    print(f"objects: {n_obj} seed {seed}", flush=True)
+    assert s.topo.object_count == n_obj
+    _compare(s)
+
+
+def test_disjoint_envelopes_exclude_object_pairs(gpu, tmp_path):
+    """Three objects read from PLY frames; the envelopes of objects 0 and 2
+    do not overlap (the reference's object_pair_allowed, :528-550), so no
+    candidate pairs them, though 0-1 and 1-2 pairs exist."""
+    import struct
+
+    from p2cccd.datasets.ply import scene_from_ply_frames
+
+    v, f = scenes.icosphere(1, 0.5)
+    x0 = np.concatenate([v + (0, 0, 0), v + (0.95, 0, 0), v + (1.9, 0, 0)])
+    x1 = x0 + 0.01
+    faces = np.concatenate([f, f + len(v), f + 2 * len(v)])
+
+    def write(path, x):
+        head = (f"ply\nformat binary_little_endian 1.0\nelement vertex {len(x)}\nproperty double x\n"
+                f"property double y\nproperty double z\nelement face {len(faces)}\n"
+                f"property list uchar int vertex_indices\nend_header\n").encode()
+        body = b"".join(struct.pack("<3d", *p) for p in x) + b"".join(struct.pack("<B3i", 3, *t) for t in faces)
+        path.write_bytes(head + body)
+        return path
+
+    s = scene_from_ply_frames(write(tmp_path / "f0.ply", x0), write(tmp_path / "f1.ply", x1), thickness=1e-3)
+    assert s.topo.object_count == 3
+    got, _ = _gpu_sets(s)
+    ref = oracle.candidates(s)
+    for k in ("vf", "ee"):
+        assert np.array_equal(got[k], ref[k]), k
+    vo, fo, eo = s.topo.vertex_object, s.topo.face_object, s.topo.edge_object
+    pairs_vf = {tuple(sorted((int(vo[a]), int(fo[b])))) for a, b in got["vf"]}
+    pairs_ee = {tuple(sorted((int(eo[a]), int(eo[b])))) for a, b in got["ee"]}
+    assert (0, 1) in pairs_vf and (1, 2) in pairs_vf and (0, 2) not in pairs_vf
+    assert (0, 1) in pairs_ee and (0, 2) not in pairs_ee
