@@ -352,8 +352,12 @@ int p2c_tuv_rounds(const void* pairs, int64_t n, const uint64_t* totals, double 
  * Pairs needing more than `light_budget` boxes save their walk in `states`
  * (n * p2c_tuv_state_bytes() bytes) and are listed in `heavy_list`, as are
  * the walks of a wave left with `handover_at` (0..64) or fewer busy lanes
- * once every pair has been claimed; p2c_tuv_wave finishes them.  `counters`:
- * 64 zero-initialised device bytes, laid out as for p2c_tuv_persistent. */
+ * once every pair has been claimed; p2c_tuv_wave finishes them.  With
+ * `totals`, the VF pairs [0, totals[0]) and the EE pairs after them are
+ * claimed through two cursors, each wave starting on the range of its share
+ * of the grid, so that waves hold one kind of pair.  `counters`: 64
+ * zero-initialised device bytes, laid out as for p2c_tuv_persistent, with the
+ * EE range's cursor (u64) at byte 24. */
 int p2c_tuv_stream(const void* pairs, int64_t n, const uint64_t* totals, double tolerance, int64_t max_itr,
                    int64_t light_budget, int refill_at, int handover_at, p2c_tuv_result* out, void* counters,
                    void* states, uint32_t* heavy_list, uint32_t list_cap, int grid_blocks, void* stream);
@@ -383,6 +387,43 @@ int p2c_tuv_wave_split(const void* pairs, void* counters, void* states, const ui
 int p2c_tuv_replay(const void* pairs, const uint32_t* replay_list, const uint32_t* replay_count,
                    uint32_t replay_cap, double tolerance, int64_t max_itr, void* heaps, int64_t heap_cap,
                    int64_t slots, p2c_tuv_result* out, void* stream);
+
+/* Compact pair records of the broad-phase path: P2C_TUV_LITE_BYTES per pair
+ * (the four vertex ids and the split sequence, in the same 64-pair tiles;
+ * p2c_tuv_lite_bytes(n) bytes for n pairs).  A kernel loading a pair
+ * rebuilds its positions, displacements and error bounds from the vertex
+ * arrays named by `verts`, which must be the arrays and the minimum
+ * separation the records were set up with; results are those of the
+ * 288-byte records.  p2c_tuv_setup_pairs_lite takes the arguments of
+ * p2c_tuv_setup_pairs; the _lite certification calls take those of
+ * p2c_tuv_plain, p2c_tuv_stream, p2c_tuv_wave_split and p2c_tuv_replay plus
+ * `verts`.  No
+ * counterpart in the reference (its certificate reads positions from the
+ * meshes per query on the CPU). */
+#define P2C_TUV_LITE_BYTES 64
+typedef struct p2c_tuv_vertices {
+  const double* x0; /* device (V, 3) positions at t = 0 */
+  const double* x1; /* device (V, 3) positions at t = 1 */
+  double ms;        /* minimum separation */
+} p2c_tuv_vertices;
+size_t p2c_tuv_lite_bytes(int64_t n);
+int p2c_tuv_setup_pairs_lite(int is_ee, const double* x0, const double* x1, const int32_t* topo,
+                             const int32_t* pairs, int64_t n, const uint64_t* totals, int64_t out_offset,
+                             int64_t out_cap, double ms, double tolerance, void* lite_pairs, void* stream);
+int p2c_tuv_plain_lite(const void* lite_pairs, const p2c_tuv_vertices* verts, int64_t n, const uint64_t* totals,
+                       double tolerance, int64_t max_itr, p2c_tuv_result* out, uint32_t* replay_list,
+                       uint32_t* replay_count, uint32_t replay_cap, void* stream);
+int p2c_tuv_stream_lite(const void* lite_pairs, const p2c_tuv_vertices* verts, int64_t n, const uint64_t* totals,
+                        double tolerance, int64_t max_itr, int64_t light_budget, int refill_at, int handover_at,
+                        p2c_tuv_result* out, void* counters, void* states, uint32_t* heavy_list, uint32_t list_cap,
+                        int grid_blocks, void* stream);
+int p2c_tuv_wave_split_lite(const void* lite_pairs, const p2c_tuv_vertices* verts, void* counters, void* states,
+                            const uint32_t* heavy_list, uint32_t list_cap, double tolerance, int64_t max_itr,
+                            p2c_tuv_result* out, uint32_t* replay_list, int64_t split_at, void* tasks,
+                            uint32_t task_cap, uint32_t* split_list, int grid_blocks, void* stream);
+int p2c_tuv_replay_lite(const void* lite_pairs, const p2c_tuv_vertices* verts, const uint32_t* replay_list,
+                        const uint32_t* replay_count, uint32_t replay_cap, double tolerance, int64_t max_itr,
+                        void* heaps, int64_t heap_cap, int64_t slots, p2c_tuv_result* out, void* stream);
 
 /* Allocation-free certificate.  No counterpart in the reference: it is for
  * callers that cannot allocate device memory (code under HIP graph capture,

@@ -93,6 +93,77 @@ __device__ inline void write_result(p2c_tuv_result* out, uint64_t best, uint64_t
   *out = r;
 }
 
+// Compact record of a pair given by vertex ids (64 bytes, 4 granules, the
+// same 64-pair tiles): the ids and the split sequence.  Positions,
+// displacements and error bounds are rebuilt from the vertex arrays when a
+// kernel loads the pair; a mesh's arrays stay resident in L2 while its pairs
+// stream, so a refill reads 64 bytes from HBM instead of 288.
+struct TuvLite {
+  int32_t vid[4];
+  uint64_t split[5];
+  uint32_t done_level;
+  uint32_t is_ee;
+};
+static_assert(sizeof(TuvLite) == P2C_TUV_LITE_BYTES, "TuvLite layout");
+constexpr int kLiteGranules = int(sizeof(TuvLite) / 16);
+
+__device__ inline const Granule* lite_granules(const TuvLite* base, int64_t p) {
+  return reinterpret_cast<const Granule*>(base) + (p / kPairTile) * (int64_t(kLiteGranules) * kPairTile) +
+         (p % kPairTile);
+}
+
+__device__ inline void store_lite(TuvLite* __restrict__ base, int64_t p, const TuvLite& r) {
+  Granule g[kLiteGranules];
+  __builtin_memcpy(g, &r, sizeof(TuvLite));
+  Granule* dst = const_cast<Granule*>(lite_granules(base, p));
+#pragma unroll
+  for (int i = 0; i < kLiteGranules; ++i) {
+    const u32x4 v = {g[i].w[0], g[i].w[1], g[i].w[2], g[i].w[3]};
+    __builtin_nontemporal_store(v, reinterpret_cast<u32x4*>(dst + int64_t(i) * kPairTile));
+  }
+}
+
+// Where the certificate kernels take pair p from: full records (FullSrc) or
+// compact records expanded from the vertex arrays (LiteSrc).  Both give the
+// same TuvPair (the split sequence was computed from the same positions),
+// so results do not depend on the record form.
+struct FullSrc {
+  const TuvPair* pairs;
+  __device__ TuvPair load(int64_t p) const { return load_pair(pairs, p); }
+};
+
+struct LiteSrc {
+  const TuvLite* pairs;
+  const double* x0;
+  const double* x1;
+  double ms;
+  __device__ TuvPair load(int64_t p) const {
+    const Granule* src = lite_granules(pairs, p);
+    Granule g[kLiteGranules];
+#pragma unroll
+    for (int i = 0; i < kLiteGranules; ++i) g[i] = src[int64_t(i) * kPairTile];
+    TuvLite r;
+    __builtin_memcpy(&r, g, sizeof(TuvLite));
+    double v[8][3];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int64_t o = int64_t(r.vid[i]) * 3;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        v[i][k] = x0[o + k];
+        v[4 + i][k] = x1[o + k];
+      }
+    }
+    TuvPair q;
+    tuv_setup_geometry(&q, v, int(r.is_ee), ms);
+    q.inv_tol[0] = q.inv_tol[1] = q.inv_tol[2] = 0.0;  // used only to build the split sequence
+#pragma unroll
+    for (int w = 0; w < 5; ++w) q.split[w] = r.split[w];
+    q.done_level = r.done_level;
+    return q;
+  }
+};
+
 __global__ __launch_bounds__(kTuvBlock) void tuv_setup_positions_kernel(int is_ee, const double* __restrict__ v,
                                                                         int64_t n, double ms, double tol,
                                                                         TuvPair* __restrict__ pairs) {
@@ -175,12 +246,58 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_setup_pairs_kernel(int is_ee, c
   }
 }
 
+// The same pairs as compact records.
+__global__ __launch_bounds__(kTuvBlock) void tuv_setup_lite_kernel(int is_ee, const double* __restrict__ x0,
+                                                                   const double* __restrict__ x1,
+                                                                   const int32_t* __restrict__ topo,
+                                                                   const int32_t* __restrict__ pairs, int64_t n,
+                                                                   const uint64_t* __restrict__ totals,
+                                                                   int64_t out_offset, int64_t out_cap, double ms,
+                                                                   double tol, TuvLite* __restrict__ out) {
+  int64_t m = n, off = out_offset;
+  if (totals != nullptr) {
+    const int64_t t = int64_t(totals[is_ee]);
+    m = t < n ? t : n;
+    off = is_ee ? int64_t(totals[0]) : 0;
+  }
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < m; i += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t a = pairs[i * 2], b = pairs[i * 2 + 1];
+    int32_t vid[4];
+    if (!is_ee) {
+      vid[0] = int32_t(a);
+      vid[1] = topo[b * 3];
+      vid[2] = topo[b * 3 + 1];
+      vid[3] = topo[b * 3 + 2];
+    } else {
+      vid[0] = topo[a * 2];
+      vid[1] = topo[a * 2 + 1];
+      vid[2] = topo[b * 2];
+      vid[3] = topo[b * 2 + 1];
+    }
+    double x[8][3];
+    for (int r = 0; r < 4; ++r)
+      for (int k = 0; k < 3; ++k) {
+        x[r][k] = x0[int64_t(vid[r]) * 3 + k];
+        x[4 + r][k] = x1[int64_t(vid[r]) * 3 + k];
+      }
+    TuvPair q;
+    tuv_setup(&q, x, is_ee, ms, tol);
+    TuvLite r;
+    for (int j = 0; j < 4; ++j) r.vid[j] = vid[j];
+    for (int w = 0; w < 5; ++w) r.split[w] = q.split[w];
+    r.done_level = q.done_level;
+    r.is_ee = q.is_ee;
+    if (off + i < out_cap) store_lite(out, off + i, r);
+  }
+}
+
 __device__ inline void flag_for_replay(uint32_t pair, uint32_t* list, uint32_t* count, uint32_t cap) {
   const uint32_t slot = atomicAdd(count, 1u);
   if (slot < cap) list[slot] = pair;
 }
 
-__global__ __launch_bounds__(kTuvBlock) void tuv_plain_kernel(const TuvPair* __restrict__ pairs, int64_t n_host,
+template <typename Src>
+__global__ __launch_bounds__(kTuvBlock) void tuv_plain_kernel(Src src, int64_t n_host,
                                                               const uint64_t* __restrict__ totals,
                                                               double tol, uint64_t max_itr,
                                                               p2c_tuv_result* __restrict__ out,
@@ -189,7 +306,7 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_plain_kernel(const TuvPair* __r
                                                               uint32_t replay_cap) {
   const int64_t n = device_count(n_host, totals);
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
-    const TuvPair q = load_pair(pairs, i);
+    const TuvPair q = src.load(i);
     uint64_t best = kTuvNoHit, boxes = 0;
     const bool complete = tuv_walk(q, 0, 0, 0, 0, tol, &best, &boxes, max_itr);
     write_result(&out[i], best, boxes, complete ? 0u : P2C_TUV_FLAG_OVER_BUDGET);
@@ -469,7 +586,8 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_round_kernel(const TuvPair* __r
 // few long walks do not hold a nearly empty wave.
 constexpr int kStreamChunk = 256;
 
-__global__ __launch_bounds__(kTuvBlock) void tuv_stream_kernel(const TuvPair* __restrict__ pairs, int64_t n_host,
+template <typename Src>
+__global__ __launch_bounds__(kTuvBlock) void tuv_stream_kernel(Src src, int64_t n_host,
                                                                const uint64_t* __restrict__ totals, double tol,
                                                                uint64_t light_budget, uint32_t refill_at,
                                                                uint32_t handover_at,
@@ -480,8 +598,19 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_stream_kernel(const TuvPair* __
                                                                uint32_t* __restrict__ heavy_count,
                                                                uint32_t list_cap) {
   const int64_t n = device_count(n_host, totals);
+  // Two ranges with a cursor each: with device totals the VF pairs [0, split)
+  // and the EE pairs [split, n).  A wave starts on its home range (the first
+  // split / n of the waves on the VF range) and takes the other only once its
+  // own is exhausted, so waves hold one kind of pair apart from the end:
+  // the corner evaluation branches on the kind, and a wave holding both
+  // kinds runs both branches every step.
+  const int64_t split = totals != nullptr ? (int64_t(totals[0]) < n ? int64_t(totals[0]) : n) : n;
   const int lane = threadIdx.x & 63;
   const uint64_t lt = (1ull << lane) - 1ull;
+  const int64_t wave = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+  const int64_t waves = (int64_t(gridDim.x) * blockDim.x) >> 6;
+  int range = wave * n < waves * split ? 0 : 1;  // wave-uniform
+  bool spent[2] = {false, false};                 // ranges found exhausted (wave-uniform)
   int64_t next = 0, end = 0;  // the wave's claimed chunk (wave-uniform)
   bool drained = false;       // no pairs left to claim (wave-uniform)
   int64_t pair = -1;
@@ -492,21 +621,31 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_stream_kernel(const TuvPair* __
     if (!drained && (uint32_t)__popcll(idle) >= refill_at) {
       while (idle != 0) {
         if (next >= end) {
-          unsigned long long base = 0;
-          if (lane == 0) base = atomicAdd(cursor, (unsigned long long)kStreamChunk);
-          base = __shfl(base, 0);
-          next = (int64_t)base;
-          end = next + kStreamChunk < n ? next + kStreamChunk : n;
-          if (next >= n) {
-            drained = true;
-            break;
+          for (;;) {
+            const int64_t lo = range == 0 ? 0 : split, hi = range == 0 ? split : n;
+            unsigned long long base = 0;
+            if (!(range == 0 ? spent[0] : spent[1])) {
+              if (lane == 0) base = atomicAdd(cursor + (range == 0 ? 0 : 3), (unsigned long long)kStreamChunk);
+              base = __shfl(base, 0);
+              next = lo + (int64_t)base;
+              end = next + kStreamChunk < hi ? next + kStreamChunk : hi;
+              if (next < hi) break;
+            }
+            if (range == 0) spent[0] = true;
+            else spent[1] = true;
+            if (spent[0] && spent[1]) {
+              drained = true;
+              break;
+            }
+            range = 1 - range;
           }
+          if (drained) break;
         }
         const int64_t avail = end - next;
         const int rank = __popcll(idle & lt);
         if (pair < 0 && rank < avail) {
           pair = next + rank;
-          q = load_pair(pairs, pair);
+          q = src.load(pair);
           start_walk(w);
         }
         const int64_t took = avail < __popcll(idle) ? avail : (int64_t)__popcll(idle);
@@ -830,8 +969,8 @@ __device__ int wave_certify(const TuvPair& q, uint32_t p, WaveTask* wq, unsigned
 
 // Three workgroups per CU (12 waves: 3 per SIMD; 147 KB of LDS): at most 168
 // VGPRs, a few spilled; the wave pass ran 20 % longer at 2 waves per SIMD.
-template <bool kSplit>
-__global__ __launch_bounds__(kTuvBlock, 3) void tuv_heavy_kernel(const TuvPair* __restrict__ pairs,
+template <bool kSplit, typename Src>
+__global__ __launch_bounds__(kTuvBlock, 3) void tuv_heavy_kernel(Src src,
                                                               const uint32_t* __restrict__ heavy_list,
                                                               const uint32_t* __restrict__ heavy_count,
                                                               uint32_t list_cap, uint32_t* __restrict__ heavy_next,
@@ -850,7 +989,7 @@ __global__ __launch_bounds__(kTuvBlock, 3) void tuv_heavy_kernel(const TuvPair* 
     h = __shfl(h, 0);
     if (h >= total) break;
     const uint32_t p = heavy_list[h];
-    const TuvPair q = load_pair(pairs, p);
+    const TuvPair q = src.load(p);
     LaneWalk w;
     uint64_t boxes = 0;
     start_walk(w);
@@ -877,7 +1016,8 @@ __global__ __launch_bounds__(kTuvBlock, 3) void tuv_heavy_kernel(const TuvPair* 
 
 // Split tasks, one wave per task; boxes and an over-budget mark go to the
 // pair's accumulators.
-__global__ __launch_bounds__(kTuvBlock, 3) void tuv_task_kernel(const TuvPair* __restrict__ pairs,
+template <typename Src>
+__global__ __launch_bounds__(kTuvBlock, 3) void tuv_task_kernel(Src src,
                                                              uint32_t* __restrict__ task_next, double tol,
                                                              uint64_t max_itr, SplitOut so) {
   __shared__ WaveTask queue[kTuvBlock / 64][kWaveQueue];
@@ -892,7 +1032,7 @@ __global__ __launch_bounds__(kTuvBlock, 3) void tuv_task_kernel(const TuvPair* _
     if (h >= total) break;
     const SplitTask t = so.tasks[h];
     const uint32_t p = t.pair;
-    const TuvPair q = load_pair(pairs, p);
+    const TuvPair q = src.load(p);
     LaneWalk w;
     start_walk(w);
     w.best = __hip_atomic_load(&so.acc[p].best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1037,7 +1177,8 @@ done:
   *capped_out = capped;
 }
 
-__global__ __launch_bounds__(64) void tuv_replay_kernel(const TuvPair* __restrict__ pairs,
+template <typename Src>
+__global__ __launch_bounds__(64) void tuv_replay_kernel(Src src,
                                                         const uint32_t* __restrict__ list,
                                                         const uint32_t* __restrict__ count, uint32_t list_cap,
                                                         double tol, int64_t max_itr, HeapBox* __restrict__ heaps,
@@ -1049,7 +1190,7 @@ __global__ __launch_bounds__(64) void tuv_replay_kernel(const TuvPair* __restric
   if (m > list_cap) m = list_cap;
   for (int64_t j = slot; j < (int64_t)m; j += slots) {
     const uint32_t p = list[j];
-    const TuvPair q = load_pair(pairs, p);
+    const TuvPair q = src.load(p);
     uint64_t best, boxes;
     bool capped;
     tuv_canonical(q, tol, max_itr, heaps + slot * heap_cap, heap_cap, &best, &boxes, &capped);
@@ -1062,6 +1203,91 @@ int check_tuv(double ms, double tol, double t_max, int64_t max_itr) {
   if (!(ms >= 0.0) || !std::isfinite(ms)) return set_error(P2C_EINVAL, "minimum separation must be finite and >= 0");
   if (t_max != 1.0) return set_error(P2C_EUNSUPPORTED, "t_max must be 1 (the full step)");
   if (max_itr <= 0) return set_error(P2C_EINVAL, "max_itr must be positive");
+  return P2C_OK;
+}
+
+// ---- launches over a pair source ----------------------------------------
+template <typename Src>
+int launch_stream(const Src& src, int64_t n, const uint64_t* totals, double tolerance, int64_t max_itr,
+                  int64_t light_budget, int refill_at, int handover_at, p2c_tuv_result* out, void* counters,
+                  void* states, uint32_t* heavy_list, uint32_t list_cap, int grid_blocks, void* stream) {
+  if (int rc = check_tuv(0.0, tolerance, 1.0, max_itr)) return rc;
+  if (n <= 0) return P2C_OK;
+  if (grid_blocks <= 0) return set_error(P2C_EINVAL, "grid_blocks must be positive");
+  if (light_budget <= 0) return set_error(P2C_EINVAL, "light_budget must be positive");
+  if (refill_at < 1 || refill_at > 64) return set_error(P2C_EINVAL, "refill_at must be in 1..64");
+  if (handover_at < 0 || handover_at > 64) return set_error(P2C_EINVAL, "handover_at must be in 0..64");
+  if (light_budget > max_itr) light_budget = max_itr;
+  // counters: [0] next pair (u64), [1] replay count (u32) | heavy count (u32), [2] heavy cursor (u32)
+  auto* c64 = static_cast<unsigned long long*>(counters);
+  auto* c32 = static_cast<uint32_t*>(counters);
+  auto* st = static_cast<WalkState*>(states);
+  hipLaunchKernelGGL(tuv_stream_kernel<Src>, dim3(grid_blocks), dim3(kTuvBlock), 0, as_stream(stream), src, n,
+                     totals, tolerance, (uint64_t)light_budget, (uint32_t)refill_at, (uint32_t)handover_at, out, c64,
+                     st, heavy_list, c32 + 3, list_cap);
+  P2C_HIP_CHECK(hipGetLastError());
+  return P2C_OK;
+}
+
+template <typename Src>
+int launch_wave_split(const Src& src, void* counters, void* states, const uint32_t* heavy_list, uint32_t list_cap,
+                      double tolerance, int64_t max_itr, p2c_tuv_result* out, uint32_t* replay_list, int64_t split_at,
+                      void* tasks, uint32_t task_cap, uint32_t* split_list, int grid_blocks, void* stream) {
+  if (int rc = check_tuv(0.0, tolerance, 1.0, max_itr)) return rc;
+  if (grid_blocks <= 0) return set_error(P2C_EINVAL, "grid_blocks must be positive");
+  if (split_at < 0) return set_error(P2C_EINVAL, "split_at must be non-negative");
+  const bool split = split_at > 0;
+  if (split && (tasks == nullptr || task_cap == 0 || split_list == nullptr))
+    return set_error(P2C_EINVAL, "splitting needs a task buffer and a split list");
+  auto* c32 = static_cast<uint32_t*>(counters);
+  hipStream_t s = as_stream(stream);
+  SplitOut so{};
+  if (split) {
+    // counters (u32): [13] reserved task slots, [14] task cursor, [15] split pairs
+    so.split_at = (uint64_t)split_at;
+    so.tasks = static_cast<SplitTask*>(tasks);
+    so.task_cap = task_cap;
+    so.task_count = c32 + 13;
+    so.split_list = split_list;
+    so.split_count = c32 + 15;
+    so.acc = static_cast<SplitAcc*>(states);
+  }
+  const auto* resume = static_cast<const WalkState*>(states);
+  if (split)
+    hipLaunchKernelGGL((tuv_heavy_kernel<true, Src>), dim3(grid_blocks), dim3(kTuvBlock), 0, s, src, heavy_list,
+                       c32 + 3, list_cap, c32 + 4, resume, tolerance, (uint64_t)max_itr, out, replay_list, c32 + 2, so);
+  else
+    hipLaunchKernelGGL((tuv_heavy_kernel<false, Src>), dim3(grid_blocks), dim3(kTuvBlock), 0, s, src, heavy_list,
+                       c32 + 3, list_cap, c32 + 4, resume, tolerance, (uint64_t)max_itr, out, replay_list, c32 + 2, so);
+  if (split) {
+    hipLaunchKernelGGL(tuv_task_kernel<Src>, dim3(grid_blocks), dim3(kTuvBlock), 0, s, src, c32 + 14, tolerance,
+                       (uint64_t)max_itr, so);
+    hipLaunchKernelGGL(tuv_split_finish_kernel, dim3(16), dim3(kTuvBlock), 0, s, so, (uint64_t)max_itr, out,
+                       replay_list, c32 + 2, list_cap);
+  }
+  P2C_HIP_CHECK(hipGetLastError());
+  return P2C_OK;
+}
+
+template <typename Src>
+int launch_replay(const Src& src, const uint32_t* replay_list, const uint32_t* replay_count, uint32_t replay_cap,
+                  double tolerance, int64_t max_itr, void* heaps, int64_t heap_cap, int64_t slots,
+                  p2c_tuv_result* out, void* stream) {
+  if (int rc = check_tuv(0.0, tolerance, 1.0, max_itr)) return rc;
+  if (heap_cap < 8 || slots <= 0) return set_error(P2C_EINVAL, "replay needs heap_cap >= 8 and slots > 0");
+  hipLaunchKernelGGL(tuv_replay_kernel<Src>, dim3((unsigned)((slots + 63) / 64)), dim3(64), 0, as_stream(stream), src,
+                     replay_list, replay_count, replay_cap, tolerance, max_itr, static_cast<HeapBox*>(heaps), heap_cap,
+                     slots, out);
+  P2C_HIP_CHECK(hipGetLastError());
+  return P2C_OK;
+}
+
+int lite_src(const void* pairs, const p2c_tuv_vertices* verts, LiteSrc* src) {
+  if (verts == nullptr || verts->x0 == nullptr || verts->x1 == nullptr)
+    return set_error(P2C_EINVAL, "compact pair records need the vertex arrays");
+  if (!(verts->ms >= 0.0) || !std::isfinite(verts->ms))
+    return set_error(P2C_EINVAL, "minimum separation must be finite and >= 0");
+  *src = LiteSrc{static_cast<const TuvLite*>(pairs), verts->x0, verts->x1, verts->ms};
   return P2C_OK;
 }
 
@@ -1180,9 +1406,24 @@ int p2c_tuv_plain(const void* pairs, int64_t n, const uint64_t* totals, double t
                   void* stream) {
   if (int rc = p2c::check_tuv(0.0, tolerance, 1.0, max_itr)) return rc;
   if (n <= 0) return P2C_OK;
-  hipLaunchKernelGGL(p2c::tuv_plain_kernel, dim3(p2c::grid_for(n, p2c::kTuvBlock)), dim3(p2c::kTuvBlock), 0,
-                     p2c::as_stream(stream), static_cast<const p2c::TuvPair*>(pairs), n, totals, tolerance,
-                     (uint64_t)max_itr, out, replay_list, replay_count, replay_cap);
+  hipLaunchKernelGGL(p2c::tuv_plain_kernel<p2c::FullSrc>, dim3(p2c::grid_for(n, p2c::kTuvBlock)),
+                     dim3(p2c::kTuvBlock), 0, p2c::as_stream(stream),
+                     p2c::FullSrc{static_cast<const p2c::TuvPair*>(pairs)}, n, totals, tolerance, (uint64_t)max_itr,
+                     out, replay_list, replay_count, replay_cap);
+  P2C_HIP_CHECK(hipGetLastError());
+  return P2C_OK;
+}
+
+int p2c_tuv_plain_lite(const void* lite_pairs, const p2c_tuv_vertices* verts, int64_t n, const uint64_t* totals,
+                       double tolerance, int64_t max_itr, p2c_tuv_result* out, uint32_t* replay_list,
+                       uint32_t* replay_count, uint32_t replay_cap, void* stream) {
+  if (int rc = p2c::check_tuv(0.0, tolerance, 1.0, max_itr)) return rc;
+  p2c::LiteSrc src;
+  if (int rc = p2c::lite_src(lite_pairs, verts, &src)) return rc;
+  if (n <= 0) return P2C_OK;
+  hipLaunchKernelGGL(p2c::tuv_plain_kernel<p2c::LiteSrc>, dim3(p2c::grid_for(n, p2c::kTuvBlock)),
+                     dim3(p2c::kTuvBlock), 0, p2c::as_stream(stream), src, n, totals, tolerance, (uint64_t)max_itr,
+                     out, replay_list, replay_count, replay_cap);
   P2C_HIP_CHECK(hipGetLastError());
   return P2C_OK;
 }
@@ -1204,8 +1445,9 @@ int p2c_tuv_persistent(const void* pairs, int64_t n, const uint64_t* totals, dou
   hipLaunchKernelGGL(p2c::tuv_persistent_kernel, dim3(grid_blocks), dim3(p2c::kTuvBlock), 0, s,
                      static_cast<const p2c::TuvPair*>(pairs), n, totals, tolerance, (uint64_t)light_budget, out,
                      c64, heavy_list, c32 + 3, list_cap);
-  hipLaunchKernelGGL(p2c::tuv_heavy_kernel<false>, dim3(grid_blocks), dim3(p2c::kTuvBlock), 0, s,
-                     static_cast<const p2c::TuvPair*>(pairs), heavy_list, c32 + 3, list_cap, c32 + 4, nullptr,
+  hipLaunchKernelGGL((p2c::tuv_heavy_kernel<false, p2c::FullSrc>), dim3(grid_blocks), dim3(p2c::kTuvBlock), 0, s,
+                     p2c::FullSrc{static_cast<const p2c::TuvPair*>(pairs)}, heavy_list, c32 + 3, list_cap, c32 + 4,
+                     nullptr,
                      tolerance, (uint64_t)max_itr, out, replay_list, c32 + 2, p2c::SplitOut{});
   P2C_HIP_CHECK(hipGetLastError());
   return P2C_OK;
@@ -1239,7 +1481,8 @@ int p2c_tuv_rounds(const void* pairs, int64_t n, const uint64_t* totals, double 
                        c32 + 2, list_cap);
   }
   const uint32_t* heavy = lists[(n_rounds - 1) & 1];
-  hipLaunchKernelGGL(p2c::tuv_heavy_kernel<false>, dim3(grid_blocks), dim3(p2c::kTuvBlock), 0, s, tp, heavy, c32 + 3,
+  hipLaunchKernelGGL((p2c::tuv_heavy_kernel<false, p2c::FullSrc>), dim3(grid_blocks), dim3(p2c::kTuvBlock), 0, s,
+                     p2c::FullSrc{tp}, heavy, c32 + 3,
                      list_cap, c32 + 4, st, tolerance, (uint64_t)max_itr, out, replay_list, c32 + 2, p2c::SplitOut{});
   P2C_HIP_CHECK(hipGetLastError());
   return P2C_OK;
@@ -1248,24 +1491,9 @@ int p2c_tuv_rounds(const void* pairs, int64_t n, const uint64_t* totals, double 
 int p2c_tuv_stream(const void* pairs, int64_t n, const uint64_t* totals, double tolerance, int64_t max_itr,
                    int64_t light_budget, int refill_at, int handover_at, p2c_tuv_result* out, void* counters,
                    void* states, uint32_t* heavy_list, uint32_t list_cap, int grid_blocks, void* stream) {
-  if (int rc = p2c::check_tuv(0.0, tolerance, 1.0, max_itr)) return rc;
-  if (n <= 0) return P2C_OK;
-  if (grid_blocks <= 0) return p2c::set_error(P2C_EINVAL, "grid_blocks must be positive");
-  if (light_budget <= 0) return p2c::set_error(P2C_EINVAL, "light_budget must be positive");
-  if (refill_at < 1 || refill_at > 64) return p2c::set_error(P2C_EINVAL, "refill_at must be in 1..64");
-  if (handover_at < 0 || handover_at > 64) return p2c::set_error(P2C_EINVAL, "handover_at must be in 0..64");
-  if (light_budget > max_itr) light_budget = max_itr;
-  // counters: [0] next pair (u64), [1] replay count (u32) | heavy count (u32), [2] heavy cursor (u32)
-  auto* c64 = static_cast<unsigned long long*>(counters);
-  auto* c32 = static_cast<uint32_t*>(counters);
-  auto* st = static_cast<p2c::WalkState*>(states);
-  const auto* tp = static_cast<const p2c::TuvPair*>(pairs);
-  hipStream_t s = p2c::as_stream(stream);
-  hipLaunchKernelGGL(p2c::tuv_stream_kernel, dim3(grid_blocks), dim3(p2c::kTuvBlock), 0, s, tp, n, totals, tolerance,
-                     (uint64_t)light_budget, (uint32_t)refill_at, (uint32_t)handover_at, out, c64, st, heavy_list,
-                     c32 + 3, list_cap);
-  P2C_HIP_CHECK(hipGetLastError());
-  return P2C_OK;
+  return p2c::launch_stream(p2c::FullSrc{static_cast<const p2c::TuvPair*>(pairs)}, n, totals, tolerance, max_itr,
+                            light_budget, refill_at, handover_at, out, counters, states, heavy_list, list_cap,
+                            grid_blocks, stream);
 }
 
 int p2c_tuv_wave(const void* pairs, void* counters, const void* states, const uint32_t* heavy_list,
@@ -1281,53 +1509,62 @@ int p2c_tuv_wave_split(const void* pairs, void* counters, void* states, const ui
                        uint32_t list_cap, double tolerance, int64_t max_itr, p2c_tuv_result* out,
                        uint32_t* replay_list, int64_t split_at, void* tasks, uint32_t task_cap, uint32_t* split_list,
                        int grid_blocks, void* stream) {
-  if (int rc = p2c::check_tuv(0.0, tolerance, 1.0, max_itr)) return rc;
-  if (grid_blocks <= 0) return p2c::set_error(P2C_EINVAL, "grid_blocks must be positive");
-  if (split_at < 0) return p2c::set_error(P2C_EINVAL, "split_at must be non-negative");
-  const bool split = split_at > 0;
-  if (split && (tasks == nullptr || task_cap == 0 || split_list == nullptr))
-    return p2c::set_error(P2C_EINVAL, "splitting needs a task buffer and a split list");
-  auto* c32 = static_cast<uint32_t*>(counters);
-  hipStream_t s = p2c::as_stream(stream);
-  const auto* tp = static_cast<const p2c::TuvPair*>(pairs);
-  p2c::SplitOut so{};
-  if (split) {
-    // counters (u32): [13] reserved task slots, [14] task cursor, [15] split pairs
-    so.split_at = (uint64_t)split_at;
-    so.tasks = static_cast<p2c::SplitTask*>(tasks);
-    so.task_cap = task_cap;
-    so.task_count = c32 + 13;
-    so.split_list = split_list;
-    so.split_count = c32 + 15;
-    so.acc = static_cast<p2c::SplitAcc*>(states);
-  }
-  const auto* resume = static_cast<const p2c::WalkState*>(states);
-  if (split)
-    hipLaunchKernelGGL(p2c::tuv_heavy_kernel<true>, dim3(grid_blocks), dim3(p2c::kTuvBlock), 0, s, tp, heavy_list,
-                       c32 + 3, list_cap, c32 + 4, resume, tolerance, (uint64_t)max_itr, out, replay_list, c32 + 2, so);
-  else
-    hipLaunchKernelGGL(p2c::tuv_heavy_kernel<false>, dim3(grid_blocks), dim3(p2c::kTuvBlock), 0, s, tp, heavy_list,
-                       c32 + 3, list_cap, c32 + 4, resume, tolerance, (uint64_t)max_itr, out, replay_list, c32 + 2, so);
-  if (split) {
-    hipLaunchKernelGGL(p2c::tuv_task_kernel, dim3(grid_blocks), dim3(p2c::kTuvBlock), 0, s, tp, c32 + 14, tolerance,
-                       (uint64_t)max_itr, so);
-    hipLaunchKernelGGL(p2c::tuv_split_finish_kernel, dim3(16), dim3(p2c::kTuvBlock), 0, s, so, (uint64_t)max_itr, out,
-                       replay_list, c32 + 2, list_cap);
-  }
-  P2C_HIP_CHECK(hipGetLastError());
-  return P2C_OK;
+  return p2c::launch_wave_split(p2c::FullSrc{static_cast<const p2c::TuvPair*>(pairs)}, counters, states, heavy_list,
+                                list_cap, tolerance, max_itr, out, replay_list, split_at, tasks, task_cap, split_list,
+                                grid_blocks, stream);
 }
 
 int p2c_tuv_replay(const void* pairs, const uint32_t* replay_list, const uint32_t* replay_count,
                    uint32_t replay_cap, double tolerance, int64_t max_itr, void* heaps, int64_t heap_cap,
                    int64_t slots, p2c_tuv_result* out, void* stream) {
-  if (int rc = p2c::check_tuv(0.0, tolerance, 1.0, max_itr)) return rc;
-  if (heap_cap < 8 || slots <= 0) return p2c::set_error(P2C_EINVAL, "replay needs heap_cap >= 8 and slots > 0");
-  hipLaunchKernelGGL(p2c::tuv_replay_kernel, dim3((unsigned)((slots + 63) / 64)), dim3(64), 0,
-                     p2c::as_stream(stream), static_cast<const p2c::TuvPair*>(pairs), replay_list, replay_count,
-                     replay_cap, tolerance, max_itr, static_cast<p2c::HeapBox*>(heaps), heap_cap, slots, out);
+  return p2c::launch_replay(p2c::FullSrc{static_cast<const p2c::TuvPair*>(pairs)}, replay_list, replay_count,
+                            replay_cap, tolerance, max_itr, heaps, heap_cap, slots, out, stream);
+}
+
+size_t p2c_tuv_lite_bytes(int64_t n) {
+  if (n < 0) n = 0;
+  return size_t((n + p2c::kPairTile - 1) / p2c::kPairTile) * p2c::kPairTile * sizeof(p2c::TuvLite);
+}
+
+int p2c_tuv_setup_pairs_lite(int is_ee, const double* x0, const double* x1, const int32_t* topo,
+                             const int32_t* pairs, int64_t n, const uint64_t* totals, int64_t out_offset,
+                             int64_t out_cap, double ms, double tolerance, void* lite_pairs, void* stream) {
+  if (int rc = p2c::check_tuv(ms, tolerance, 1.0, 1)) return rc;
+  if (n <= 0) return P2C_OK;
+  hipLaunchKernelGGL(p2c::tuv_setup_lite_kernel, dim3(p2c::grid_for(n, p2c::kTuvBlock)), dim3(p2c::kTuvBlock), 0,
+                     p2c::as_stream(stream), is_ee, x0, x1, topo, pairs, n, totals, out_offset, out_cap, ms,
+                     tolerance, static_cast<p2c::TuvLite*>(lite_pairs));
   P2C_HIP_CHECK(hipGetLastError());
   return P2C_OK;
+}
+
+int p2c_tuv_stream_lite(const void* lite_pairs, const p2c_tuv_vertices* verts, int64_t n, const uint64_t* totals,
+                        double tolerance, int64_t max_itr, int64_t light_budget, int refill_at, int handover_at,
+                        p2c_tuv_result* out, void* counters, void* states, uint32_t* heavy_list, uint32_t list_cap,
+                        int grid_blocks, void* stream) {
+  p2c::LiteSrc src;
+  if (int rc = p2c::lite_src(lite_pairs, verts, &src)) return rc;
+  return p2c::launch_stream(src, n, totals, tolerance, max_itr, light_budget, refill_at, handover_at, out, counters,
+                            states, heavy_list, list_cap, grid_blocks, stream);
+}
+
+int p2c_tuv_wave_split_lite(const void* lite_pairs, const p2c_tuv_vertices* verts, void* counters, void* states,
+                            const uint32_t* heavy_list, uint32_t list_cap, double tolerance, int64_t max_itr,
+                            p2c_tuv_result* out, uint32_t* replay_list, int64_t split_at, void* tasks,
+                            uint32_t task_cap, uint32_t* split_list, int grid_blocks, void* stream) {
+  p2c::LiteSrc src;
+  if (int rc = p2c::lite_src(lite_pairs, verts, &src)) return rc;
+  return p2c::launch_wave_split(src, counters, states, heavy_list, list_cap, tolerance, max_itr, out, replay_list,
+                                split_at, tasks, task_cap, split_list, grid_blocks, stream);
+}
+
+int p2c_tuv_replay_lite(const void* lite_pairs, const p2c_tuv_vertices* verts, const uint32_t* replay_list,
+                        const uint32_t* replay_count, uint32_t replay_cap, double tolerance, int64_t max_itr,
+                        void* heaps, int64_t heap_cap, int64_t slots, p2c_tuv_result* out, void* stream) {
+  p2c::LiteSrc src;
+  if (int rc = p2c::lite_src(lite_pairs, verts, &src)) return rc;
+  return p2c::launch_replay(src, replay_list, replay_count, replay_cap, tolerance, max_itr, heaps, heap_cap, slots,
+                            out, stream);
 }
 
 size_t p2c_tuv_static_max_pairs(void) { return size_t(p2c::kStaticMaxPairs); }

@@ -254,8 +254,10 @@ P2C_TUV_HD void tuv_split_sequence(const double inv_tol[3], uint64_t split[5], u
   *done_level = level;
 }
 
-// Per-pair constants from the 8 positions (rows 0-3 at t=0, 4-7 at t=1).
-P2C_TUV_HD void tuv_setup(TuvPair* q, const double v[8][3], int is_ee, double ms, double tolerance) {
+// The positional constants of a pair from its 8 positions (rows 0-3 at
+// t=0, 4-7 at t=1): start positions, displacements and the per-axis error
+// bound.  Also what a compact (vertex-id) pair record is expanded with.
+P2C_TUV_HD void tuv_setup_geometry(TuvPair* q, const double v[8][3], int is_ee, double ms) {
   q->is_ee = (uint32_t)is_ee;
   for (int i = 0; i < 4; ++i)
     for (int k = 0; k < 3; ++k) {
@@ -270,6 +272,11 @@ P2C_TUV_HD void tuv_setup(TuvPair* q, const double v[8][3], int is_ee, double ms
     }
     q->err[k] = kTuvErrScale * m + kTuvErrFloor + ms;
   }
+}
+
+// Per-pair constants from the 8 positions (rows 0-3 at t=0, 4-7 at t=1).
+P2C_TUV_HD void tuv_setup(TuvPair* q, const double v[8][3], int is_ee, double ms, double tolerance) {
+  tuv_setup_geometry(q, v, is_ee, ms);
   // unit-cube corner values (the operations of tuv_corner_range, one corner
   // at a time) and the largest change along each axis
   double F[2][2][2][3];

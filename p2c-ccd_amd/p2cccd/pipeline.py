@@ -48,6 +48,9 @@ class CcdPipeline:
         self.cert = TuvCertifier(config, self.device)
         self.headroom = headroom
         self.batched_mode = "stream"
+        # batched stream form: 64-byte pair records expanded from the scene's
+        # vertex arrays (L2-resident) instead of 288-byte records
+        self.compact_records = True
         self.cap_vf = self.cap_ee = 0
         self.vf = self.ee = None
         self.graph: torch.cuda.CUDAGraph | None = None
@@ -93,8 +96,9 @@ class CcdPipeline:
         bp.build(self.x0, self.x1, self.thickness)
         bp.count()
         bp.emit(self.vf, self.ee)
-        c.setup_pairs(self.x0, self.x1, bp.faces, self.vf, False, self.cap_vf, totals=bp.totals)
-        c.setup_pairs(self.x0, self.x1, bp.edges, self.ee, True, self.cap_ee, totals=bp.totals)
+        lite = self.compact_records and self.batched_mode in ("stream", "plain")
+        c.setup_pairs(self.x0, self.x1, bp.faces, self.vf, False, self.cap_vf, totals=bp.totals, lite=lite)
+        c.setup_pairs(self.x0, self.x1, bp.edges, self.ee, True, self.cap_ee, totals=bp.totals, lite=lite)
         c.certify(self.cap_vf + self.cap_ee, self.batched_mode, totals=bp.totals, events=events)
 
     def _capture(self) -> None:

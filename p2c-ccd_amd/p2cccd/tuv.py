@@ -48,6 +48,17 @@ nat.lib.p2c_tuv_state_bytes.restype = nat.ctypes.c_size_t
 nat._sig("p2c_tuv_replay", [_vp, _vp, _vp, _u32, _f64, _i64, _vp, _i64, _i64, _vp, _vp])
 nat.lib.p2c_tuv_pair_bytes.restype = nat.ctypes.c_size_t
 nat._sig("p2c_tuv_certify_static", [_vp, _i64, _f64, _i64, _vp, _vp])
+nat._sig("p2c_tuv_setup_pairs_lite", [_i32, _vp, _vp, _vp, _vp, _i64, _vp, _i64, _i64, _f64, _f64, _vp, _vp])
+nat._sig("p2c_tuv_plain_lite", [_vp, _vp, _i64, _vp, _f64, _i64, _vp, _vp, _vp, _u32, _vp])
+nat._sig("p2c_tuv_stream_lite", [_vp, _vp, _i64, _vp, _f64, _i64, _i64, _i32, _i32, _vp, _vp, _vp, _vp, _u32, _i32,
+                                  _vp])
+nat._sig("p2c_tuv_wave_split_lite", [_vp, _vp, _vp, _vp, _vp, _u32, _f64, _i64, _vp, _vp, _i64, _vp, _u32, _vp, _i32,
+                                      _vp])
+nat._sig("p2c_tuv_replay_lite", [_vp, _vp, _vp, _vp, _u32, _f64, _i64, _vp, _i64, _i64, _vp, _vp])
+nat.lib.p2c_tuv_lite_bytes.argtypes = [nat._i64]
+nat.lib.p2c_tuv_lite_bytes.restype = nat.ctypes.c_size_t
+# p2c_tuv_vertices: the vertex arrays compact pair records are expanded from
+VERTICES_DTYPE = np.dtype([("x0", "<u8"), ("x1", "<u8"), ("ms", "<f8")])
 nat.lib.p2c_tuv_static_max_pairs.restype = nat.ctypes.c_size_t
 nat.lib.p2c_tuv_pairs_bytes.argtypes = [nat._i64]
 nat.lib.p2c_tuv_pairs_bytes.restype = nat.ctypes.c_size_t
@@ -87,13 +98,16 @@ class TuvCertifier:
         self.grid_blocks = int(props.multi_processor_count) * self.cfg.blocks_per_cu
         self.heaps = torch.empty(self.cfg.replay_slots * self.cfg.heap_cap * 32, dtype=torch.uint8,
                                  device=self.device)
-        # u32 view: [0:2] next pair, [2] replay count, [3] heavy count, [4] heavy cursor, [5:13] round survivors,
+        # u32 view: [0:2] next pair, [2] replay count, [3] heavy count, [4] heavy cursor, [5:13] round survivors
+        # (stream form: [6:8] the EE range cursor),
         # [13] split pieces, [14] piece cursor, [15] split pairs
         self.counters = torch.zeros(8, dtype=torch.int64, device=self.device)
         self.budgets = np.ascontiguousarray(self.cfg.round_budgets, dtype=np.uint32)
         self.state_bytes = int(nat.lib.p2c_tuv_state_bytes())
         self.tasks = torch.empty(max(self.cfg.split_tasks, 1) * int(nat.lib.p2c_tuv_task_bytes()), dtype=torch.uint8,
                                  device=self.device)
+        self.verts = np.zeros(1, VERTICES_DTYPE)  # set by setup_pairs(lite=True)
+        self.lite_active = False  # the last setup wrote compact records
 
     def reserve(self, n: int) -> None:
         self._reserve(n)
@@ -103,6 +117,7 @@ class TuvCertifier:
             return
         cap = (max(n, 1024) + 63) // 64 * 64  # pair records are stored in tiles of 64
         self.pairs = torch.empty(int(nat.lib.p2c_tuv_pairs_bytes(cap)), dtype=torch.uint8, device=self.device)
+        self.lite = torch.empty(int(nat.lib.p2c_tuv_lite_bytes(cap)), dtype=torch.uint8, device=self.device)
         self.results = torch.empty(cap * TUV_RESULT_DTYPE.itemsize, dtype=torch.uint8, device=self.device)
         self.replay_list = torch.empty(cap, dtype=torch.int32, device=self.device)
         self.heavy_list = torch.empty(cap, dtype=torch.int32, device=self.device)
@@ -116,6 +131,7 @@ class TuvCertifier:
         assert x.dtype == torch.float64 and x.is_cuda and x.shape[1:] == (8, 3) and x.is_contiguous()
         n = x.shape[0]
         self._reserve(n)
+        self.lite_active = False
         nat.call("p2c_tuv_setup_positions", int(is_ee), x.data_ptr(), n, self.cfg.ms, self.cfg.tolerance,
                  self.pairs.data_ptr(), nat.stream_handle(stream))
         return n
@@ -124,6 +140,7 @@ class TuvCertifier:
         assert x0.dtype == torch.float64 and idx4.dtype == torch.int32 and idx4.shape[1] == 4
         n = idx4.shape[0]
         self._reserve(n)
+        self.lite_active = False
         nat.call("p2c_tuv_setup_indexed", int(is_ee), x0.data_ptr(), x1.data_ptr(), idx4.data_ptr(), n,
                  self.cfg.ms, self.cfg.tolerance, self.pairs.data_ptr(), nat.stream_handle(stream))
         return n
@@ -136,6 +153,8 @@ class TuvCertifier:
         kernels (stream form: `mid` between the stream and the wave pass),
         before the replay."""
         s = nat.stream_handle(stream)
+        if self.lite_active and mode not in ("stream", "plain"):
+            raise ValueError("compact pair records are certified by the stream and plain forms only")
         c = self.counters
         c.zero_()
         if events is not None:
@@ -143,8 +162,12 @@ class TuvCertifier:
         rc_ptr = c.data_ptr() + 8  # replay count (uint32 in the low half of slot 1)
         tp = 0 if totals is None else totals.data_ptr()
         if mode == "plain":
-            nat.call("p2c_tuv_plain", self.pairs.data_ptr(), n, tp, self.cfg.tolerance, self.cfg.max_itr,
-                     self.results.data_ptr(), self.replay_list.data_ptr(), rc_ptr, self._cap, s)
+            plain_args = (n, tp, self.cfg.tolerance, self.cfg.max_itr, self.results.data_ptr(),
+                          self.replay_list.data_ptr(), rc_ptr, self._cap, s)
+            if self.lite_active:
+                nat.call("p2c_tuv_plain_lite", self.lite.data_ptr(), self.verts.ctypes.data, *plain_args)
+            else:
+                nat.call("p2c_tuv_plain", self.pairs.data_ptr(), *plain_args)
         elif mode == "rounds":
             nat.call("p2c_tuv_rounds", self.pairs.data_ptr(), n, tp, self.cfg.tolerance, self.cfg.max_itr,
                      self.budgets.ctypes.data, len(self.budgets), self.results.data_ptr(), c.data_ptr(),
@@ -156,15 +179,23 @@ class TuvCertifier:
             budget = self.cfg.light_budget
             if self.cfg.small_batch_budget > 0 and n * 8 <= self.grid_blocks * 256:
                 budget = min(budget, self.cfg.small_batch_budget)
-            nat.call("p2c_tuv_stream", self.pairs.data_ptr(), n, tp, self.cfg.tolerance, self.cfg.max_itr,
-                     budget, self.cfg.refill_at, self.cfg.handover_at, self.results.data_ptr(),
-                     c.data_ptr(), self.states.data_ptr(), self.heavy_list.data_ptr(), self._cap, self.grid_blocks, s)
+            stream_args = (n, tp, self.cfg.tolerance, self.cfg.max_itr, budget, self.cfg.refill_at,
+                           self.cfg.handover_at, self.results.data_ptr(), c.data_ptr(), self.states.data_ptr(),
+                           self.heavy_list.data_ptr(), self._cap, self.grid_blocks, s)
+            wave_args = (c.data_ptr(), self.states.data_ptr(), self.heavy_list.data_ptr(), self._cap,
+                         self.cfg.tolerance, self.cfg.max_itr, self.results.data_ptr(), self.replay_list.data_ptr(),
+                         self.cfg.split_at, self.tasks.data_ptr(), self.cfg.split_tasks, self.split_list.data_ptr(),
+                         self.grid_blocks, s)
+            if self.lite_active:
+                nat.call("p2c_tuv_stream_lite", self.lite.data_ptr(), self.verts.ctypes.data, *stream_args)
+            else:
+                nat.call("p2c_tuv_stream", self.pairs.data_ptr(), *stream_args)
             if events is not None and len(events) > 2:
                 events[2].record()
-            nat.call("p2c_tuv_wave_split", self.pairs.data_ptr(), c.data_ptr(), self.states.data_ptr(),
-                     self.heavy_list.data_ptr(), self._cap, self.cfg.tolerance, self.cfg.max_itr,
-                     self.results.data_ptr(), self.replay_list.data_ptr(), self.cfg.split_at, self.tasks.data_ptr(),
-                     self.cfg.split_tasks, self.split_list.data_ptr(), self.grid_blocks, s)
+            if self.lite_active:
+                nat.call("p2c_tuv_wave_split_lite", self.lite.data_ptr(), self.verts.ctypes.data, *wave_args)
+            else:
+                nat.call("p2c_tuv_wave_split", self.pairs.data_ptr(), *wave_args)
         elif mode == "persistent":
             nat.call("p2c_tuv_persistent", self.pairs.data_ptr(), n, tp, self.cfg.tolerance, self.cfg.max_itr,
                      self.cfg.light_budget, self.results.data_ptr(), c.data_ptr(), self.heavy_list.data_ptr(),
@@ -173,16 +204,29 @@ class TuvCertifier:
             raise ValueError(f"unknown certificate mode {mode!r}")
         if events is not None:
             events[1].record()
-        nat.call("p2c_tuv_replay", self.pairs.data_ptr(), self.replay_list.data_ptr(), rc_ptr, self._cap,
-                 self.cfg.tolerance, self.cfg.max_itr, self.heaps.data_ptr(), self.cfg.heap_cap,
-                 self.cfg.replay_slots, self.results.data_ptr(), s)
+        replay_args = (self.replay_list.data_ptr(), rc_ptr, self._cap, self.cfg.tolerance, self.cfg.max_itr,
+                       self.heaps.data_ptr(), self.cfg.heap_cap, self.cfg.replay_slots, self.results.data_ptr(), s)
+        if self.lite_active:
+            nat.call("p2c_tuv_replay_lite", self.lite.data_ptr(), self.verts.ctypes.data, *replay_args)
+        else:
+            nat.call("p2c_tuv_replay", self.pairs.data_ptr(), *replay_args)
 
     def setup_pairs(self, x0: torch.Tensor, x1: torch.Tensor, topo: torch.Tensor, pairs: torch.Tensor, is_ee: bool,
-                    n: int, totals: torch.Tensor | None = None, out_offset: int = 0, stream=None) -> None:
-        """Pair records straight from broad-phase pair lists (see p2c_tuv_setup_pairs)."""
-        nat.call("p2c_tuv_setup_pairs", int(is_ee), x0.data_ptr(), x1.data_ptr(), topo.data_ptr(), pairs.data_ptr(),
-                 n, 0 if totals is None else totals.data_ptr(), out_offset, self._cap, self.cfg.ms,
-                 self.cfg.tolerance, self.pairs.data_ptr(), nat.stream_handle(stream))
+                    n: int, totals: torch.Tensor | None = None, out_offset: int = 0, stream=None,
+                    lite: bool = False) -> None:
+        """Pair records straight from broad-phase pair lists (see
+        p2c_tuv_setup_pairs).  lite: 64-byte records expanded from x0 and x1
+        when loaded (p2c_tuv_setup_pairs_lite); every pair of the batch must
+        then index the same x0 and x1, and the batch is certified by the
+        stream form."""
+        args = (int(is_ee), x0.data_ptr(), x1.data_ptr(), topo.data_ptr(), pairs.data_ptr(), n,
+                0 if totals is None else totals.data_ptr(), out_offset, self._cap, self.cfg.ms, self.cfg.tolerance)
+        if lite:
+            self.verts["x0"], self.verts["x1"], self.verts["ms"] = x0.data_ptr(), x1.data_ptr(), self.cfg.ms
+            nat.call("p2c_tuv_setup_pairs_lite", *args, self.lite.data_ptr(), nat.stream_handle(stream))
+        else:
+            nat.call("p2c_tuv_setup_pairs", *args, self.pairs.data_ptr(), nat.stream_handle(stream))
+        self.lite_active = lite
 
     def replay_count(self) -> int:
         return int(self.counters[1].item() & 0xFFFFFFFF)

@@ -65,3 +65,32 @@ def test_two_icospheres_level6_matches_oracle(gpu):
     r = p.results()
     assert r["vf_res"]["hit"].sum() > 0
     _check_against_oracle(s, r)
+
+
+@pytest.mark.parametrize("make", [
+    lambda: scenes.two_icospheres(level=3, gap=-0.01),
+    lambda: scenes.cloth_on_sphere(n=64, sphere_level=3, cloth_size=1.5),
+])
+@pytest.mark.parametrize("mode", ["stream", "plain"])
+def test_compact_records_equal_full_records(gpu, make, mode):
+    """64-byte pair records expanded from the vertex arrays give every pair
+    the result of the 288-byte records, box count and flags included (the
+    walk is the same), in the stream form with its wave pass and replay and
+    in the plain form."""
+    s = make()
+    out = {}
+    for compact in (False, True):
+        p = CcdPipeline(s)
+        p.compact_records = compact
+        p.batched_mode = mode
+        p.step_batched(use_graph=False)
+        p.step_batched(use_graph=True)
+        r = p.results()
+        out[compact] = r
+    for kind in ("vf", "ee"):
+        a = _sorted(out[False][kind], out[False][kind + "_res"])
+        b = _sorted(out[True][kind], out[True][kind + "_res"])
+        assert np.array_equal(a[0], b[0]), kind
+        for f in ("hit", "toi", "boxes", "flags"):
+            assert np.array_equal(a[1][f], b[1][f]), (kind, f)
+    _check_against_oracle(s, out[True])
