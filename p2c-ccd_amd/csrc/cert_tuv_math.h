@@ -198,21 +198,42 @@ P2C_TUV_HD void tuv_split_sequence(const double inv_tol[3], uint64_t split[5], u
     if (before(o1, o0)) { const int t = o0; o0 = o1; o1 = t; }
     if (before(o2, o1)) { const int t = o1; o1 = o2; o2 = t; }
     if (before(o1, o0)) { const int t = o0; o0 = o1; o1 = t; }
-    const int ord[3] = {o0, o1, o2};
+    // dimension k splits in rounds [E_k - n_k + 1, E_k]; a round's codes
+    // (the active dimensions in `ord`) are appended to the accumulator at
+    // once, and a full 64-bit word is flushed to w0..w4
+    const int k0 = o0, k1 = o1, k2 = o2;
+    const int hi0 = get3(E, k0), lo0 = hi0 - get3(n, k0) + 1;
+    const int hi1 = get3(E, k1), lo1 = hi1 - get3(n, k1) + 1;
+    const int hi2 = get3(E, k2), lo2 = hi2 - get3(n, k2) + 1;
     int vmax = -100000, vmin = 100000;
     for (int k = 0; k < 3; ++k) {
       if (n[k] == 0) continue;
       vmax = E[k] > vmax ? E[k] : vmax;
       vmin = E[k] - n[k] + 1 < vmin ? E[k] - n[k] + 1 : vmin;
     }
+    uint64_t acc = 0;  // codes of levels [64 * flushed / 2, level) not yet flushed
+    uint32_t bits = 0, flushed = 0;
     for (int v = vmax; v >= vmin; --v) {
-#pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        const int k = ord[i];
-        const int j = get3(E, k) - v;
-        if (j >= 0 && j < get3(n, k)) split_append(cur, level, k, w0, w1, w2, w3, w4);
+      uint64_t pat = 0;
+      uint32_t pb = 0;
+      if (v >= lo0 && v <= hi0) { pat |= (uint64_t)k0 << pb; pb += 2; }
+      if (v >= lo1 && v <= hi1) { pat |= (uint64_t)k1 << pb; pb += 2; }
+      if (v >= lo2 && v <= hi2) { pat |= (uint64_t)k2 << pb; pb += 2; }
+      acc |= pat << bits;
+      bits += pb;
+      if (bits >= 64) {
+        w0 = flushed == 0 ? acc : w0;
+        w1 = flushed == 1 ? acc : w1;
+        w2 = flushed == 2 ? acc : w2;
+        w3 = flushed == 3 ? acc : w3;
+        w4 = flushed == 4 ? acc : w4;
+        ++flushed;
+        bits -= 64;
+        acc = bits > 0 ? pat >> (pb - bits) : 0ull;
       }
     }
+    level = flushed * 32 + bits / 2;
+    cur = acc | (bits >= 64 ? 0ull : (~0ull << bits));  // unused levels read 3 (no split)
   } else {
     double r0 = inv_tol[0], r1 = inv_tol[1], r2 = inv_tol[2];
     int e0 = 0, e1 = 0, e2 = 0;
