@@ -90,10 +90,57 @@ def stream_roofline(pipe, steps: int):
              "flops_per_launch": flops, "boxes_per_launch": stream_vf + stream_ee}, stage_ms, hc)
 
 
+def broadphase_roofline(pipe, steps: int) -> dict:
+    """The broad phase (build, count, emit) against HBM: algorithmic bytes of
+    one step over its event-timed duration.  Bytes per step, with V vertices,
+    F faces, E edges, B = V + F + E swept boxes and C candidate pairs:
+      V * 48 (x0, x1 read) + B * 48 (boxes written) + B * 8 * 2 * 4 (u32 key
+      and value read and written by each of 4 sort passes) + (2F - 1 + 2E - 1)
+      * 64 (nodes written) + visited * 64 (one node read per visit of the
+      count and emit walks, counted by a host replay of the walks) + C * 8
+      (pairs written)."""
+    import numpy as np
+    import torch
+
+    bp = pipe.bp
+    ms = []
+    for _ in range(steps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        bp.build(pipe.x0, pipe.x1, pipe.thickness)
+        bp.count()
+        bp.emit(pipe.vf, pipe.ee)
+        e1.record()
+        torch.cuda.synchronize(pipe.device)
+        ms.append(e0.elapsed_time(e1))
+    visits = bp.traversal_visits()
+    nv, nf, ne = bp.n_vertices, bp.n_faces, bp.n_edges
+    n_vf, n_ee, _ = (int(v) for v in bp.totals.cpu())
+    boxes = nv + nf + ne
+    walked = sum(int(v) for v in visits.values())
+    traffic = (nv * 48 + boxes * 48 + boxes * 8 * 2 * 4 + (max(2 * nf - 1, 0) + max(2 * ne - 1, 0)) * 64
+               + walked * 64 + (n_vf + n_ee) * 8)
+    avg = float(np.mean(ms))
+    achieved = traffic / (avg * 1e-3) / 1e9
+    return {"bound": "hbm", "stage": "broad phase (build + count + emit)", "achieved": achieved,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "avg_ms": avg, "visited_nodes": {k: int(v) for k, v in visits.items()}, "candidates": n_vf + n_ee}
+
+
 def percentile(v, q):
     import numpy as np
 
     return float(np.percentile(np.asarray(v), q))
+
+
+def _cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(scene, sample_pairs: int, seed: int = 7) -> dict:
@@ -112,6 +159,7 @@ def cpu_baseline(scene, sample_pairs: int, seed: int = 7) -> dict:
     rng = np.random.default_rng(seed)
     t_cert = 0.0
     sampled = 0
+    samples = []
     for kind in ("vf", "ee"):
         pairs = cand[kind]
         k = int(round(sample_pairs * len(pairs) / max(n_total, 1)))
@@ -123,11 +171,20 @@ def cpu_baseline(scene, sample_pairs: int, seed: int = 7) -> dict:
         else:
             vid = np.concatenate([t.edges[sel[:, 0]], t.edges[sel[:, 1]]], axis=1)
         x = np.concatenate([scene.x0[vid], scene.x1[vid]], axis=1)
+        scale = len(pairs) / max(len(idx), 1)
+        samples.append((kind, x, scale))
         t1 = time.perf_counter()
         oracle.tuv_cert(kind == "ee", x, threads=1)
-        t_cert += (time.perf_counter() - t1) * len(pairs) / max(len(idx), 1)
+        t_cert += (time.perf_counter() - t1) * scale
         sampled += len(idx)
     step_s = t_sweep + t_cert
+    # the same sample on every host core (OpenMP over the sampled candidates; the sweep stays one thread)
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    t_all = 0.0
+    for kind, x, scale in samples:
+        t1 = time.perf_counter()
+        oracle.tuv_cert(kind == "ee", x, threads=cores)
+        t_all += (time.perf_counter() - t1) * scale
     return {
         "value": n_total / step_s,
         "unit": "pairs/s",
@@ -135,6 +192,9 @@ def cpu_baseline(scene, sample_pairs: int, seed: int = 7) -> dict:
         "kind": "port",
         "sample": f"full sort-and-sweep of the scene ({n_total} candidates, {t_sweep:.1f} s) + certificate on "
                   f"{sampled} uniformly sampled candidates scaled to all ({t_cert:.1f} s est.), 1 thread",
+        "all_cores": {"value": n_total / (t_sweep + t_all), "cores": cores, "nproc": os.cpu_count(),
+                      "cpu_model": _cpu_model(),
+                      "sample": f"same sample, certificate on {cores} threads ({t_all:.1f} s est.), sweep 1 thread"},
     }
 
 
@@ -502,6 +562,10 @@ def main() -> int:
     log(f"stream kernel {rl['avg_ms']:.2f} ms ({rl['boxes_per_launch']} boxes), certificate stage "
         f"{np.mean(stage_ms):.2f} ms, wave pass took {hc} pairs")
 
+    bp_rl = broadphase_roofline(pipe, args.steps)
+    log(f"broad phase {bp_rl['avg_ms']:.2f} ms, {bp_rl['traffic'] / 1e9:.2f} GB algorithmic, "
+        f"{bp_rl['achieved']:.0f} GB/s")
+
     # plain one-launch-per-stage form, same scene
     log("plain form")
     pipe.step_plain()
@@ -546,9 +610,10 @@ def main() -> int:
         "roofline": dict(rl, traffic=profiled_traffic("tuv_stream_kernel") if args.workload == "cloth" else None,
                          traffic_source="rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE "
                                         "(profiles/hbm_traffic_cloth256_step.md)"),
+        "broad_phase_roofline": bp_rl,
     }
     if not args.no_cpu_baseline:
-        log("cpu baseline (oracle, 1 thread)")
+        log("cpu baseline (oracle, 1 thread, then every host core)")
         line["cpu_baseline"] = cpu_baseline(scene, args.cpu_sample)
     else:
         line["cpu_baseline"] = None

@@ -470,6 +470,24 @@ typedef struct p2c_bp_mesh {
 } p2c_bp_mesh;
 
 size_t p2c_bp_workspace_bytes(int64_t n_vertices, int64_t n_faces, int64_t n_edges, int32_t n_objects);
+/* Byte offsets, within a workspace of these sizes, of what the build and
+ * count passes leave there (for inspection and measurement): the face and
+ * edge trees (2n - 1 nodes of 64 bytes: box min[3], max[3] as doubles, then
+ * int32 left, right, escape, prim; internal nodes at [0, n - 1), leaves in
+ * tree order at [n - 1, 2n - 1); a walk goes to `left` when a node's box
+ * overlaps the query and it has children, else to `escape`, -1 ending it),
+ * the vertex and edge query boxes (min[3], max[3] doubles), the vertex and
+ * edge query orders (u32; query s is vertex qorder[s] / edge eorder[s]), the
+ * per-query candidate counts (u32, vertex queries then edge queries) and
+ * the nodes at which the emit pass resumes a query with more candidates
+ * than the hit cache holds (int32, same order).  A vertex query walks the
+ * face tree from node 0; edge query s walks the edge tree from the escape
+ * of leaf s (each unordered pair once).  No counterpart in the reference. */
+typedef struct p2c_bp_layout {
+  size_t fnodes, enodes, vbox, ebox, qorder, eorder, counts, resume;
+} p2c_bp_layout;
+int p2c_bp_workspace_layout(int64_t n_vertices, int64_t n_faces, int64_t n_edges, int32_t n_objects,
+                            p2c_bp_layout* out);
 /* Swept boxes (FP64, +/- thickness), object envelopes, both LBVHs (faces for
  * VF, edges for EE) and the vertex query order.  x0, x1: device (V, 3). */
 int p2c_bp_build(const p2c_bp_mesh* mesh, const double* x0, const double* x1, double thickness, void* workspace,

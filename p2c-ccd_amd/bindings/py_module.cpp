@@ -15,6 +15,7 @@ using namespace p2cccd;
 
 void BindPolicy(py::module_& m);    // py_policy.cpp
 void BindDatasets(py::module_& m);  // datasets_io.cpp
+void BindReplay(py::module_& m);    // bp_replay.cpp
 
 namespace {
 
@@ -591,6 +592,7 @@ PYBIND11_MODULE(_p2cccd_cpp, m) {
         py::arg("scene"), py::arg("raw_buffer"));
   BindPolicy(m);
   BindDatasets(m);
+  BindReplay(m);
   m.def("audit_log_rows_for_query",
         [](const std::vector<AuditLogRow>& rows, std::uint64_t query_id) {
           std::vector<AuditLogRow> out;

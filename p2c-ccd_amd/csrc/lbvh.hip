@@ -825,6 +825,14 @@ size_t p2c_bp_workspace_bytes(int64_t nv, int64_t nf, int64_t ne, int32_t nobj) 
   return p2c::make_layout(nv, nf, ne, nobj).total;
 }
 
+int p2c_bp_workspace_layout(int64_t nv, int64_t nf, int64_t ne, int32_t nobj, p2c_bp_layout* out) {
+  if (out == nullptr) return p2c::set_error(P2C_EINVAL, "layout output is required");
+  if (nv < 0 || nf < 0 || ne < 0 || nobj < 1) return p2c::set_error(P2C_EINVAL, "mesh sizes must be non-negative");
+  const p2c::Layout L = p2c::make_layout(nv, nf, ne, nobj);
+  *out = p2c_bp_layout{L.fnodes, L.enodes, L.vbox, L.ebox, L.qorder, L.eorder, L.counts, L.resume};
+  return P2C_OK;
+}
+
 int p2c_bp_build(const p2c_bp_mesh* m, const double* x0, const double* x1, double thickness, void* ws,
                  size_t ws_bytes, void* stream) {
   using namespace p2c;

@@ -39,6 +39,8 @@ nat.lib.p2c_bp_workspace_bytes.restype = ctypes.c_size_t
 nat._sig("p2c_bp_build", [_vp, _vp, _vp, nat._f64, _vp, ctypes.c_size_t, _vp])
 nat._sig("p2c_bp_count", [_vp, _vp, _vp, _vp])
 nat._sig("p2c_bp_emit", [_vp, _vp, _vp, _i64, _vp, _i64, _vp])
+nat._sig("p2c_bp_workspace_layout", [_i64, _i64, _i64, _i32, _vp])
+_LAYOUT_FIELDS = ("fnodes", "enodes", "vbox", "ebox", "qorder", "eorder", "counts", "resume")
 
 
 DEFAULT_CACHE_SLOTS = 256  # partners per query kept by the count pass (covers nearly all queries of dense scenes)
@@ -66,6 +68,8 @@ class BroadPhase:
                                                            topo.object_count))
         self.workspace = torch.empty(self.ws_bytes, dtype=torch.uint8, device=dev)
         self.totals = torch.zeros(3, dtype=torch.int64, device=dev)
+        self.n_objects = topo.object_count
+        self.vertex_object = np.ascontiguousarray(topo.vertex_object, np.int32)
 
     @staticmethod
     def workspace_bytes(n_vertices: int, n_faces: int, n_edges: int, n_objects: int,
@@ -103,3 +107,24 @@ class BroadPhase:
         ee = torch.empty((n_ee, 2), dtype=torch.int32, device=self.device)
         self.emit(vf, ee)
         return vf, ee
+
+    # -- measurement ----------------------------------------------------------
+    def traversal_visits(self) -> dict:
+        """Nodes the last count and emit passes visited, replayed on the host
+        (p2cccd_cpp.lbvh_traversal_visits) from the trees, query data, counts
+        and resume nodes in the workspace (p2c_bp_workspace_layout)."""
+        import p2cccd_cpp
+
+        lay = np.zeros(len(_LAYOUT_FIELDS), np.uint64)
+        nat.call("p2c_bp_workspace_layout", self.n_vertices, self.n_faces, self.n_edges, self.n_objects,
+                 lay.ctypes.data)
+        off = dict(zip(_LAYOUT_FIELDS, (int(v) for v in lay)))
+        nv, nf, ne = self.n_vertices, self.n_faces, self.n_edges
+        sizes = dict(fnodes=64 * max(2 * nf - 1, 0), enodes=64 * max(2 * ne - 1, 0), vbox=48 * nv, ebox=48 * ne,
+                     qorder=4 * nv, eorder=4 * ne, counts=4 * (nv + ne), resume=4 * (nv + ne))
+        torch.cuda.synchronize(self.device)
+        host = {k: self.workspace[off[k]:off[k] + sizes[k]].cpu().numpy() for k in _LAYOUT_FIELDS}
+        return p2cccd_cpp.lbvh_traversal_visits(host["fnodes"], nf, host["enodes"], ne, host["vbox"], host["ebox"],
+                                                host["qorder"], host["eorder"], self.vertex_object, host["counts"],
+                                                host["resume"], nv, self.cache_slots)
+

@@ -134,3 +134,60 @@ def test_disjoint_envelopes_exclude_object_pairs(gpu, tmp_path):
     pairs_ee = {tuple(sorted((int(eo[a]), int(eo[b])))) for a, b in got["ee"]}
     assert (0, 1) in pairs_vf and (1, 2) in pairs_vf and (0, 2) not in pairs_vf
     assert (0, 1) in pairs_ee and (0, 2) not in pairs_ee
+
+
+def test_traversal_visit_replay_matches_a_python_walk(gpu):
+    """The host replay of the count and emit walks (used for the broad-phase
+    roofline) against a plain Python walk of the same exported trees: the
+    count walk of each query from its start node, and the emit walk of the
+    queries with more candidates than the hit cache from their resume node."""
+    from p2cccd.broadphase import BroadPhase, _LAYOUT_FIELDS
+    from p2cccd import _native as nat
+
+    s = scenes.two_icospheres(level=2, gap=-0.02, thickness=1e-3)
+    for slots in (0, 2):
+        bp = BroadPhase.for_scene(s, cache_slots=slots)
+        x0 = torch.from_numpy(s.x0).cuda()
+        x1 = torch.from_numpy(s.x1).cuda()
+        bp.candidates(x0, x1, s.thickness)
+        got = bp.traversal_visits()
+        lay = np.zeros(len(_LAYOUT_FIELDS), np.uint64)
+        nat.call("p2c_bp_workspace_layout", bp.n_vertices, bp.n_faces, bp.n_edges, bp.n_objects, lay.ctypes.data)
+        off = dict(zip(_LAYOUT_FIELDS, (int(v) for v in lay)))
+        ws = bp.workspace.cpu().numpy()
+        nv, nf, ne = bp.n_vertices, bp.n_faces, bp.n_edges
+        node_t = np.dtype([("mn", "<f8", 3), ("mx", "<f8", 3), ("left", "<i4"), ("right", "<i4"), ("escape", "<i4"),
+                           ("prim", "<i4")])
+        F = ws[off["fnodes"]:off["fnodes"] + 64 * (2 * nf - 1)].view(node_t)
+        E = ws[off["enodes"]:off["enodes"] + 64 * (2 * ne - 1)].view(node_t)
+        VB = ws[off["vbox"]:off["vbox"] + 48 * nv].view("<f8").reshape(nv, 6)
+        EB = ws[off["ebox"]:off["ebox"] + 48 * ne].view("<f8").reshape(ne, 6)
+        QO = ws[off["qorder"]:off["qorder"] + 4 * nv].view("<u4")
+        EO = ws[off["eorder"]:off["eorder"] + 4 * ne].view("<u4")
+        C = ws[off["counts"]:off["counts"] + 4 * (nv + ne)].view("<u4")
+        R = ws[off["resume"]:off["resume"] + 4 * (nv + ne)].view("<i4")
+
+        def walk(T, start, q):
+            n, node = 0, int(start)
+            while node >= 0:
+                t = T[node]
+                n += 1
+                hit = (q[:3] <= t["mx"]).all() and (t["mn"] <= q[3:]).all()
+                node = int(t["left"]) if hit and t["left"] >= 0 else int(t["escape"])
+            return n
+
+        want = dict(vf_count_visits=0, vf_emit_visits=0, ee_count_visits=0, ee_emit_visits=0)
+        for q in range(nv):
+            v = QO[q]
+            if s.topo.vertex_object[v] < 0:
+                continue
+            want["vf_count_visits"] += walk(F, 0, VB[v])
+            if C[q] > slots:
+                want["vf_emit_visits"] += walk(F, R[q] if slots else 0, VB[v])
+        for e in range(ne):
+            start = E[ne - 1 + e]["escape"]
+            want["ee_count_visits"] += walk(E, start, EB[EO[e]])
+            if C[nv + e] > slots:
+                want["ee_emit_visits"] += walk(E, R[nv + e] if slots else start, EB[EO[e]])
+        assert got == want, (slots, got, want)
+        assert got["vf_count_visits"] > nv and got["vf_emit_visits"] <= got["vf_count_visits"]
