@@ -319,3 +319,23 @@ def test_external_ccd_query_module(gpu):
     assert r[4].toi_upper == 0.5
     with pytest.raises(ValueError, match="eps_time"):
         pe.evaluate_external_ccd_query(vf(1.0, -1.0, 1), pe.PureExactCPUConfig(eps_time=0.0))
+
+
+def test_python_twin_rules_closed_forms_on_device(gpu):
+    # The two Python-twin rules on the device walk (P2C_CERT_PYTHON_TWIN),
+    # against closed forms: the segment-segment guard abs(denom) > 1e-14
+    # (nearly parallel crossing segments: the s = 0 branch's distance
+    # 1.000297e-3, not the true 1e-3) and the motion radius 0.5 * (t1 - t0) *
+    # (point + largest triangle-vertex displacement): margin 2.1 - 0.4 - eps.
+    import torch
+    from test_oracle_ref_cert import _ee_crossing, _pt_radius_case, ee_guard_closed_form
+
+    cfg = _native.cert_config(1e-5, 1e-8, 32)
+    cfg["flags"] = 2
+    p, (L, h, z) = _ee_crossing()
+    res, bad = _device_batch(torch, 2, p.view(_native.PRIM_DTYPE), cfg=cfg)
+    assert bad == 0 and res["status"][0] == 1
+    assert abs(res["safe_margin_lb"][0] + 1e-8 - ee_guard_closed_form(L, h, z)) <= 1e-15
+    res, bad = _device_batch(torch, 1, _pt_radius_case().view(_native.PRIM_DTYPE), cfg=cfg)
+    assert bad == 0 and res["status"][0] == 1 and res["nodes"][0] == 1
+    assert abs(res["safe_margin_lb"][0] - (2.1 - 0.4 - 1e-8)) <= 4e-16

@@ -228,3 +228,74 @@ def test_python_twin_agrees_with_engine_decisions():
         assert np.mean(e["status"] == y["status"]) > 0.98
         both = (e["status"] == 0) & (y["status"] == 0)
         assert (y["toi_upper"][both] >= e["interval_t0"][both]).all()
+
+
+def _ee_crossing():
+    """Static nearly parallel segments of length L = 2^-10 at height z = 1e-3
+    whose projections cross at their midpoints, at angle h = 0.05: the
+    denominator a e - b^2 = L^4 h^2 (2.27e-15) is below the twin's guard."""
+    L, h, z = 2.0 ** -10, 0.05, 1e-3
+    x = np.zeros((1, 8, 3))
+    x[0, 0], x[0, 1] = (0, 0, 0), (L, 0, 0)
+    x[0, 2], x[0, 3] = (0, -h * L / 2, z), (L, h * L / 2, z)
+    x[0, 4:] = x[0, :4]
+    return oracle.prims_from_positions(x), (L, h, z)
+
+
+def ee_guard_closed_form(L, h, z) -> float:
+    """Distance the twin's segment-segment rule gives when abs(denom) <= 1e-14
+    (pure_exact_cpu.py:126): s = 0, t = f / e, the closest points p1 and
+    p2 + t d2; in exact rational arithmetic, then a 50-digit square root."""
+    import decimal
+    from fractions import Fraction as Fr
+
+    Lq, hq, zq = Fr(L), Fr(h), Fr(z)
+    d2, r = (Lq, hq * Lq, Fr(0)), (Fr(0), hq * Lq / 2, -zq)
+    t = sum(u * v for u, v in zip(d2, r)) / sum(c * c for c in d2)
+    q = (t * Lq, -hq * Lq / 2 + t * hq * Lq, zq)
+    d2sum = sum(c * c for c in q)
+    decimal.getcontext().prec = 50
+    return float(decimal.Decimal(d2sum.numerator).sqrt() / decimal.Decimal(d2sum.denominator).sqrt())
+
+
+def test_python_twin_ee_guard_closed_form():
+    # With the twin's guard abs(denom) > 1e-14 the crossing pair's distance is
+    # the s = 0 branch's, 1.000297e-3; the engine's guard (denom != 0) solves
+    # for the crossing and gets the true 1e-3.  Static pair: separation at the
+    # root with margin = distance - eps_space.
+    p, (L, h, z) = _ee_crossing()
+    want = ee_guard_closed_form(L, h, z)
+    tw = oracle.ref_cert_py(2, p, eps_time=1e-5, eps_space=1e-8, max_depth=32)[0]
+    en = oracle.ref_cert(2, p, eps_time=1e-5, eps_space=1e-8)[0]
+    assert tw["status"] == 1 and en["status"] == 1
+    assert abs((tw["safe_margin_lb"] + 1e-8) - want) <= 1e-15
+    assert abs(en["safe_margin_lb"] + 1e-8 - z) <= 1e-15 and abs(want - z) > 2e-7
+
+
+def _pt_radius_case():
+    """A point moving 0.6 along x at height 2 above the unit triangle, which
+    drops 0.2 along z: distance at t = 0.5 is 2.1; point displacement 0.6,
+    every triangle vertex 0.2."""
+    x = np.zeros((1, 8, 3))
+    x[0, 0], x[0, 4] = (0.1, 0.1, 2.0), (0.7, 0.1, 2.0)
+    for k, v in ((1, (0, 0, 0)), (2, (1, 0, 0)), (3, (0, 1, 0))):
+        x[0, k] = v
+        x[0, 4 + k] = (v[0], v[1], -0.2)
+    return oracle.prims_from_positions(x)
+
+
+def test_python_twin_motion_radius_closed_form():
+    # The twin's radius is 0.5 * (t1 - t0) * (point displacement + the largest
+    # triangle-vertex displacement), the norms taken once over the whole step
+    # (pure_exact_cpu.py:259-262): at the root 0.5 * (0.6 + 0.2) = 0.4, so
+    # the margin is 2.1 - 0.4 - eps_space.  A sum over the triangle's
+    # vertices (1.0 in place of 0.2) or the full length (no 0.5) would give
+    # 1.5 or 1.3.  On dyadic intervals (every node of a [0, 1] walk) the
+    # engine's per-vertex product 0.5 * len * |d| equals it bit for bit:
+    # scaling by a power of two is exact, so both margins agree.
+    p = _pt_radius_case()
+    tw = oracle.ref_cert_py(1, p, eps_time=1e-5, eps_space=1e-8, max_depth=32)[0]
+    en = oracle.ref_cert(1, p, eps_time=1e-5, eps_space=1e-8)[0]
+    assert tw["status"] == 1 and tw["nodes"] == 1
+    assert abs(tw["safe_margin_lb"] - (2.1 - 0.4 - 1e-8)) <= 4e-16
+    assert tw["safe_margin_lb"] == en["safe_margin_lb"]
