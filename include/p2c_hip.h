@@ -410,6 +410,23 @@ size_t p2c_tuv_lite_bytes(int64_t n);
 int p2c_tuv_setup_pairs_lite(int is_ee, const double* x0, const double* x1, const int32_t* topo,
                              const int32_t* pairs, int64_t n, const uint64_t* totals, int64_t out_offset,
                              int64_t out_cap, double ms, double tolerance, void* lite_pairs, void* stream);
+/* Compact records of a batch of scenes' pairs (the pair list and offsets of
+ * p2c_bp_batch_count / p2c_bp_batch_emit), in one launch.  x0, x1 hold every
+ * scene's vertices, scene k's from row scenes[k].vertex_base on; the
+ * records name rows of x0 / x1, so the certificate entry points above run on
+ * them with verts = {x0, x1, ms}.  Record p is pair p of the list: the same
+ * record p2c_tuv_setup_pairs_lite writes for that pair of that scene alone.
+ * scenes: device array of n_scenes entries; offsets: device, as written by
+ * p2c_bp_batch_count; n: the number of pairs (offsets[2 * n_scenes]).  No
+ * counterpart in the reference. */
+typedef struct p2c_tuv_scene {
+  const int32_t* faces; /* device (F, 3) scene-local vertex ids */
+  const int32_t* edges; /* device (E, 2) scene-local vertex ids */
+  int64_t vertex_base;  /* first row of the scene in x0 / x1 */
+} p2c_tuv_scene;
+int p2c_tuv_setup_pairs_lite_batch(const double* x0, const double* x1, const p2c_tuv_scene* scenes,
+                                   int32_t n_scenes, const int32_t* pairs, const uint64_t* offsets, int64_t n,
+                                   double ms, double tolerance, void* lite_pairs, void* stream);
 int p2c_tuv_plain_lite(const void* lite_pairs, const p2c_tuv_vertices* verts, int64_t n, const uint64_t* totals,
                        double tolerance, int64_t max_itr, p2c_tuv_result* out, uint32_t* replay_list,
                        uint32_t* replay_count, uint32_t replay_cap, void* stream);
@@ -502,6 +519,46 @@ int p2c_bp_count(const p2c_bp_mesh* mesh, void* workspace, uint64_t* totals, voi
  * in traversal order. */
 int p2c_bp_emit(const p2c_bp_mesh* mesh, void* workspace, int32_t* vf_pairs, int64_t vf_cap, int32_t* ee_pairs,
                 int64_t ee_cap, void* stream);
+
+/* ---- batched broad phase over many resident scenes -----------------------
+ * No counterpart in the reference (its scene tool runs one scene per
+ * process, scene_object_envelope_native_ti_walltime.cpp:1239-1281); it is
+ * for serving per-timestep queries of many resident scenes.  Each scene
+ * keeps its boxes, envelopes, object-pair matrix and both trees in a
+ * workspace of its own (p2c_bp_scene_workspace_bytes); a batch of scenes is
+ * built, counted and emitted with one launch per stage for the whole batch
+ * (the scenes' blocks side by side in each launch, one segmented sort over
+ * every scene's keys), with sort keys, per-query counts and the hit cache in
+ * a batch workspace.  For every scene the pairs, and their order, are those
+ * p2c_bp_build + p2c_bp_count + p2c_bp_emit give for that scene alone.
+ * The hit_cache fields of each mesh are ignored: the batch workspace holds
+ * hit_cache_slots partners per query.  At most 1024 scenes per batch, each
+ * scene workspace at most once. */
+typedef struct p2c_bp_scene {
+  p2c_bp_mesh mesh;       /* topology and sizes (device pointers) */
+  const double* x0;       /* device (V, 3) positions at t = 0 */
+  const double* x1;       /* device (V, 3) positions at t = 1 */
+  double thickness;
+  void* workspace;        /* device, p2c_bp_scene_workspace_bytes(...) bytes */
+  size_t workspace_bytes;
+} p2c_bp_scene;
+
+size_t p2c_bp_scene_workspace_bytes(int64_t n_vertices, int64_t n_faces, int64_t n_edges, int32_t n_objects);
+size_t p2c_bp_batch_workspace_bytes(const p2c_bp_scene* scenes, int32_t n_scenes, int64_t hit_cache_slots);
+/* Build and count every scene of the batch (scenes: host array).  Writes
+ * the device array offsets[2 * n_scenes + 2]: scene k's VF pairs go to
+ * [offsets[2k], offsets[2k + 1]) of the batch's pair list, its EE pairs to
+ * [offsets[2k + 1], offsets[2k + 2]); offsets[2 * n_scenes] is the total and
+ * offsets[2 * n_scenes + 1] the number of traversals that hit the step
+ * bound (always 0 for well-formed trees). */
+int p2c_bp_batch_count(const p2c_bp_scene* scenes, int32_t n_scenes, int64_t hit_cache_slots, void* batch_workspace,
+                       size_t batch_workspace_bytes, uint64_t* offsets, void* stream);
+/* Fill the batch's pair list (int32 pairs; VF pairs (vertex, face), EE
+ * pairs (edge i, edge j), i < j, in scene-local ids, at the offsets of
+ * p2c_bp_batch_count); pairs at or beyond `cap` are dropped.  Called after
+ * p2c_bp_batch_count with the same scenes, slots and batch workspace. */
+int p2c_bp_batch_emit(const p2c_bp_scene* scenes, int32_t n_scenes, int64_t hit_cache_slots, void* batch_workspace,
+                      size_t batch_workspace_bytes, int32_t* pairs, int64_t cap, void* stream);
 
 /* ---- proxy-scene raw candidates --------------------------------------------
  * Replaces GenerateRawCandidatesCpu (src/cpp/rt_candidate/candidate_buffer.cpp:

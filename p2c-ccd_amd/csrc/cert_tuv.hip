@@ -246,6 +246,42 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_setup_pairs_kernel(int is_ee, c
   }
 }
 
+// The compact record of a pair whose four vertices are rows vid of x0 / x1
+// (VF: vertex, then the face's three; EE: edge a's two, then edge b's).
+__device__ inline TuvLite lite_record(const double* __restrict__ x0, const double* __restrict__ x1,
+                                      const int32_t vid[4], int is_ee, double ms, double tol) {
+  double x[8][3];
+  for (int r = 0; r < 4; ++r)
+    for (int k = 0; k < 3; ++k) {
+      x[r][k] = x0[int64_t(vid[r]) * 3 + k];
+      x[4 + r][k] = x1[int64_t(vid[r]) * 3 + k];
+    }
+  TuvPair q;
+  tuv_setup(&q, x, is_ee, ms, tol);
+  TuvLite r;
+  for (int j = 0; j < 4; ++j) r.vid[j] = vid[j];
+  for (int w = 0; w < 5; ++w) r.split[w] = q.split[w];
+  r.done_level = q.done_level;
+  r.is_ee = q.is_ee;
+  return r;
+}
+
+// Vertex ids of pair (a, b) through the topology, plus `base`.
+__device__ inline void pair_vertices(int is_ee, const int32_t* __restrict__ topo, int64_t a, int64_t b, int64_t base,
+                                     int32_t vid[4]) {
+  if (!is_ee) {
+    vid[0] = int32_t(a + base);
+    vid[1] = int32_t(topo[b * 3] + base);
+    vid[2] = int32_t(topo[b * 3 + 1] + base);
+    vid[3] = int32_t(topo[b * 3 + 2] + base);
+  } else {
+    vid[0] = int32_t(topo[a * 2] + base);
+    vid[1] = int32_t(topo[a * 2 + 1] + base);
+    vid[2] = int32_t(topo[b * 2] + base);
+    vid[3] = int32_t(topo[b * 2 + 1] + base);
+  }
+}
+
 // The same pairs as compact records.
 __global__ __launch_bounds__(kTuvBlock) void tuv_setup_lite_kernel(int is_ee, const double* __restrict__ x0,
                                                                    const double* __restrict__ x1,
@@ -261,33 +297,37 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_setup_lite_kernel(int is_ee, co
     off = is_ee ? int64_t(totals[0]) : 0;
   }
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < m; i += int64_t(gridDim.x) * blockDim.x) {
-    const int64_t a = pairs[i * 2], b = pairs[i * 2 + 1];
     int32_t vid[4];
-    if (!is_ee) {
-      vid[0] = int32_t(a);
-      vid[1] = topo[b * 3];
-      vid[2] = topo[b * 3 + 1];
-      vid[3] = topo[b * 3 + 2];
-    } else {
-      vid[0] = topo[a * 2];
-      vid[1] = topo[a * 2 + 1];
-      vid[2] = topo[b * 2];
-      vid[3] = topo[b * 2 + 1];
-    }
-    double x[8][3];
-    for (int r = 0; r < 4; ++r)
-      for (int k = 0; k < 3; ++k) {
-        x[r][k] = x0[int64_t(vid[r]) * 3 + k];
-        x[4 + r][k] = x1[int64_t(vid[r]) * 3 + k];
-      }
-    TuvPair q;
-    tuv_setup(&q, x, is_ee, ms, tol);
-    TuvLite r;
-    for (int j = 0; j < 4; ++j) r.vid[j] = vid[j];
-    for (int w = 0; w < 5; ++w) r.split[w] = q.split[w];
-    r.done_level = q.done_level;
-    r.is_ee = q.is_ee;
+    pair_vertices(is_ee, topo, pairs[i * 2], pairs[i * 2 + 1], 0, vid);
+    const TuvLite r = lite_record(x0, x1, vid, is_ee, ms, tol);
     if (off + i < out_cap) store_lite(out, off + i, r);
+  }
+}
+
+// A batch of scenes' pair list: pair i belongs to the part j of the last
+// offsets[j] <= i (scene j / 2; VF for even j, EE for odd j), its vertex ids
+// move to the scene's rows of the batch position arrays.
+__global__ __launch_bounds__(kTuvBlock) void tuv_setup_lite_batch_kernel(const double* __restrict__ x0,
+                                                                         const double* __restrict__ x1,
+                                                                         const p2c_tuv_scene* __restrict__ sc, int S,
+                                                                         const int32_t* __restrict__ pairs,
+                                                                         const uint64_t* __restrict__ offsets,
+                                                                         int64_t n, double ms, double tol,
+                                                                         TuvLite* __restrict__ out) {
+  const int64_t total = int64_t(offsets[2 * S]);
+  const int64_t m = total < n ? total : n;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < m; i += int64_t(gridDim.x) * blockDim.x) {
+    int lo = 0, hi = 2 * S - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (offsets[mid] <= uint64_t(i)) lo = mid;
+      else hi = mid - 1;
+    }
+    const int is_ee = lo & 1;
+    const p2c_tuv_scene& d = sc[lo >> 1];
+    int32_t vid[4];
+    pair_vertices(is_ee, is_ee ? d.edges : d.faces, pairs[i * 2], pairs[i * 2 + 1], d.vertex_base, vid);
+    store_lite(out, i, lite_record(x0, x1, vid, is_ee, ms, tol));
   }
 }
 
@@ -1534,6 +1574,22 @@ int p2c_tuv_setup_pairs_lite(int is_ee, const double* x0, const double* x1, cons
   hipLaunchKernelGGL(p2c::tuv_setup_lite_kernel, dim3(p2c::grid_for(n, p2c::kTuvBlock)), dim3(p2c::kTuvBlock), 0,
                      p2c::as_stream(stream), is_ee, x0, x1, topo, pairs, n, totals, out_offset, out_cap, ms,
                      tolerance, static_cast<p2c::TuvLite*>(lite_pairs));
+  P2C_HIP_CHECK(hipGetLastError());
+  return P2C_OK;
+}
+
+int p2c_tuv_setup_pairs_lite_batch(const double* x0, const double* x1, const p2c_tuv_scene* scenes,
+                                   int32_t n_scenes, const int32_t* pairs, const uint64_t* offsets, int64_t n,
+                                   double ms, double tolerance, void* lite_pairs, void* stream) {
+  if (int rc = p2c::check_tuv(ms, tolerance, 1.0, 1)) return rc;
+  if (n_scenes < 1) return p2c::set_error(P2C_EINVAL, "a batch holds at least one scene");
+  if (n <= 0) return P2C_OK;
+  if (x0 == nullptr || x1 == nullptr || scenes == nullptr || pairs == nullptr || offsets == nullptr ||
+      lite_pairs == nullptr)
+    return p2c::set_error(P2C_EINVAL, "batch setup: null pointer");
+  hipLaunchKernelGGL(p2c::tuv_setup_lite_batch_kernel, dim3(p2c::grid_for(n, p2c::kTuvBlock)), dim3(p2c::kTuvBlock),
+                     0, p2c::as_stream(stream), x0, x1, scenes, int(n_scenes), pairs, offsets, n, ms, tolerance,
+                     static_cast<p2c::TuvLite*>(lite_pairs));
   P2C_HIP_CHECK(hipGetLastError());
   return P2C_OK;
 }

@@ -16,6 +16,13 @@ int hip_fail(hipError_t e, const char* what);
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Copies a small host table (a batch's launch plan and scene descriptors) to
+// device memory `dev` on stream s without waiting for the stream: the bytes
+// pass through a pinned staging ring of the calling thread (one per device,
+// held until the process ends), and a stretch of the ring is written again
+// only after the copy that read it has completed.
+int upload_table(const void* host, size_t bytes, void* dev, hipStream_t s);
+
 // Grid for a grid-stride kernel over n items.
 inline unsigned grid_for(int64_t n, int block, int64_t max_blocks = int64_t(1) << 20) {
   int64_t g = (n + block - 1) / block;

@@ -22,8 +22,30 @@ constexpr int kSortItems = 16;  // keys per lane per radix tile
 constexpr int kSortTile = kScanBlock * kSortItems;
 
 // ---- 2. radix sort -------------------------------------------------------
+// Where tile `blk`'s count of digit d sits in the histogram: hist[base +
+// d * stride].  One segment (nseg == 0): digit-major over all nblk tiles.
+// Segments (a segmented sort; seg[k] is the first tile of segment k, seg[nseg]
+// the tile count): digit-major within each segment, segment after segment,
+// so the exclusive scan of the histogram gives every tile its place inside
+// its own segment's stretch of keys and the segments sort independently.
+struct HistSlot {
+  int64_t base, stride;
+};
+__device__ inline HistSlot hist_slot(int64_t blk, int64_t nblk, const int32_t* seg, int nseg) {
+  if (nseg == 0) return HistSlot{blk, nblk};
+  int lo = 0, hi = nseg - 1;  // the last segment starting at or before blk
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (seg[mid] <= blk) lo = mid;
+    else hi = mid - 1;
+  }
+  const int64_t t0 = seg[lo], nt = int64_t(seg[lo + 1]) - t0;
+  return HistSlot{256 * t0 + (blk - t0), nt};
+}
+
 __global__ __launch_bounds__(kScanBlock) void radix_count_kernel(const uint32_t* __restrict__ keys, int64_t n, int shift,
-                                                             uint32_t* __restrict__ hist, int64_t nblk) {
+                                                             uint32_t* __restrict__ hist, int64_t nblk,
+                                                             const int32_t* __restrict__ seg, int nseg) {
   __shared__ uint32_t cnt[256];
   cnt[threadIdx.x] = 0;
   __syncthreads();
@@ -33,7 +55,8 @@ __global__ __launch_bounds__(kScanBlock) void radix_count_kernel(const uint32_t*
     if (e < n) atomicAdd(&cnt[(keys[e] >> shift) & 0xffu], 1u);
   }
   __syncthreads();
-  hist[int64_t(threadIdx.x) * nblk + blockIdx.x] = cnt[threadIdx.x];
+  const HistSlot h = hist_slot(blockIdx.x, nblk, seg, nseg);
+  hist[h.base + int64_t(threadIdx.x) * h.stride] = cnt[threadIdx.x];
 }
 
 // Stable scatter.  Wave w of a tile owns the contiguous chunk
@@ -44,6 +67,7 @@ __global__ __launch_bounds__(kScanBlock) void radix_count_kernel(const uint32_t*
 __global__ __launch_bounds__(kScanBlock) void radix_scatter_kernel(const uint32_t* __restrict__ kin,
                                                                const uint32_t* __restrict__ vin, int64_t n, int shift,
                                                                const uint64_t* __restrict__ hist_off, int64_t nblk,
+                                                               const int32_t* __restrict__ seg, int nseg,
                                                                uint32_t* __restrict__ kout,
                                                                uint32_t* __restrict__ vout) {
   constexpr int kWaves = kScanBlock / 64;
@@ -87,12 +111,13 @@ __global__ __launch_bounds__(kScanBlock) void radix_scatter_kernel(const uint32_
     }
   }
   __syncthreads();
+  const HistSlot h = hist_slot(blockIdx.x, nblk, seg, nseg);
 #pragma unroll
   for (int j = 0; j < kSortItems; ++j) {
     const int64_t e = chunk + int64_t(j) * 64 + lane;
     if (e < n) {
       const uint32_t d = (key[j] >> shift) & 0xffu;
-      const uint64_t dst = hist_off[int64_t(d) * nblk + blockIdx.x] + wcnt[w][d] + pos[j];
+      const uint64_t dst = hist_off[h.base + int64_t(d) * h.stride] + wcnt[w][d] + pos[j];
       kout[dst] = key[j];
       vout[dst] = val[j];
     }
@@ -253,15 +278,20 @@ __global__ __launch_bounds__(kScanSmallBlock) void scan_small_kernel(const uint3
 // keys, ping-ponging between (k0, v0) and (k1, v1); an even number of passes
 // leaves the result in (k0, v0), an odd number in (k1, v1).  Workspace: hist
 // u32[256 * tiles], hist_off u64[256 * tiles], part u64[256 * tiles / 1024 + 2]
-// with tiles = ceil(n / kSortTile).
+// with tiles = ceil(n / kSortTile).  With a segment table (seg: device
+// int32[nseg + 1], first tile of each segment, then the tile count; n a
+// multiple of kSortTile) each segment of whole tiles is sorted on its own.
 inline int radix_sort_passes(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, int64_t n, int passes,
-                             uint32_t* hist, uint64_t* hist_off, uint64_t* part, hipStream_t s) {
+                             uint32_t* hist, uint64_t* hist_off, uint64_t* part, hipStream_t s,
+                             const int32_t* seg = nullptr, int nseg = 0) {
   if (n <= 1) return P2C_OK;
   if (n > int64_t(UINT32_MAX)) return set_error(P2C_ECAPACITY, "radix sort: more than 2^32 - 1 keys");
+  if (nseg > 0 && n % kSortTile != 0) return set_error(P2C_EINVAL, "radix sort: segments must be whole tiles");
   const int64_t nblk = (n + kSortTile - 1) / kSortTile;
   for (int pass = 0; pass < passes; ++pass) {
     const int shift = pass * 8;
-    hipLaunchKernelGGL(radix_count_kernel, dim3(unsigned(nblk)), dim3(kScanBlock), 0, s, k0, n, shift, hist, nblk);
+    hipLaunchKernelGGL(radix_count_kernel, dim3(unsigned(nblk)), dim3(kScanBlock), 0, s, k0, n, shift, hist, nblk,
+                       seg, nseg);
     const int64_t m = 256 * nblk;
     if (m <= kScanSmallMax) {
       // every prefix of a digit histogram is at most n < 2^32
@@ -274,7 +304,7 @@ inline int radix_sort_passes(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t*
       hipLaunchKernelGGL(scan_add_kernel, dim3(unsigned(sb)), dim3(kScanBlock), 0, s, hist_off, m, part);
     }
     hipLaunchKernelGGL(radix_scatter_kernel, dim3(unsigned(nblk)), dim3(kScanBlock), 0, s, k0, v0, n, shift, hist_off,
-                       nblk, k1, v1);
+                       nblk, seg, nseg, k1, v1);
     std::swap(k0, k1);
     std::swap(v0, v1);
   }
@@ -284,8 +314,9 @@ inline int radix_sort_passes(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t*
 
 // Full 32-bit sort (4 passes): the result is back in (k0, v0).
 inline int radix_sort(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, int64_t n, uint32_t* hist,
-                      uint64_t* hist_off, uint64_t* part, hipStream_t s) {
-  return radix_sort_passes(k0, v0, k1, v1, n, 4, hist, hist_off, part, s);
+                      uint64_t* hist_off, uint64_t* part, hipStream_t s, const int32_t* seg = nullptr,
+                      int nseg = 0) {
+  return radix_sort_passes(k0, v0, k1, v1, n, 4, hist, hist_off, part, s, seg, nseg);
 }
 
 // Exclusive scan of n u32 values into u64 (out[n] = total); part holds

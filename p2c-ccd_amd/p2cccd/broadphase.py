@@ -4,6 +4,12 @@ A `BroadPhase` holds one mesh's topology on the device (faces, edges, object
 ids -- prepared once, as the reference's PrepareScene does) and a reusable
 workspace; each step builds both BVHs from that step's positions and emits
 the VF and EE candidate pairs.
+
+For many scenes at once (serving), a `ResidentMesh` keeps a scene's topology
+and its trees on the device, and a `BatchBroadPhase` builds, counts and emits
+a whole batch of them with one launch per stage (p2c_bp_batch_count /
+p2c_bp_batch_emit): every scene's pairs, in the same order, as `BroadPhase`
+gives for that scene alone, in one pair list.
 """
 from __future__ import annotations
 
@@ -41,6 +47,25 @@ nat._sig("p2c_bp_count", [_vp, _vp, _vp, _vp])
 nat._sig("p2c_bp_emit", [_vp, _vp, _vp, _i64, _vp, _i64, _vp])
 nat._sig("p2c_bp_workspace_layout", [_i64, _i64, _i64, _i32, _vp])
 _LAYOUT_FIELDS = ("fnodes", "enodes", "vbox", "ebox", "qorder", "eorder", "counts", "resume")
+
+
+class _Scene(ctypes.Structure):
+    _fields_ = [
+        ("mesh", _Mesh),
+        ("x0", ctypes.c_void_p),
+        ("x1", ctypes.c_void_p),
+        ("thickness", ctypes.c_double),
+        ("workspace", ctypes.c_void_p),
+        ("workspace_bytes", ctypes.c_size_t),
+    ]
+
+
+nat.lib.p2c_bp_scene_workspace_bytes.argtypes = [_i64, _i64, _i64, _i32]
+nat.lib.p2c_bp_scene_workspace_bytes.restype = ctypes.c_size_t
+nat.lib.p2c_bp_batch_workspace_bytes.argtypes = [_vp, _i32, _i64]
+nat.lib.p2c_bp_batch_workspace_bytes.restype = ctypes.c_size_t
+nat._sig("p2c_bp_batch_count", [_vp, _i32, _i64, _vp, ctypes.c_size_t, _vp, _vp])
+nat._sig("p2c_bp_batch_emit", [_vp, _i32, _i64, _vp, ctypes.c_size_t, _vp, _i64, _vp])
 
 
 DEFAULT_CACHE_SLOTS = 256  # partners per query kept by the count pass (covers nearly all queries of dense scenes)
@@ -128,3 +153,96 @@ class BroadPhase:
                                                 host["qorder"], host["eorder"], self.vertex_object, host["counts"],
                                                 host["resume"], nv, self.cache_slots)
 
+
+
+class ResidentMesh:
+    """A scene's topology and broad-phase state (boxes, envelopes, object-pair
+    matrix, both trees) kept on the device for batched builds."""
+
+    def __init__(self, topo: Topology, n_vertices: int, device: str | torch.device = "cuda"):
+        self.device = torch.device(device)
+
+        def up(a):
+            return torch.from_numpy(np.ascontiguousarray(a, np.int32)).to(self.device)
+
+        self.faces, self.edges = up(topo.faces), up(topo.edges)
+        self.vobj, self.fobj, self.eobj = up(topo.vertex_object), up(topo.face_object), up(topo.edge_object)
+        self.n_vertices, self.n_faces, self.n_edges = n_vertices, topo.faces.shape[0], topo.edges.shape[0]
+        self.n_objects = topo.object_count
+        self.ws_bytes = self.workspace_bytes(n_vertices, self.n_faces, self.n_edges, self.n_objects)
+        self.workspace = torch.empty(max(self.ws_bytes, 1), dtype=torch.uint8, device=self.device)
+        self.mesh = _Mesh(self.faces.data_ptr(), self.edges.data_ptr(), self.vobj.data_ptr(), self.fobj.data_ptr(),
+                          self.eobj.data_ptr(), self.n_vertices, self.n_faces, self.n_edges, self.n_objects, 0,
+                          None, 0)
+
+    @staticmethod
+    def workspace_bytes(n_vertices: int, n_faces: int, n_edges: int, n_objects: int) -> int:
+        return int(nat.lib.p2c_bp_scene_workspace_bytes(n_vertices, n_faces, n_edges, n_objects))
+
+    @classmethod
+    def footprint(cls, topo: Topology, n_vertices: int) -> int:
+        """Device bytes of a resident mesh: its workspace and its topology."""
+        nf, ne = topo.faces.shape[0], topo.edges.shape[0]
+        return cls.workspace_bytes(n_vertices, nf, ne, topo.object_count) + 4 * (3 * nf + 2 * ne + n_vertices + nf + ne)
+
+    def nbytes(self) -> int:
+        return self.workspace.numel() + 4 * sum(t.numel() for t in (self.faces, self.edges, self.vobj, self.fobj,
+                                                                     self.eobj))
+
+
+class BatchBroadPhase:
+    """Broad phase of a batch of resident meshes, one launch per stage: count()
+    builds every mesh's trees and counts its pairs, emit() writes the batch's
+    pair list (scene after scene: its VF pairs, then its EE pairs, in
+    scene-local ids).  The batch workspace (sort keys, per-query counts, the
+    hit cache of `cache_slots` partners per query) grows to the largest batch
+    seen; allocate and call on one stream."""
+
+    def __init__(self, device: str | torch.device = "cuda", cache_slots: int = 16):
+        self.device = torch.device(device)
+        self.cache_slots = int(cache_slots)
+        self.workspace = torch.empty(0, dtype=torch.uint8, device=self.device)
+        self.offsets = torch.zeros(2, dtype=torch.int64, device=self.device)
+        self.scenes = None
+        self.n_scenes = 0
+
+    def _describe(self, items) -> None:
+        arr = (_Scene * len(items))()
+        for k, (m, x0, x1, th) in enumerate(items):
+            arr[k] = _Scene(m.mesh, x0, x1, float(th), m.workspace.data_ptr(), m.ws_bytes)
+        self.scenes, self.n_scenes = arr, len(items)
+
+    def count(self, items, stream=None) -> torch.Tensor:
+        """items: (ResidentMesh, x0 address, x1 address, thickness) per scene,
+        positions device (V, 3) float64.  Returns the device offsets
+        (int64[2 S + 2], p2c_bp_batch_count)."""
+        if not items:
+            raise ValueError("a batch holds at least one scene")
+        self._describe(items)
+        need = int(nat.lib.p2c_bp_batch_workspace_bytes(self.scenes, self.n_scenes, self.cache_slots))
+        if need == 0:
+            raise ValueError("invalid batch")
+        if self.workspace.numel() < need:
+            self.workspace = torch.empty(int(need * 1.25) + (1 << 20), dtype=torch.uint8, device=self.device)
+        if self.offsets.numel() < 2 * self.n_scenes + 2:
+            self.offsets = torch.zeros(2 * self.n_scenes + 2, dtype=torch.int64, device=self.device)
+        nat.call("p2c_bp_batch_count", self.scenes, self.n_scenes, self.cache_slots, self.workspace.data_ptr(),
+                 self.workspace.numel(), self.offsets.data_ptr(), nat.stream_handle(stream))
+        return self.offsets[: 2 * self.n_scenes + 2]
+
+    def emit(self, pairs: torch.Tensor, stream=None) -> None:
+        """Fills `pairs` (int32 (cap, 2)) for the batch of the last count()."""
+        assert pairs.dtype == torch.int32 and pairs.is_contiguous()
+        nat.call("p2c_bp_batch_emit", self.scenes, self.n_scenes, self.cache_slots, self.workspace.data_ptr(),
+                 self.workspace.numel(), pairs.data_ptr(), pairs.shape[0], nat.stream_handle(stream))
+
+    def candidates(self, items) -> tuple[np.ndarray, torch.Tensor]:
+        """Count, read the offsets (one host sync), emit; returns the host
+        offsets and the device pair list."""
+        offs = self.count(items).cpu().numpy().astype(np.int64)
+        s = self.n_scenes
+        if offs[2 * s + 1]:
+            raise RuntimeError(f"{offs[2 * s + 1]} BVH traversals exceeded their step bound")
+        pairs = torch.empty((max(int(offs[2 * s]), 1), 2), dtype=torch.int32, device=self.device)
+        self.emit(pairs)
+        return offs, pairs[: int(offs[2 * s])]

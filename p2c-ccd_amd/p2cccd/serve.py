@@ -1,21 +1,29 @@
 """Serving: per-timestep CCD requests against many resident scenes.
 
-Each scene keeps its topology, LBVH workspace, positions and candidate
-buffers in HBM.  A request carries one timestep's positions (x0, x1) for one
-scene.  The server runs a continuous-batching loop over two lanes (a HIP
-stream and a certificate workspace each):
+Each resident scene keeps its topology and its broad-phase state (boxes,
+object envelopes and pair matrix, both LBVHs) in HBM.  A request carries one
+timestep's positions (x0, x1) for one scene.  The server runs a
+continuous-batching loop over two lanes (a HIP stream, a batch broad-phase
+workspace, batch position and pair buffers and a certificate workspace
+each):
 
   * admission: a queued request joins the next batch when its scene is
     resident or can be made resident within the HBM budget (least recently
     used idle scenes are evicted), its scene has no request in flight, and
     the batch has request and pair slots left;
-  * a batch runs every request's broad phase on its lane, reads all candidate
-    counts with one wait on that lane, then sets up every request's pairs into
-    one record buffer and certifies them in one shared launch (stream kernel
-    + wave pass + replay), so requests of mixed size share launches;
+  * a batch uploads every request's positions in one copy, builds and counts
+    every scene's trees with one launch per stage for the whole batch
+    (BatchBroadPhase), reads the pair offsets with one wait on its lane,
+    emits one pair list (scene after scene), sets up compact certificate
+    records for all of it in one launch and certifies them in one shared
+    launch (stream kernel + wave pass + replay), so requests of mixed size
+    share every launch;
   * while one lane's batch is on the GPU, the host admits new arrivals into
     the other lane's batch, whose broad phase overlaps the first batch's
     certificate on the device.
+
+The HBM budget covers the resident scenes; each lane's working buffers are
+sized by the largest batch it has run.
 
 Per-pair answers do not depend on what else shares a batch: candidate order
 is fixed per scene and the certificate's result per pair is order-free, so a
@@ -36,8 +44,8 @@ import numpy as np
 import torch
 
 from . import scenes
-from .broadphase import BroadPhase
-from .tuv import TUV_RESULT_DTYPE, TuvCertifier, TuvConfig
+from .broadphase import BatchBroadPhase, ResidentMesh
+from .tuv import TUV_RESULT_DTYPE, TUV_SCENE_DTYPE, TuvCertifier, TuvConfig
 
 
 @dataclass
@@ -129,69 +137,75 @@ class Residency:
 
 
 # ---- device state -------------------------------------------------------------
-CACHE_SLOTS = 64  # hit-cache partners per query for resident scenes (kept small: one cache per scene)
+CACHE_SLOTS = 16  # hit-cache partners per query of a batch (most queries of these scenes have fewer partners)
 
 
 class ResidentScene:
-    def __init__(self, scene: scenes.Scene, device, headroom: float = 1.25):
-        self.scene = scene
-        self.device = device
-        self.bp = BroadPhase.for_scene(scene, device, CACHE_SLOTS)
-        self.x0 = torch.empty((scene.n_vertices, 3), dtype=torch.float64, device=device)
-        self.x1 = torch.empty_like(self.x0)
-        self.headroom = headroom
-        self.cap_vf = self.cap_ee = 0
-        self.vf = self.ee = None
+    """A scene's topology and broad-phase state on the device."""
 
-    def ensure(self, n_vf: int, n_ee: int) -> None:
-        if self.vf is not None and n_vf <= self.cap_vf and n_ee <= self.cap_ee:
-            return
-        self.cap_vf = max(int(n_vf * self.headroom) + 256, self.cap_vf)
-        self.cap_ee = max(int(n_ee * self.headroom) + 256, self.cap_ee)
-        self.vf = torch.empty((self.cap_vf, 2), dtype=torch.int32, device=self.device)
-        self.ee = torch.empty((self.cap_ee, 2), dtype=torch.int32, device=self.device)
+    def __init__(self, scene: scenes.Scene, device):
+        self.scene = scene
+        self.mesh = ResidentMesh(scene.topo, scene.n_vertices, device)
 
     @staticmethod
-    def estimate_bytes(scene: scenes.Scene, pairs_guess: int = 0) -> int:
-        nv, nf, ne = scene.n_vertices, len(scene.topo.faces), len(scene.topo.edges)
-        ws = BroadPhase.workspace_bytes(nv, nf, ne, scene.topo.object_count, CACHE_SLOTS)
-        return ws + 2 * nv * 24 + (nf * 12 + ne * 8) + 8 * int(pairs_guess * 1.25)
+    def estimate_bytes(scene: scenes.Scene) -> int:
+        return ResidentMesh.footprint(scene.topo, scene.n_vertices)
 
     def nbytes(self) -> int:
-        return self.bp.ws_bytes + self.bp.cache.numel() * 4 + 2 * self.x0.numel() * 8 + \
-            8 * (self.cap_vf + self.cap_ee) + self.bp.faces.numel() * 4 + self.bp.edges.numel() * 4
+        return self.mesh.nbytes()
 
 
 @dataclass
 class _Batch:
     reqs: list
-    offsets: list = field(default_factory=list)
-    counts: list = field(default_factory=list)
+    offsets: list = field(default_factory=list)  # first pair of each request
+    counts: list = field(default_factory=list)   # (n_vf, n_ee) of each request
     total: int = 0
     event: torch.cuda.Event | None = None
 
 
+def _grow(t: torch.Tensor, n: int) -> torch.Tensor:
+    return t if t.numel() >= n else torch.empty(int(n * 1.25) + 1024, dtype=t.dtype, device=t.device)
+
+
 class Lane:
-    def __init__(self, device, cfg: TuvConfig | None):
+    """A stream and the working buffers of the batch it runs."""
+
+    def __init__(self, device, cfg: TuvConfig | None, cache_slots: int):
+        self.device = device
         self.stream = torch.cuda.Stream(device)
         self.cert = TuvCertifier(cfg, device)
+        self.bp = BatchBroadPhase(device, cache_slots)
         self.batch: _Batch | None = None
         self.stage = torch.empty(0, dtype=torch.float64).pin_memory()  # pinned staging of the batch's positions
+        self.tab_host = torch.empty(0, dtype=torch.uint8).pin_memory()  # pinned staging of the setup's scene table
+        self.xb = torch.empty(0, dtype=torch.float64, device=device)  # the batch's x0 rows, then its x1 rows
+        self.tab = torch.empty(0, dtype=torch.uint8, device=device)
+        self.pairs = torch.empty((0, 2), dtype=torch.int32, device=device)
 
     def staging(self, n: int) -> torch.Tensor:
         if self.stage.numel() < n:
             self.stage = torch.empty(int(n * 1.25) + 1024, dtype=torch.float64).pin_memory()
         return self.stage
 
+    def table_staging(self, n: int) -> torch.Tensor:
+        if self.tab_host.numel() < n:
+            self.tab_host = torch.empty(int(n * 1.25) + 1024, dtype=torch.uint8).pin_memory()
+        return self.tab_host
+
+    def reserve(self, n_pos: int, n_tab: int, n_pairs: int) -> None:
+        """Device buffers for a batch (call on the lane's stream)."""
+        self.xb = _grow(self.xb, n_pos)
+        self.tab = _grow(self.tab, n_tab)
+        if self.pairs.shape[0] < n_pairs:
+            self.pairs = torch.empty((int(n_pairs * 1.25) + 1024, 2), dtype=torch.int32, device=self.device)
+
 
 class Server:
     def __init__(self, scene_table: dict[int, scenes.Scene], hbm_budget: int, device="cuda",
                  max_batch_requests: int = 64, max_batch_pairs: int = 1 << 23, cfg: TuvConfig | None = None,
-                 lanes: int = 2, full_results: bool = False, side_streams: int = 4):
+                 lanes: int = 2, full_results: bool = False, cache_slots: int = CACHE_SLOTS):
         self.full_results = full_results
-        # the per-scene stages of a batch (uploads, broad phase, pair setup) are spread over these streams so the
-        # many small kernels of small scenes run side by side; default priority, like every stream here
-        self.side = [torch.cuda.Stream(torch.device(device)) for _ in range(side_streams)]
         self.table = scene_table
         self.device = torch.device(device)
         self.res = Residency(hbm_budget)
@@ -199,7 +213,7 @@ class Server:
         self.last_pairs: dict[int, int] = {}
         self.max_batch_requests = max_batch_requests
         self.max_batch_pairs = max_batch_pairs
-        self.lanes = [Lane(self.device, cfg) for _ in range(lanes)]
+        self.lanes = [Lane(self.device, cfg, cache_slots) for _ in range(lanes)]
         self.batches = 0
         self.loads = 0
 
@@ -208,7 +222,7 @@ class Server:
         if sid in self.resident:
             self.res.touch(sid)
             return True
-        need = ResidentScene.estimate_bytes(self.table[sid], self.last_pairs.get(sid, 0))
+        need = ResidentScene.estimate_bytes(self.table[sid])
         plan = self.res.plan(sid, need)
         if plan is None:
             return False
@@ -241,76 +255,53 @@ class Server:
         queue.extendleft(reversed(skipped))
         return batch
 
-    def _streams(self, lane: Lane, k: int, nside: int):
-        return self.side[k % nside] if nside > 0 else lane.stream
-
-    def _fork(self, lane: Lane, n: int) -> None:
-        for s in self.side[:n]:
-            s.wait_stream(lane.stream)
-
-    def _join(self, lane: Lane, n: int) -> None:
-        for s in self.side[:n]:
-            lane.stream.wait_stream(s)
-
     def _launch(self, lane: Lane, reqs: list) -> None:
         b = _Batch(reqs=reqs)
-        # positions go through the lane's pinned staging, so the copies are asynchronous
-        sizes = [2 * self.table[r.scene_id].n_vertices * 3 for r in reqs]
-        stage = lane.staging(sum(sizes))
+        S = len(reqs)
+        nvs = [self.table[r.scene_id].n_vertices for r in reqs]
+        base = np.concatenate([[0], np.cumsum(nvs)]).astype(np.int64)
+        V = int(base[-1])
+        # positions through the lane's pinned staging: every x0 row, then every x1 row, one copy
+        stage = lane.staging(6 * V)
         host = stage.numpy()
-        # only deep batches (a loaded server) fan out; a light load keeps each batch on its lane's stream, where
-        # it does not compete with the other lane's certificate
-        nside = min(len(self.side), len(reqs)) if len(reqs) >= 4 else 0
-        self._fork(lane, nside)
-        at = 0
-        for k, (r, sz) in enumerate(zip(reqs, sizes)):
-            rs = self.resident[r.scene_id]
-            st = self._streams(lane, k, nside)
-            half = sz // 2
-            host[at:at + half] = r.x0.reshape(-1)
-            host[at + half:at + sz] = r.x1.reshape(-1)
-            with torch.cuda.stream(st):
-                rs.x0.view(-1).copy_(stage[at:at + half], non_blocking=True)
-                rs.x1.view(-1).copy_(stage[at + half:at + sz], non_blocking=True)
-                rs.bp.build(rs.x0, rs.x1, rs.scene.thickness, stream=st)
-                rs.bp.count(stream=st)
-            at += sz
-        self._join(lane, nside)
-        with torch.cuda.stream(lane.stream):
-            totals = torch.stack([self.resident[r.scene_id].bp.totals for r in reqs]).to("cpu", non_blocking=True)
-            lane.stream.synchronize()  # the counts size this batch; waits on this lane only
-        off = 0
-        for r, t in zip(reqs, totals.tolist()):
-            n_vf, n_ee, bad = t
-            if bad:
-                raise RuntimeError(f"{bad} BVH traversals exceeded their step bound (scene {r.scene_id})")
-            rs = self.resident[r.scene_id]
-            with torch.cuda.stream(lane.stream):
-                rs.ensure(n_vf, n_ee)
-            self.res.bytes[r.scene_id] = rs.nbytes()  # the estimate, corrected by the real footprint
-            b.offsets.append(off)
+        for k, r in enumerate(reqs):
+            host[3 * base[k]: 3 * base[k + 1]] = r.x0.reshape(-1)
+            host[3 * (V + base[k]): 3 * (V + base[k + 1])] = r.x1.reshape(-1)
+        tab = np.zeros(S, TUV_SCENE_DTYPE)
+        s = lane.stream
+        with torch.cuda.stream(s):
+            lane.reserve(6 * V, tab.nbytes, 0)
+            lane.xb[: 6 * V].copy_(stage[: 6 * V], non_blocking=True)
+            x0, x1 = lane.xb[: 3 * V], lane.xb[3 * V: 6 * V]
+            items = []
+            for k, r in enumerate(reqs):
+                rs = self.resident[r.scene_id]
+                items.append((rs.mesh, x0.data_ptr() + 24 * int(base[k]), x1.data_ptr() + 24 * int(base[k]),
+                              rs.scene.thickness))
+                tab[k] = (rs.mesh.faces.data_ptr(), rs.mesh.edges.data_ptr(), base[k])
+            offs_dev = lane.bp.count(items, stream=s)
+            offs = offs_dev.cpu().numpy().astype(np.int64)  # the counts size this batch; waits on this lane only
+        if offs[2 * S + 1]:
+            raise RuntimeError(f"{offs[2 * S + 1]} BVH traversals exceeded their step bound")
+        for k, r in enumerate(reqs):
+            n_vf, n_ee = int(offs[2 * k + 1] - offs[2 * k]), int(offs[2 * k + 2] - offs[2 * k + 1])
+            b.offsets.append(int(offs[2 * k]))
             b.counts.append((n_vf, n_ee))
-            off += n_vf + n_ee
             self.last_pairs[r.scene_id] = n_vf + n_ee
-        b.total = off
+        b.total = total = int(offs[2 * S])
         cert = lane.cert
-        with torch.cuda.stream(lane.stream):
-            cert.reserve(max(off, 1))
-        self._fork(lane, nside)
-        for k, (r, o, (n_vf, n_ee)) in enumerate(zip(reqs, b.offsets, b.counts)):
-            rs = self.resident[r.scene_id]
-            st = self._streams(lane, k, nside)
-            rs.bp.emit(rs.vf, rs.ee, stream=st)
-            if n_vf:
-                cert.setup_pairs(rs.x0, rs.x1, rs.bp.faces, rs.vf, False, n_vf, out_offset=o, stream=st)
-            if n_ee:
-                cert.setup_pairs(rs.x0, rs.x1, rs.bp.edges, rs.ee, True, n_ee, out_offset=o + n_vf, stream=st)
-        self._join(lane, nside)
-        with torch.cuda.stream(lane.stream):
-            if off:
-                cert.certify(off, "stream", stream=lane.stream)
+        th = lane.table_staging(tab.nbytes)
+        th.numpy()[: tab.nbytes] = tab.view(np.uint8)
+        with torch.cuda.stream(s):
+            lane.reserve(0, tab.nbytes, total)
+            cert.reserve(max(total, 1))
+            lane.bp.emit(lane.pairs, stream=s)
+            if total:
+                lane.tab[: tab.nbytes].copy_(th[: tab.nbytes], non_blocking=True)
+                cert.setup_batch_lite(x0, x1, lane.tab, S, lane.pairs, offs_dev, total, stream=s)
+                cert.certify(total, "stream", stream=s)
             b.event = torch.cuda.Event()
-            b.event.record(lane.stream)
+            b.event.record(s)
         lane.batch = b
         self.batches += 1
 
@@ -323,32 +314,26 @@ class Server:
             res = lane.cert.result_tensor(max(b.total, 1)).view(torch.float64).view(-1, 4)[: b.total]
             hit = (res[:, 2].view(torch.int64) & 0xFFFFFFFF) != 0
             idx = torch.nonzero(hit).squeeze(1)  # ascending: requests' VF then EE ranges in order
-            toi = res[idx, 0]
             ends = [e for o, (n_vf, n_ee) in zip(b.offsets, b.counts) for e in (o, o + n_vf, o + n_vf + n_ee)]
             cut = torch.searchsorted(idx, torch.tensor(ends, dtype=torch.int64, device=idx.device)).cpu().tolist()
-            host = []
-            for k, (r, o, (n_vf, n_ee)) in enumerate(zip(b.reqs, b.offsets, b.counts)):
-                rs = self.resident[r.scene_id]
-                lo, mid, hi = cut[3 * k: 3 * k + 3]
-                part = (rs.vf[idx[lo:mid] - o], toi[lo:mid], rs.ee[idx[mid:hi] - (o + n_vf)], toi[mid:hi])
-                host.append(tuple(t.to("cpu", non_blocking=True) for t in part))
+            hit_pairs = lane.pairs[idx].to("cpu", non_blocking=True)
+            hit_toi = res[idx, 0].to("cpu", non_blocking=True)
             full = None
             if self.full_results:
-                full = [(self.resident[r.scene_id].vf[:n_vf].cpu().numpy(),
-                         self.resident[r.scene_id].ee[:n_ee].cpu().numpy(),
-                         res[o:o + n_vf + n_ee].cpu().numpy()) for r, o, (n_vf, n_ee) in
-                        zip(b.reqs, b.offsets, b.counts)]
+                full = [(lane.pairs[o:o + n_vf].cpu().numpy(), lane.pairs[o + n_vf:o + n_vf + n_ee].cpu().numpy(),
+                         res[o:o + n_vf + n_ee].cpu().numpy()) for o, (n_vf, n_ee) in zip(b.offsets, b.counts)]
             lane.stream.synchronize()
         now = time.perf_counter()
+        hp, ht = hit_pairs.numpy(), hit_toi.numpy()
         for k, (r, (n_vf, n_ee)) in enumerate(zip(b.reqs, b.counts)):
-            vf_hits, vf_toi, ee_hits, ee_toi = (t.numpy() for t in host[k])
+            lo, mid, hi = cut[3 * k: 3 * k + 3]
             extra = None
             if full is not None:
                 vf, ee, rr = full[k]
                 view = rr.view(TUV_RESULT_DTYPE).reshape(-1)
                 extra = {"vf": vf, "ee": ee, "hit": view["hit"].copy(), "toi": view["toi"].copy()}
-            out.append(Completed(r.rid, r.scene_id, r.arrival, now, n_vf, n_ee, vf_hits, vf_toi, ee_hits, ee_toi,
-                                 extra))
+            out.append(Completed(r.rid, r.scene_id, r.arrival, now, n_vf, n_ee, hp[lo:mid], ht[lo:mid], hp[mid:hi],
+                                 ht[mid:hi], extra))
             self.res.busy.discard(r.scene_id)
         lane.batch = None
 

@@ -59,6 +59,10 @@ nat.lib.p2c_tuv_lite_bytes.argtypes = [nat._i64]
 nat.lib.p2c_tuv_lite_bytes.restype = nat.ctypes.c_size_t
 # p2c_tuv_vertices: the vertex arrays compact pair records are expanded from
 VERTICES_DTYPE = np.dtype([("x0", "<u8"), ("x1", "<u8"), ("ms", "<f8")])
+# p2c_tuv_scene: one scene of a batch's pair list (device topology addresses, first vertex row)
+TUV_SCENE_DTYPE = np.dtype([("faces", "<u8"), ("edges", "<u8"), ("vertex_base", "<i8")])
+nat._sig("p2c_tuv_setup_pairs_lite_batch", [nat._vp, nat._vp, nat._vp, nat._i32, nat._vp, nat._vp, nat._i64, nat._f64,
+                                            nat._f64, nat._vp, nat._vp])
 nat.lib.p2c_tuv_static_max_pairs.restype = nat.ctypes.c_size_t
 nat.lib.p2c_tuv_pairs_bytes.argtypes = [nat._i64]
 nat.lib.p2c_tuv_pairs_bytes.restype = nat.ctypes.c_size_t
@@ -227,6 +231,19 @@ class TuvCertifier:
         else:
             nat.call("p2c_tuv_setup_pairs", *args, self.pairs.data_ptr(), nat.stream_handle(stream))
         self.lite_active = lite
+
+    def setup_batch_lite(self, x0: torch.Tensor, x1: torch.Tensor, scenes: torch.Tensor, n_scenes: int,
+                         pairs: torch.Tensor, offsets: torch.Tensor, n: int, stream=None) -> None:
+        """Compact records of a batch of scenes' pair list (the pairs and
+        offsets of BatchBroadPhase; p2c_tuv_setup_pairs_lite_batch): x0 and
+        x1 hold every scene's vertices, `scenes` is the device table of
+        TUV_SCENE_DTYPE rows (faces, edges, first vertex row)."""
+        self._reserve(n)
+        self.verts["x0"], self.verts["x1"], self.verts["ms"] = x0.data_ptr(), x1.data_ptr(), self.cfg.ms
+        nat.call("p2c_tuv_setup_pairs_lite_batch", x0.data_ptr(), x1.data_ptr(), scenes.data_ptr(), n_scenes,
+                 pairs.data_ptr(), offsets.data_ptr(), n, self.cfg.ms, self.cfg.tolerance, self.lite.data_ptr(),
+                 nat.stream_handle(stream))
+        self.lite_active = True
 
     def replay_count(self) -> int:
         return int(self.counters[1].item() & 0xFFFFFFFF)
