@@ -457,9 +457,10 @@ __device__ inline void refit_block(int64_t n, Node* __restrict__ nodes, const in
 }
 
 constexpr int kTopBlock = 1024;
-// The climb's arrival flags as bits in LDS (64 KB: trees of up to 524,288
-// internal nodes): an LDS atomic per level instead of an L2 one.
-constexpr int kTopLdsWords = 16384;
+// The climb's arrival flags as bits in LDS (160 KB, a gfx950 CU's whole LDS:
+// trees of up to 1,310,720 internal nodes): an LDS atomic per level instead
+// of an L2 one.
+constexpr int kTopLdsWords = 40960;
 
 // Second launch, one workgroup of kTopBlock threads per tree: the climb with
 // per-node counters in global memory (flag, cleared by leaf_item).

@@ -159,14 +159,21 @@ class ResidentMesh:
     """A scene's topology and broad-phase state (boxes, envelopes, object-pair
     matrix, both trees) kept on the device for batched builds."""
 
-    def __init__(self, topo: Topology, n_vertices: int, device: str | torch.device = "cuda"):
+    TOPOLOGY = ("faces", "edges", "vertex_object", "face_object", "edge_object")
+
+    def __init__(self, topo: Topology, n_vertices: int, device: str | torch.device = "cuda",
+                 host: dict | None = None):
+        """host: pinned copies of the topology (pinned_topology); the uploads
+        are then asynchronous on the current stream."""
         self.device = torch.device(device)
 
-        def up(a):
-            return torch.from_numpy(np.ascontiguousarray(a, np.int32)).to(self.device)
+        def up(name):
+            if host is not None:
+                return host[name].to(self.device, non_blocking=True)
+            return torch.from_numpy(np.ascontiguousarray(getattr(topo, name), np.int32)).to(self.device)
 
-        self.faces, self.edges = up(topo.faces), up(topo.edges)
-        self.vobj, self.fobj, self.eobj = up(topo.vertex_object), up(topo.face_object), up(topo.edge_object)
+        self.faces, self.edges = up("faces"), up("edges")
+        self.vobj, self.fobj, self.eobj = up("vertex_object"), up("face_object"), up("edge_object")
         self.n_vertices, self.n_faces, self.n_edges = n_vertices, topo.faces.shape[0], topo.edges.shape[0]
         self.n_objects = topo.object_count
         self.ws_bytes = self.workspace_bytes(n_vertices, self.n_faces, self.n_edges, self.n_objects)
@@ -174,6 +181,11 @@ class ResidentMesh:
         self.mesh = _Mesh(self.faces.data_ptr(), self.edges.data_ptr(), self.vobj.data_ptr(), self.fobj.data_ptr(),
                           self.eobj.data_ptr(), self.n_vertices, self.n_faces, self.n_edges, self.n_objects, 0,
                           None, 0)
+
+    @classmethod
+    def pinned_topology(cls, topo: Topology) -> dict:
+        return {k: torch.from_numpy(np.ascontiguousarray(getattr(topo, k), np.int32)).pin_memory()
+                for k in cls.TOPOLOGY}
 
     @staticmethod
     def workspace_bytes(n_vertices: int, n_faces: int, n_edges: int, n_objects: int) -> int:

@@ -143,9 +143,9 @@ CACHE_SLOTS = 16  # hit-cache partners per query of a batch (most queries of the
 class ResidentScene:
     """A scene's topology and broad-phase state on the device."""
 
-    def __init__(self, scene: scenes.Scene, device):
+    def __init__(self, scene: scenes.Scene, device, host: dict | None = None):
         self.scene = scene
-        self.mesh = ResidentMesh(scene.topo, scene.n_vertices, device)
+        self.mesh = ResidentMesh(scene.topo, scene.n_vertices, device, host)
 
     @staticmethod
     def estimate_bytes(scene: scenes.Scene) -> int:
@@ -216,9 +216,12 @@ class Server:
         self.lanes = [Lane(self.device, cfg, cache_slots) for _ in range(lanes)]
         self.batches = 0
         self.loads = 0
+        self._host_topo: dict[int, dict] = {}  # pinned topology of every scene loaded so far
 
     # -- residency ---------------------------------------------------------
-    def _make_resident(self, sid: int) -> bool:
+    def _make_resident(self, sid: int, stream) -> bool:
+        """Resident or loaded now (topology from a pinned host copy, uploaded
+        on `stream`, the stream of the lane whose batch needs it)."""
         if sid in self.resident:
             self.res.touch(sid)
             return True
@@ -229,13 +232,17 @@ class Server:
         for victim in plan:
             del self.resident[victim]
             self.res.remove(victim)
-        self.resident[sid] = ResidentScene(self.table[sid], self.device)
+        host = self._host_topo.get(sid)
+        if host is None:
+            host = self._host_topo[sid] = ResidentMesh.pinned_topology(self.table[sid].topo)
+        with torch.cuda.stream(stream):
+            self.resident[sid] = ResidentScene(self.table[sid], self.device, host)
         self.res.add(sid, need)
         self.loads += 1
         return True
 
     # -- batches -----------------------------------------------------------
-    def _admit(self, queue: collections.deque) -> list:
+    def _admit(self, queue: collections.deque, stream) -> list:
         batch, pairs, skipped = [], 0, collections.deque()
         busy_now = set()
         while queue and len(batch) < self.max_batch_requests:
@@ -245,7 +252,7 @@ class Server:
             if r.scene_id in self.res.busy or r.scene_id in busy_now or (batch and pairs + est > self.max_batch_pairs):
                 skipped.append(r)
                 continue
-            if not self._make_resident(r.scene_id):
+            if not self._make_resident(r.scene_id, stream):
                 skipped.append(r)
                 continue
             batch.append(r)
@@ -359,7 +366,7 @@ class Server:
                     launched.remove(lane)
             free = [lane for lane in self.lanes if lane.batch is None]
             if free and queue:
-                reqs = self._admit(queue)
+                reqs = self._admit(queue, free[0].stream)
                 if reqs:
                     self._launch(free[0], reqs)
                     launched.append(free[0])
@@ -447,6 +454,35 @@ def poisson_workload(n_scenes: int, n_requests: int, rate_hz: float, min_tri: in
     return table, reqs
 
 
+def poisson_requests(table: dict, n_requests: int, rate_hz: float, seed: int) -> list:
+    """More requests over an existing scene table: Poisson arrivals at
+    rate_hz, scenes drawn uniformly."""
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate_hz, n_requests))
+    ids = sorted(table)
+    sids = rng.integers(0, len(ids), n_requests)
+    return [step_request(k, ids[int(s)], table[ids[int(s)]], seed + 1000 + k, float(a)) for k, (s, a) in
+            enumerate(zip(sids, arrivals))]
+
+
+def footprint(table: dict) -> int:
+    """Device bytes of every scene resident at once."""
+    return sum(ResidentScene.estimate_bytes(s) for s in table.values())
+
+
+def serve_timed(srv: Server, reqs: list) -> dict:
+    """Serves `reqs` in real time on a warmed server; the report plus the
+    batches, scene loads and evictions of this run."""
+    b, loads, ev = srv.batches, srv.loads, srv.res.evictions
+    t = time.perf_counter()
+    done = srv.run(reqs, realtime=True)
+    wall = time.perf_counter() - t
+    rep = report(done, wall)
+    rep.update(batches=srv.batches - b, scene_loads_during_run=srv.loads - loads, evictions=srv.res.evictions - ev,
+               wall_s=wall)
+    return rep
+
+
 def report(done: list, wall: float) -> dict:
     lat = np.array([c.latency_ms for c in done])
     pairs = int(sum(c.pairs for c in done))
@@ -464,23 +500,23 @@ def main() -> int:
     ap.add_argument("--rate", type=float, default=400.0, help="arrivals per second")
     ap.add_argument("--max-triangles", type=int, default=500_000)
     ap.add_argument("--budget-gb", type=float, default=64.0)
+    ap.add_argument("--budget-frac", type=float, default=0.0,
+                    help="HBM budget as a fraction of every scene's resident footprint (overrides --budget-gb)")
     args = ap.parse_args()
     t = time.perf_counter()
     table, reqs = poisson_workload(args.scenes, args.requests, args.rate, max_tri=args.max_triangles)
-    print(f"[serve] built {len(table)} scenes in {time.perf_counter() - t:.1f} s", flush=True)
-    srv = Server(table, int(args.budget_gb * (1 << 30)))
-    # make every scene resident and size its buffers with one step each
+    fp = footprint(table)
+    budget = int(args.budget_frac * fp) if args.budget_frac > 0 else int(args.budget_gb * (1 << 30))
+    print(f"[serve] built {len(table)} scenes in {time.perf_counter() - t:.1f} s; footprint {fp / 2**30:.2f} GiB, "
+          f"budget {budget / 2**30:.2f} GiB", flush=True)
+    srv = Server(table, budget)
+    # make every scene resident (as far as the budget allows) and size the lanes with one step each
     warm = [step_request(-1 - i, sid, table[sid], 99 + i) for i, sid in enumerate(sorted(table))]
     srv.run(warm, realtime=False)
-    loads = srv.loads
-    print(f"[serve] {loads} scenes resident ({srv.res.used / 2**30:.2f} GiB)", flush=True)
-    t = time.perf_counter()
-    done = srv.run(reqs, realtime=True)
-    wall = time.perf_counter() - t
-    print(f"[serve] served {len(done)} requests in {wall:.1f} s", flush=True)
-    rep = report(done, wall)
-    rep.update(scenes=args.scenes, rate_hz=args.rate, batches=srv.batches, scene_loads_during_run=srv.loads - loads,
-               evictions=srv.res.evictions, budget_gb=args.budget_gb, wall_s=wall)
+    print(f"[serve] {len(srv.resident)} scenes resident ({srv.res.used / 2**30:.2f} GiB)", flush=True)
+    rep = serve_timed(srv, reqs)
+    print(f"[serve] served {rep['requests']} requests in {rep['wall_s']:.1f} s", flush=True)
+    rep.update(scenes=args.scenes, rate_hz=args.rate, budget_gb=budget / 2**30, footprint_gb=fp / 2**30)
     print(json.dumps(rep), flush=True)
     return 0
 
