@@ -339,6 +339,75 @@ def rows_suite() -> int:
                                                      "the oracle restatement")},
              None, {"bit_exact": same})
 
+    # exact work queue (ProcessExactWorkQueueCpu + refinement children) on the device
+    log("rows: exact work queue")
+    from p2cccd import exact_queue as eq
+
+    qe, qpts, qees = workloads.exact_queue(100_000)
+    dq = eq.DeviceQueue(qpts, qees, dev)
+    d_entries = dq.upload_entries(qe)
+    pt_pairs, ee_pairs = eq._pair_bounds(qe)
+    refine = eq.RefineConfig()
+    holder = {}
+    ms = _gpu_ms(lambda: holder.__setitem__(0, dq.run(d_entries, pt_pairs, ee_pairs, eq.QueueConfig(), refine)))
+    got_c = holder[0][0].cpu().numpy().view(eq.CERTIFICATE_DTYPE)
+    got_r = holder[0][1].cpu().numpy().view(eq.AUDIT_ROW_DTYPE)
+    compiled = oracle.ref_available()
+    ref = [None]
+    t_cpu = _cpu_s(lambda: ref.__setitem__(0, oracle.queue_process(qe, qpts, qees, compiled=compiled)))
+    same = bool(np.array_equal(got_c.view(np.uint8), ref[0][0].view(np.uint8)) and
+                np.array_equal(got_r.view(np.uint8), ref[0][1].view(np.uint8)))
+    _row("exact_work_queue", f"{len(qe)} work items over 10k point-triangle and 10k edge-edge primitives (masks 1-3, "
+         "0-4 primitives per family, eps_time 1e-4, depth 32): certificates, audit rows, refinement children",
+         {"value": len(qe) / (ms * 1e-3), "unit": "work items/s", "ms": ms, "children": int(holder[0][3])},
+         {"value": len(qe) / t_cpu, "unit": "work items/s", "cores": 1, "kind": "reference" if compiled else "port",
+          "sample": "the whole queue through " + ("the compiled ProcessExactWorkQueueCpu" if compiled else
+                                                  "the oracle restatement")},
+         None, {"certificates_and_audit_rows_identical": same})
+
+    # external-batch candidates (GenerateCandidatesForExternalBatch)
+    log("rows: external batch candidates")
+    import p2cccd_cpp as cpp
+    from types import SimpleNamespace as NS
+
+    n_ext = 100_000
+    rng = np.random.default_rng(7)
+    xq = np.zeros(n_ext, oracle.EXT_QUERY)
+    xq["source_query_id"] = np.arange(n_ext) * 3 + 1
+    xq["source_query_index"] = np.arange(n_ext)
+    xq["family"] = rng.integers(0, 2, n_ext)
+    xq["v0"] = rng.uniform(-1, 1, (n_ext, 1, 3)) + rng.uniform(-0.2, 0.2, (n_ext, 4, 3))
+    xq["v1"] = xq["v0"] + rng.normal(0, 0.05, (n_ext, 4, 3))
+    def ext_batch(rows):
+        return NS(queries=[NS(query_id=int(r["source_query_id"]), source_query_index=int(r["source_query_index"]),
+                              family="vf" if r["family"] == 0 else "ee", vertices_t0=r["v0"].tolist(),
+                              vertices_t1=r["v1"].tolist(), ground_truth_collides=None, box_pair=None)
+                           for r in rows])
+
+    batch = ext_batch(xq)
+    cpp.generate_candidates_for_external_batch(ext_batch(xq[:1000]))  # warm
+    t0 = time.perf_counter()
+    res = cpp.generate_candidates_for_external_batch(batch)
+    wall = time.perf_counter() - t0
+    # the reference tests every proxy pair of the batch (its cost grows with n^2): a 5,000-query sample, and the
+    # device on the same sample for the parity check
+    n_cpu = 5000
+    cpu, same = None, None
+    if oracle.ref_available():
+        ref = [None]
+        t_cpu = _cpu_s(lambda: ref.__setitem__(0, oracle.ref_external_batch(xq[:n_cpu])))
+        small = cpp.generate_candidates_for_external_batch(ext_batch(xq[:n_cpu]))
+        same = bool(np.array_equal(np.array([c.candidate_id for c in small.candidates], np.uint64),
+                                   ref[0][0]["candidate_id"]))
+        cpu = {"value": n_cpu / t_cpu, "unit": "queries/s", "cores": 1, "kind": "reference",
+               "sample": f"the first {n_cpu} queries through the compiled external_batch_candidate.cpp (all proxy "
+                         "pairs: per query its cost grows with the batch, so the 100k batch runs slower per query)"}
+    _row("external_batch_candidates", f"{n_ext} external CCD queries (VF and EE, starts U[-1,1]^3, jitter 0.2, "
+         f"motion N(0, 0.05)): {len(res.candidates)} candidates",
+         {"value": n_ext / (res.timing.trace_ms * 1e-3), "unit": "queries/s", "device_ms": res.timing.trace_ms,
+          "call_ms": wall * 1e3, "note": "call_ms adds the pybind conversion of the query objects and the copies"},
+         cpu, None, {"identical_candidate_ids_on_the_sample": same})
+
     # full plain pipeline on two level-6 icospheres
     log("rows: two level-6 icospheres, full step")
     sc = scenes.two_icospheres()
@@ -386,7 +455,13 @@ def rows_suite() -> int:
     log("rows: scorer kernels")
     packed = stpf.load_stpf(stpf.DEFAULT_PRIORITY_WEIGHTS, dev)
     rng = np.random.default_rng(3)
-    feats = torch.from_numpy(rng.normal(size=(65536, 32)).astype(np.float32)).to(dev)
+    from p2cccd import train_stpf
+
+    # the group features of the icosphere step's spatial cells (VF and EE), repeated to 65,536 groups: the
+    # trained first layer carries the features' normalisation, so inputs follow their real distribution
+    cells = train_stpf.scene_cells(TuvCertifier(device=dev), sc)
+    real = torch.cat([d["gfeat"] for d in cells])
+    feats = real.repeat((65536 + real.shape[0] - 1) // real.shape[0], 1)[:65536].contiguous()
     ms = _gpu_ms(lambda: stpf.stpf_priority(packed, feats))
     model = stpf.StpfPriority()
     p = packed.cpu()
@@ -401,7 +476,8 @@ def rows_suite() -> int:
         t_cpu = _cpu_s(lambda: model(fc))
     got = stpf.stpf_priority(packed, feats).cpu()
     ach = 65536 * STPF_FLOPS_ROW / (ms * 1e-3) / 1e12
-    _row("stpf_priority_scorer", "65,536 groups x 32 FP32 features, 32-128-128-1 GELU, sigmoid",
+    _row("stpf_priority_scorer", f"65,536 groups x 32 FP32 features ({real.shape[0]} spatial cells of the icosphere "
+         "step, repeated), 32-128-128-1 GELU, sigmoid",
          {"value": 65536 / (ms * 1e-3), "unit": "groups/s", "ms": ms},
          {"value": 65536 / t_cpu, "unit": "groups/s", "cores": 1, "kind": "port",
           "sample": "the same model in torch on the CPU, FP32, one thread"},

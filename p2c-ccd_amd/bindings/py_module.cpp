@@ -39,10 +39,11 @@ std::vector<PrimitiveIntervalResult> BatchOrThrow(
 }
 
 // Backend names of ParseCandidateBackend (py_module.cpp:87-95) plus this
-// build's own; every accepted name runs the gfx950 path.
+// build's own ("hip_exact"); every accepted name runs the gfx950 all-pairs
+// proxy kernel with the CPU reference's FP64 semantics.
 void CheckCandidateBackend(const std::string& name) {
   static const char* const kNames[] = {"cpu_reference", "cpu_reference_rt", "optix", "optix_compatible",
-                                       "optix_rt",      "hip_exact",        "hip_lbvh"};
+                                       "optix_rt", "hip_exact"};
   for (const char* n : kNames)
     if (name == n) return;
   throw py::value_error("unsupported candidate backend: " + name);

@@ -217,6 +217,7 @@ class BatchBroadPhase:
         self.offsets = torch.zeros(2, dtype=torch.int64, device=self.device)
         self.scenes = None
         self.n_scenes = 0
+        self.slots = self.cache_slots
 
     def _describe(self, items) -> None:
         arr = (_Scene * len(items))()
@@ -224,28 +225,30 @@ class BatchBroadPhase:
             arr[k] = _Scene(m.mesh, x0, x1, float(th), m.workspace.data_ptr(), m.ws_bytes)
         self.scenes, self.n_scenes = arr, len(items)
 
-    def count(self, items, stream=None) -> torch.Tensor:
+    def count(self, items, stream=None, cache_slots: int | None = None) -> torch.Tensor:
         """items: (ResidentMesh, x0 address, x1 address, thickness) per scene,
-        positions device (V, 3) float64.  Returns the device offsets
+        positions device (V, 3) float64; cache_slots overrides the default
+        for this batch (and its emit).  Returns the device offsets
         (int64[2 S + 2], p2c_bp_batch_count)."""
         if not items:
             raise ValueError("a batch holds at least one scene")
         self._describe(items)
-        need = int(nat.lib.p2c_bp_batch_workspace_bytes(self.scenes, self.n_scenes, self.cache_slots))
+        self.slots = int(self.cache_slots if cache_slots is None else cache_slots)
+        need = int(nat.lib.p2c_bp_batch_workspace_bytes(self.scenes, self.n_scenes, self.slots))
         if need == 0:
             raise ValueError("invalid batch")
         if self.workspace.numel() < need:
             self.workspace = torch.empty(int(need * 1.25) + (1 << 20), dtype=torch.uint8, device=self.device)
         if self.offsets.numel() < 2 * self.n_scenes + 2:
             self.offsets = torch.zeros(2 * self.n_scenes + 2, dtype=torch.int64, device=self.device)
-        nat.call("p2c_bp_batch_count", self.scenes, self.n_scenes, self.cache_slots, self.workspace.data_ptr(),
+        nat.call("p2c_bp_batch_count", self.scenes, self.n_scenes, self.slots, self.workspace.data_ptr(),
                  self.workspace.numel(), self.offsets.data_ptr(), nat.stream_handle(stream))
         return self.offsets[: 2 * self.n_scenes + 2]
 
     def emit(self, pairs: torch.Tensor, stream=None) -> None:
         """Fills `pairs` (int32 (cap, 2)) for the batch of the last count()."""
         assert pairs.dtype == torch.int32 and pairs.is_contiguous()
-        nat.call("p2c_bp_batch_emit", self.scenes, self.n_scenes, self.cache_slots, self.workspace.data_ptr(),
+        nat.call("p2c_bp_batch_emit", self.scenes, self.n_scenes, self.slots, self.workspace.data_ptr(),
                  self.workspace.numel(), pairs.data_ptr(), pairs.shape[0], nat.stream_handle(stream))
 
     def candidates(self, items) -> tuple[np.ndarray, torch.Tensor]:

@@ -137,7 +137,19 @@ class Residency:
 
 
 # ---- device state -------------------------------------------------------------
-CACHE_SLOTS = 16  # hit-cache partners per query of a batch (most queries of these scenes have fewer partners)
+CACHE_SLOTS = 16  # hit-cache partners per query of a batch of sparse scenes (most of their queries have fewer)
+CACHE_BYTES = 1 << 30  # the batch hit cache grows with the batch's pairs per query up to this size
+
+
+def cache_slots(queries: int, pairs_estimate: int, base: int = CACHE_SLOTS) -> int:
+    """Hit-cache partners per query for a batch: doubled from `base` while
+    under 4x the batch's pairs per query (the count pass then keeps nearly
+    every query's partners and the fill re-traverses few), at most 256 and at
+    most CACHE_BYTES of cache."""
+    slots = base
+    while slots < 256 and slots < 4 * pairs_estimate / max(queries, 1) and 4 * queries * slots * 2 <= CACHE_BYTES:
+        slots *= 2
+    return slots
 
 
 class ResidentScene:
@@ -286,7 +298,9 @@ class Server:
                 items.append((rs.mesh, x0.data_ptr() + 24 * int(base[k]), x1.data_ptr() + 24 * int(base[k]),
                               rs.scene.thickness))
                 tab[k] = (rs.mesh.faces.data_ptr(), rs.mesh.edges.data_ptr(), base[k])
-            offs_dev = lane.bp.count(items, stream=s)
+            queries = sum(self.table[r.scene_id].n_vertices + len(self.table[r.scene_id].topo.edges) for r in reqs)
+            est = sum(self.last_pairs.get(r.scene_id, 0) for r in reqs)
+            offs_dev = lane.bp.count(items, stream=s, cache_slots=cache_slots(queries, est, lane.bp.cache_slots))
             offs = offs_dev.cpu().numpy().astype(np.int64)  # the counts size this batch; waits on this lane only
         if offs[2 * S + 1]:
             raise RuntimeError(f"{offs[2 * S + 1]} BVH traversals exceeded their step bound")

@@ -110,17 +110,26 @@ class TuvCertifier:
         self.state_bytes = int(nat.lib.p2c_tuv_state_bytes())
         self.tasks = torch.empty(max(self.cfg.split_tasks, 1) * int(nat.lib.p2c_tuv_task_bytes()), dtype=torch.uint8,
                                  device=self.device)
+        self._full = torch.empty(0, dtype=torch.uint8, device=self.device)
         self.verts = np.zeros(1, VERTICES_DTYPE)  # set by setup_pairs(lite=True)
         self.lite_active = False  # the last setup wrote compact records
 
     def reserve(self, n: int) -> None:
         self._reserve(n)
 
+    @property
+    def pairs(self) -> torch.Tensor:
+        """The 288-byte records, allocated on first use: the compact form
+        (64 bytes per pair) needs none."""
+        need = int(nat.lib.p2c_tuv_pairs_bytes(self._cap))
+        if self._full.numel() < need:
+            self._full = torch.empty(need, dtype=torch.uint8, device=self.device)
+        return self._full
+
     def _reserve(self, n: int) -> None:
         if n <= self._cap:
             return
         cap = (max(n, 1024) + 63) // 64 * 64  # pair records are stored in tiles of 64
-        self.pairs = torch.empty(int(nat.lib.p2c_tuv_pairs_bytes(cap)), dtype=torch.uint8, device=self.device)
         self.lite = torch.empty(int(nat.lib.p2c_tuv_lite_bytes(cap)), dtype=torch.uint8, device=self.device)
         self.results = torch.empty(cap * TUV_RESULT_DTYPE.itemsize, dtype=torch.uint8, device=self.device)
         self.replay_list = torch.empty(cap, dtype=torch.int32, device=self.device)

@@ -102,3 +102,35 @@ def make_query_set(n_vf: int = 10_000, n_ee: int = 10_000, seed: int = BASE_SEED
     vf, vk = one(n_vf, _touch_vf, _degenerate_vf)
     ee, ek = one(n_ee, _touch_ee, _degenerate_ee)
     return QuerySet(vf=vf, ee=ee, vf_kind=vk, ee_kind=ek)
+
+
+def exact_queue(n: int, pool: int = 10_000, seed: int = BASE_SEED):
+    """An exact work queue of n entries (p2cccd.exact_queue.QUEUE_ENTRY_DTYPE)
+    over pools of point-triangle and edge-edge primitives taken from the plain
+    query set: masks 1, 2 and 3, full and partial intervals, 0-4 primitives
+    of each family per entry, the reference's default certificate config
+    (eps_time 1e-4, eps_space 1e-6, depth 32).  Returns (entries, pts, ees)."""
+    from . import _native
+    from .exact_queue import QUEUE_ENTRY_DTYPE
+
+    rng = np.random.Generator(np.random.PCG64(seed + 11))
+    qs = make_query_set(pool, pool, seed=seed)
+    pts = _native.prims_from_positions(qs.vf)
+    ees = _native.prims_from_positions(qs.ee)
+    e = np.zeros(n, QUEUE_ENTRY_DTYPE)
+    e["work_item_id"] = np.arange(1, n + 1, dtype=np.uint64)
+    e["parent_candidate_id"] = rng.integers(1, 1 << 40, n).astype(np.uint64)
+    e["query_id"] = rng.integers(1, max(2, n // 4), n).astype(np.uint64)
+    e["feature_family_mask"] = rng.choice([1, 2, 3], n, p=[0.4, 0.4, 0.2])
+    t = np.sort(rng.uniform(0.0, 1.0, (n, 2)), axis=1)
+    full = rng.uniform(size=n) < 0.5
+    t[full] = (0.0, 1.0)
+    e["interval_t0"], e["interval_t1"] = t[:, 0], t[:, 1]
+    e["max_subdivision_depth"] = 32
+    e["eps_time"] = 1.0e-4
+    e["eps_space"] = 1.0e-6
+    for fam in ("pt", "ee"):
+        count = rng.integers(0, 5, n)
+        e[f"{fam}_count"] = count
+        e[f"{fam}_begin"] = rng.integers(0, pool - count + 1)
+    return e, pts, ees
