@@ -627,7 +627,7 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_round_kernel(const TuvPair* __r
 constexpr int kStreamChunk = 256;
 
 template <typename Src>
-__global__ __launch_bounds__(kTuvBlock) void tuv_stream_kernel(Src src, int64_t n_host,
+__global__ __launch_bounds__(kTuvBlock, 3) void tuv_stream_kernel(Src src, int64_t n_host,
                                                                const uint64_t* __restrict__ totals, double tol,
                                                                uint64_t light_budget, uint32_t refill_at,
                                                                uint32_t handover_at,
