@@ -100,7 +100,9 @@ __device__ inline void write_result(p2c_tuv_result* out, uint64_t best, uint64_t
 // stream, so a refill reads 64 bytes from HBM instead of 288.
 struct TuvLite {
   int32_t vid[4];
-  uint64_t split[5];
+  uint64_t tmask[2];
+  uint64_t umask[2];
+  uint64_t split_hi;
   uint32_t done_level;
   uint32_t is_ee;
 };
@@ -157,8 +159,11 @@ struct LiteSrc {
     TuvPair q;
     tuv_setup_geometry(&q, v, int(r.is_ee), ms);
     q.inv_tol[0] = q.inv_tol[1] = q.inv_tol[2] = 0.0;  // used only to build the split sequence
-#pragma unroll
-    for (int w = 0; w < 5; ++w) q.split[w] = r.split[w];
+    q.tmask[0] = r.tmask[0];
+    q.tmask[1] = r.tmask[1];
+    q.umask[0] = r.umask[0];
+    q.umask[1] = r.umask[1];
+    q.split_hi = r.split_hi;
     q.done_level = r.done_level;
     return q;
   }
@@ -260,7 +265,11 @@ __device__ inline TuvLite lite_record(const double* __restrict__ x0, const doubl
   tuv_setup(&q, x, is_ee, ms, tol);
   TuvLite r;
   for (int j = 0; j < 4; ++j) r.vid[j] = vid[j];
-  for (int w = 0; w < 5; ++w) r.split[w] = q.split[w];
+  for (int w = 0; w < 2; ++w) {
+    r.tmask[w] = q.tmask[w];
+    r.umask[w] = q.umask[w];
+  }
+  r.split_hi = q.split_hi;
   r.done_level = q.done_level;
   r.is_ee = q.is_ee;
   return r;
@@ -388,7 +397,8 @@ __device__ inline bool box_skipped(const TuvPair& q, const LaneWalk& w) {
 }
 
 // Evaluates the box the lane stands on: 0 reject, 1 done, 2 split.
-__device__ inline int evaluate(const TuvPair& q, const LaneWalk& w, double tol) {
+template <typename W>
+__device__ inline int evaluate(const TuvPair& q, const W& w, double tol) {
   const int e[3] = {exp_of(w.ep, 0), exp_of(w.ep, 1), exp_of(w.ep, 2)};
   return tuv_classify_at(q, key_to_f64(w.key[0]), key_to_f64(w.key[1]), key_to_f64(w.key[2]), e, w.level, tol);
 }
@@ -454,6 +464,38 @@ __device__ inline bool lane_step(const TuvPair& q, LaneWalk& w, double tol, uint
 #pragma unroll
   for (int m = 0; m < kAdvanceMoves; ++m) {
     if (!advance_once(q, w, 0)) return true;
+    if (w.ready) return false;
+  }
+  return false;
+}
+
+#ifndef P2C_JUMPS
+#define P2C_JUMPS 1
+#endif
+constexpr int kJumps = P2C_JUMPS;
+
+// One step of the jump walk: evaluates the box the lane stands on (when
+// ready), then descends or makes at most kJumps jumps.  Returns true when the
+// walk has finished; sets *over (leaving the box unevaluated) when the box
+// budget ran out.
+__device__ inline bool jump_step(const TuvPair& q, JumpWalk& w, double tol, uint64_t budget, bool* over) {
+  if (w.ready) {
+    if (w.boxes >= budget) {
+      *over = true;
+      return true;
+    }
+    ++w.boxes;
+    const int outcome = evaluate(q, w, tol);
+    if (outcome == 2) {
+      jump_descend(q, w);
+      return false;
+    }
+    if (outcome == 1 && w.key[0] < w.best) w.best = w.key[0];
+    w.ready = false;
+  }
+#pragma unroll
+  for (int m = 0; m < kJumps; ++m) {
+    if (!jump_next(q, w)) return true;
     if (w.ready) return false;
   }
   return false;
@@ -527,6 +569,21 @@ static_assert(sizeof(WalkState) == 48, "walk state layout");
 static_assert(kTuvMaxLevel < 256, "pending sibling counts fit 8 bits");
 
 __device__ inline void save_walk(WalkState* st, const LaneWalk& w) {
+  WalkState s;
+  s.key[0] = w.key[0];
+  s.key[1] = w.key[1];
+  s.key[2] = w.key[2];
+  s.best = w.best;
+  s.boxes = w.boxes;
+  s.level = (uint16_t)w.level;
+  s.e[0] = (uint8_t)exp_of(w.ep, 0);
+  s.e[1] = (uint8_t)exp_of(w.ep, 1);
+  s.e[2] = (uint8_t)exp_of(w.ep, 2);
+  s.ready = w.ready ? 1 : 0;
+  *st = s;
+}
+
+__device__ inline void save_walk(WalkState* st, const JumpWalk& w) {
   WalkState s;
   s.key[0] = w.key[0];
   s.key[1] = w.key[1];
@@ -655,7 +712,7 @@ __global__ __launch_bounds__(kTuvBlock, 3) void tuv_stream_kernel(Src src, int64
   bool drained = false;       // no pairs left to claim (wave-uniform)
   int64_t pair = -1;
   TuvPair q;
-  LaneWalk w;
+  JumpWalk w;
   for (;;) {
     uint64_t idle = __ballot(pair < 0);
     if (!drained && (uint32_t)__popcll(idle) >= refill_at) {
@@ -686,7 +743,12 @@ __global__ __launch_bounds__(kTuvBlock, 3) void tuv_stream_kernel(Src src, int64
         if (pair < 0 && rank < avail) {
           pair = next + rank;
           q = src.load(pair);
-          start_walk(w);
+          start_jump_walk(w);
+          if (q.done_level > kTuvMaskLevels) {  // past the jump walk's masks: to the wave pass
+            save_walk(&states[pair], w);
+            flag_for_replay((uint32_t)pair, heavy_list, heavy_count, list_cap);
+            pair = -1;
+          }
         }
         const int64_t took = avail < __popcll(idle) ? avail : (int64_t)__popcll(idle);
         next += took;
@@ -704,7 +766,7 @@ __global__ __launch_bounds__(kTuvBlock, 3) void tuv_stream_kernel(Src src, int64
     }
     if (pair >= 0) {
       bool over = false;
-      if (lane_step(q, w, tol, light_budget, &over)) {
+      if (jump_step(q, w, tol, light_budget, &over)) {
         if (over) {
           save_walk(&states[pair], w);
           flag_for_replay((uint32_t)pair, heavy_list, heavy_count, list_cap);

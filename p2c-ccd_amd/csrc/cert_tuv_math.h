@@ -29,13 +29,22 @@ constexpr int kTuvMaxLevel = 3 * kTuvMaxExp;
 constexpr uint64_t kOne63 = 1ull << 63;
 constexpr uint64_t kTuvNoHit = ~0ull;
 
-// Per-pair constants (288 bytes).
+constexpr uint32_t kTuvMaskLevels = 128;  // levels held as one-hot masks
+
+// Per-pair constants (288 bytes).  The split sequence (the dimension each
+// level splits) is held as one-hot level masks for levels 0..127 -- bit l of
+// tmask (umask) set when level l splits t (u); a level below done_level with
+// neither bit splits v -- and as 2-bit codes (3 = none) for levels 128..155,
+// which only pairs with extreme extents reach.  Masks make the exponents of a
+// level two population counts.
 struct TuvPair {
   double x0[4][3];   // vertex positions at t = 0
   double d[4][3];    // x1 - x0
   double err[3];     // corner-evaluation error bound + minimum separation
   double inv_tol[3]; // 3 L_k / tolerance
-  uint64_t split[5]; // split dimension per level, 2 bits each (3 = none: done level)
+  uint64_t tmask[2]; // levels 0..127 that split t
+  uint64_t umask[2]; // levels 0..127 that split u
+  uint64_t split_hi; // levels 128..155: split dimension, 2 bits each
   uint32_t done_level;
   uint32_t is_ee;
 };
@@ -130,16 +139,49 @@ P2C_TUV_HD void put3(T v[3], int d, T x) {
   v[2] = d == 2 ? x : v[2];
 }
 
-// Split dimension of `level`.  The word is chosen with constant indices so
-// the pair record stays in registers (a dynamic index would move it to
-// scratch memory).
-// Levels below 64 (the first two words) take one select; deeper levels,
-// which few walks reach, a branch the wave skips when no lane needs it.
+// Split dimension of a level below kTuvMaskLevels (and below done_level).
+// Words are chosen with constant indices so the pair record stays in
+// registers (a dynamic index would move it to scratch memory).
+P2C_TUV_HD int tuv_mask_dim(const TuvPair& q, uint32_t level) {
+  const bool hi = level >= 64;
+  const uint64_t bit = 1ull << (level & 63u);
+  const uint64_t t = hi ? q.tmask[1] : q.tmask[0];
+  const uint64_t u = hi ? q.umask[1] : q.umask[0];
+  return (t & bit) ? 0 : ((u & bit) ? 1 : 2);
+}
+
+// Split dimension of `level` (below done_level); levels past the masks, which
+// few walks reach, take a branch the wave skips when no lane needs it.
 P2C_TUV_HD int tuv_level_dim(const TuvPair& q, uint32_t level) {
-  const uint32_t w = level >> 5;
-  uint64_t word = w == 0 ? q.split[0] : q.split[1];
-  if (w >= 2) word = w == 2 ? q.split[2] : (w == 3 ? q.split[3] : q.split[4]);
-  return (int)((word >> ((level & 31u) * 2u)) & 3ull);
+  if (level < kTuvMaskLevels) return tuv_mask_dim(q, level);
+  return (int)((q.split_hi >> ((level - kTuvMaskLevels) * 2u)) & 3ull);
+}
+
+// Bits 0, 2, 4, ... of x (its odd bits zero) packed into bits 0..31.
+P2C_TUV_HD uint64_t pack_even_bits(uint64_t x) {
+  x = (x | (x >> 1)) & 0x3333333333333333ull;
+  x = (x | (x >> 2)) & 0x0F0F0F0F0F0F0F0Full;
+  x = (x | (x >> 4)) & 0x00FF00FF00FF00FFull;
+  x = (x | (x >> 8)) & 0x0000FFFF0000FFFFull;
+  x = (x | (x >> 16)) & 0x00000000FFFFFFFFull;
+  return x;
+}
+
+// The pair's level masks and high codes from the 2-bit split words (32
+// levels per word, code 0 = t, 1 = u, 2 = v, 3 = none).
+P2C_TUV_HD void tuv_split_masks(const uint64_t split[5], uint64_t tmask[2], uint64_t umask[2], uint64_t* split_hi) {
+  constexpr uint64_t kEven = 0x5555555555555555ull;
+  uint64_t t[4], u[4];
+  for (int w = 0; w < 4; ++w) {
+    const uint64_t b0 = split[w] & kEven, b1 = (split[w] >> 1) & kEven;
+    t[w] = pack_even_bits(kEven & ~(b0 | b1));  // code 0
+    u[w] = pack_even_bits(b0 & ~b1);            // code 1
+  }
+  tmask[0] = t[0] | (t[1] << 32);
+  tmask[1] = t[2] | (t[3] << 32);
+  umask[0] = u[0] | (u[1] << 32);
+  umask[1] = u[2] | (u[3] << 32);
+  *split_hi = split[4];
 }
 
 // Appends dimension d at `level` to the packed sequence (one register word,
@@ -331,7 +373,9 @@ P2C_TUV_HD void tuv_setup(TuvPair* q, const double v[8][3], int is_ee, double ms
         L[2] = dv > L[2] ? dv : L[2];
       }
   for (int k = 0; k < 3; ++k) q->inv_tol[k] = (3.0 * L[k]) / tolerance;
-  tuv_split_sequence(q->inv_tol, q->split, &q->done_level);
+  uint64_t split[5];
+  tuv_split_sequence(q->inv_tol, split, &q->done_level);
+  tuv_split_masks(split, q->tmask, q->umask, &q->split_hi);
 }
 
 // Box outcome: 0 = no zero inside, 1 = done (certifies a hit at its t_lo),
@@ -363,10 +407,17 @@ P2C_TUV_HD int tuv_classify(const TuvPair& q, uint64_t kt, uint64_t ku, uint64_t
   return tuv_classify_at(q, key_to_f64(kt), key_to_f64(ku), key_to_f64(kv), e, level, tolerance);
 }
 
-// Exponents of the boxes at `level` (replays the split sequence).
+// Exponents of the boxes at `level` (at most done_level): the splits of each
+// dimension on levels 0..level-1, counted in the masks, then level by level
+// past them.
 P2C_TUV_HD void tuv_exponents(const TuvPair& q, uint32_t level, int e[3]) {
-  int e0 = 0, e1 = 0, e2 = 0;
-  for (uint32_t l = 0; l < level; ++l) {
+  const uint32_t lm = level < kTuvMaskLevels ? level : kTuvMaskLevels;
+  const uint64_t m0 = lm >= 64 ? ~0ull : ((1ull << lm) - 1ull);
+  const uint64_t m1 = lm >= 128 ? ~0ull : (lm > 64 ? ((1ull << (lm - 64)) - 1ull) : 0ull);
+  int e0 = __builtin_popcountll(q.tmask[0] & m0) + __builtin_popcountll(q.tmask[1] & m1);
+  int e1 = __builtin_popcountll(q.umask[0] & m0) + __builtin_popcountll(q.umask[1] & m1);
+  int e2 = (int)lm - e0 - e1;
+  for (uint32_t l = kTuvMaskLevels; l < level; ++l) {
     const int d = tuv_level_dim(q, l);
     e0 += d == 0;
     e1 += d == 1;
@@ -428,6 +479,91 @@ P2C_TUV_HD bool tuv_walk(const TuvPair& q, uint64_t kt, uint64_t ku, uint64_t kv
     }
     if (!visit) return true;
   }
+}
+
+// The stream kernel's walk: the same depth-first pre-order as tuv_walk (the same boxes, evaluated in the same order, pruned by the same
+// rules), with the climb to the next box made in one jump instead of one
+// level per move.  low[] marks the levels 0..127 at which the path took the
+// lower child while the upper sibling is still to be visited; an upper
+// sibling outside the triangle (VF, u + v > 1) is never marked, since its
+// whole subtree is skipped.  The next box after a finished subtree is the
+// upper sibling at the deepest marked level; its exponents are population
+// counts of the pair's level masks, and its keys keep the bits of the
+// splits above it.  Pairs whose split sequence runs past the masks
+// (done_level > kTuvMaskLevels) are handed to the wave pass unwalked.
+struct JumpWalk {
+  uint64_t key[3];
+  uint64_t best;
+  uint64_t low[2];
+  uint32_t boxes;
+  uint32_t ep;
+  uint32_t level;
+  bool ready;
+};
+
+P2C_TUV_HD void start_jump_walk(JumpWalk& w) {
+  w.key[0] = w.key[1] = w.key[2] = 0;
+  w.best = kTuvNoHit;
+  w.low[0] = w.low[1] = 0;
+  w.boxes = 0;
+  w.ep = 0;
+  w.level = 0;
+  w.ready = true;
+}
+
+// Moves to the lower child of a box that splits (the same lower corner, so it
+// passes every skip test its parent passed) and marks the level when the
+// upper sibling is to be visited later.
+P2C_TUV_HD void jump_descend(const TuvPair& q, JumpWalk& w) {
+  const uint32_t l = w.level;  // < done_level <= kTuvMaskLevels
+  const int d = tuv_mask_dim(q, l);
+  w.ep += 1u << (8 * d);
+  const uint64_t half = kOne63 >> (int)((w.ep >> (8 * d)) & 0xffu);
+  const uint64_t ku = w.key[1] + (d == 1 ? half : 0ull), kv = w.key[2] + (d == 2 ? half : 0ull);
+  const bool visit = q.is_ee || ku + kv <= kOne63;
+  const uint64_t m = visit ? (1ull << (l & 63u)) : 0ull;
+  w.low[0] |= l < 64 ? m : 0ull;
+  w.low[1] |= l < 64 ? 0ull : m;
+  w.level = l + 1;
+}
+
+// Moves to the next box after the current box's subtree; false when the walk
+// is over.  The new box is ready unless its t_lo is no longer below best (then
+// its subtree is skipped by the next jump).
+P2C_TUV_HD bool jump_next(const TuvPair& q, JumpWalk& w) {
+  const bool hi = w.low[1] != 0;
+  const uint64_t x = hi ? w.low[1] : w.low[0];
+  if (x == 0) return false;
+  const uint32_t b = 63u - (uint32_t)__builtin_clzll(x);
+  const uint64_t bit = 1ull << b;
+  w.low[1] = hi ? (x ^ bit) : w.low[1];
+  w.low[0] = hi ? w.low[0] : (x ^ bit);
+  // splits on levels 0..l, l = the marked level
+  const uint64_t upto = bit | (bit - 1ull);
+  const uint64_t m0 = hi ? ~0ull : upto, m1 = hi ? upto : 0ull;
+  const uint32_t l = b + (hi ? 64u : 0u);
+  const uint32_t et = (uint32_t)(__builtin_popcountll(q.tmask[0] & m0) + __builtin_popcountll(q.tmask[1] & m1));
+  const uint32_t eu = (uint32_t)(__builtin_popcountll(q.umask[0] & m0) + __builtin_popcountll(q.umask[1] & m1));
+  const uint32_t ev = l + 1u - et - eu;
+  const bool st = ((hi ? q.tmask[1] : q.tmask[0]) & bit) != 0;
+  const bool su = !st && ((hi ? q.umask[1] : q.umask[0]) & bit) != 0;
+  const bool sv = !st && !su;
+  // keep each key's bits of the splits on levels 0..l (positions >= 63 - e),
+  // then take the upper child at level l
+  const uint64_t s0 = 1ull << (63u - et), s1 = 1ull << (63u - eu), s2 = 1ull << (63u - ev);
+  w.key[0] = (w.key[0] | (st ? s0 : 0ull)) & (0ull - s0);
+  w.key[1] = (w.key[1] | (su ? s1 : 0ull)) & (0ull - s1);
+  w.key[2] = (w.key[2] | (sv ? s2 : 0ull)) & (0ull - s2);
+  w.ep = et | (eu << 8) | (ev << 16);
+  w.level = l + 1u;
+  w.ready = w.key[0] < w.best;
+  return true;
+}
+
+// Evaluates the box a jump walk stands on: 0 reject, 1 done, 2 split.
+P2C_TUV_HD int jump_classify(const TuvPair& q, const JumpWalk& w, double tol) {
+  const int e[3] = {(int)(w.ep & 0xffu), (int)((w.ep >> 8) & 0xffu), (int)((w.ep >> 16) & 0xffu)};
+  return tuv_classify_at(q, key_to_f64(w.key[0]), key_to_f64(w.key[1]), key_to_f64(w.key[2]), e, w.level, tol);
 }
 
 }  // namespace p2c

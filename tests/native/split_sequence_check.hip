@@ -1,6 +1,8 @@
 // Host check: tuv_split_sequence (running ratios, one packed word per 32
 // levels) against the level-by-level tuv_split_dim replay, on special values
-// (0, 1, ties, powers of two, subnormals, inf, NaN) and 2M random inputs.
+// (0, 1, ties, powers of two, subnormals, inf, NaN) and 2M random inputs;
+// then the one-hot level masks built from it (tuv_split_masks) against the
+// packed words: each level's dimension and the exponents at each level.
 #include <cstdio>
 #include <cmath>
 #include <random>
@@ -45,6 +47,30 @@ int main() {
     if (dlo != dln || memcmp(o, w, sizeof o)) {
       if (bad < 5) printf("mismatch %g %g %g: %u vs %u\n", a, b, c, dlo, dln);
       ++bad;
+      return;
+    }
+    // the level masks the kernels read: every level's dimension and the
+    // exponents at every level up to done_level, against the 2-bit words
+    TuvPair q; memset(&q, 0, sizeof q);
+    tuv_split_masks(w, q.tmask, q.umask, &q.split_hi);
+    q.done_level = dln;
+    int e[3] = {0, 0, 0};
+    for (uint32_t l = 0; l <= dln; ++l) {
+      int ex[3];
+      tuv_exponents(q, l, ex);
+      if (ex[0] != e[0] || ex[1] != e[1] || ex[2] != e[2]) {
+        if (bad < 5) printf("exponent mismatch %g %g %g at level %u\n", a, b, c, l);
+        ++bad;
+        return;
+      }
+      if (l == dln) break;
+      const int code = (int)((w[l >> 5] >> ((l & 31u) * 2u)) & 3ull);
+      if (tuv_level_dim(q, l) != code) {
+        if (bad < 5) printf("level mask mismatch %g %g %g at level %u\n", a, b, c, l);
+        ++bad;
+        return;
+      }
+      ++e[code];
     }
   };
   for (int i = 0; i < ns; ++i) for (int j = 0; j < ns; ++j) for (int k = 0; k < ns; ++k) check(specials[i], specials[j], specials[k]);

@@ -37,3 +37,18 @@ def test_corner_range_matches_definition(tmp_path):
     out = subprocess.run([str(exe)], check=False, capture_output=True, text=True, timeout=600)
     assert out.returncode == 0, out.stdout + out.stderr
     assert "8000000 evaluations, 0 mismatches" in out.stdout
+
+
+@pytest.mark.skipif(not Path(HIPCC).exists(), reason="hipcc is not installed")
+def test_jump_walk_matches_depth_first_walk(tmp_path):
+    """The stream kernel's jump walk (the climb to the next box in one jump,
+    from the pending-sibling marks and the level masks) evaluates the same
+    boxes as the plain depth-first walk and finds the same least done t_lo,
+    on uniform, near-contact, coplanar and scaled pairs, past level 64."""
+    exe = tmp_path / "jump_walk_check"
+    subprocess.run([HIPCC, "-O2", "-std=c++17", "-ffp-contract=off", f"-I{ROOT}/p2c-ccd_amd/csrc", f"-I{ROOT}/include",
+                    str(ROOT / "tests/native/jump_walk_check.hip"), "-o", str(exe)], check=True,
+                   stderr=subprocess.DEVNULL)
+    out = subprocess.run([str(exe)], check=False, capture_output=True, text=True, timeout=600)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert " 0 mismatches" in out.stdout
