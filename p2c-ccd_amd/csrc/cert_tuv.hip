@@ -485,7 +485,7 @@ __device__ inline bool jump_step(const TuvPair& q, JumpWalk& w, double tol, uint
       return true;
     }
     ++w.boxes;
-    const int outcome = evaluate(q, w, tol);
+    const int outcome = jump_classify(q, w, tol);
     if (outcome == 2) {
       jump_descend(q, w);
       return false;
@@ -493,9 +493,12 @@ __device__ inline bool jump_step(const TuvPair& q, JumpWalk& w, double tol, uint
     if (outcome == 1 && w.key[0] < w.best) w.best = w.key[0];
     w.ready = false;
   }
+  // marks past level 63 are rare: the wave takes the one-word climb unless
+  // one of its lanes has such a mark
+  const bool wide = __ballot(w.low[1] != 0) != 0;
 #pragma unroll
   for (int m = 0; m < kJumps; ++m) {
-    if (!jump_next(q, w)) return true;
+    if (!(wide ? jump_next(q, w) : jump_next_lo(q, w))) return true;
     if (w.ready) return false;
   }
   return false;
@@ -585,10 +588,10 @@ __device__ inline void save_walk(WalkState* st, const LaneWalk& w) {
 
 __device__ inline void save_walk(WalkState* st, const JumpWalk& w) {
   WalkState s;
-  s.key[0] = w.key[0];
-  s.key[1] = w.key[1];
-  s.key[2] = w.key[2];
-  s.best = w.best;
+  s.key[0] = jump_key64(w.key[0]);
+  s.key[1] = jump_key64(w.key[1]);
+  s.key[2] = jump_key64(w.key[2]);
+  s.best = jump_best64(w.best);
   s.boxes = w.boxes;
   s.level = (uint16_t)w.level;
   s.e[0] = (uint8_t)exp_of(w.ep, 0);
@@ -744,7 +747,7 @@ __global__ __launch_bounds__(kTuvBlock, 3) void tuv_stream_kernel(Src src, int64
           pair = next + rank;
           q = src.load(pair);
           start_jump_walk(w);
-          if (q.done_level > kTuvMaskLevels) {  // past the jump walk's masks: to the wave pass
+          if (!jump_walk_fits(q)) {  // keys or split sequence past the jump walk's: to the wave pass
             save_walk(&states[pair], w);
             flag_for_replay((uint32_t)pair, heavy_list, heavy_count, list_cap);
             pair = -1;
@@ -771,7 +774,7 @@ __global__ __launch_bounds__(kTuvBlock, 3) void tuv_stream_kernel(Src src, int64
           save_walk(&states[pair], w);
           flag_for_replay((uint32_t)pair, heavy_list, heavy_count, list_cap);
         } else {
-          write_result(&out[pair], w.best, w.boxes, 0u);
+          write_result(&out[pair], jump_best64(w.best), w.boxes, 0u);
         }
         pair = -1;
       }

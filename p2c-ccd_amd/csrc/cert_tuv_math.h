@@ -481,19 +481,26 @@ P2C_TUV_HD bool tuv_walk(const TuvPair& q, uint64_t kt, uint64_t ku, uint64_t kv
   }
 }
 
-// The stream kernel's walk: the same depth-first pre-order as tuv_walk (the same boxes, evaluated in the same order, pruned by the same
-// rules), with the climb to the next box made in one jump instead of one
-// level per move.  low[] marks the levels 0..127 at which the path took the
-// lower child while the upper sibling is still to be visited; an upper
-// sibling outside the triangle (VF, u + v > 1) is never marked, since its
-// whole subtree is skipped.  The next box after a finished subtree is the
-// upper sibling at the deepest marked level; its exponents are population
-// counts of the pair's level masks, and its keys keep the bits of the
-// splits above it.  Pairs whose split sequence runs past the masks
-// (done_level > kTuvMaskLevels) are handed to the wave pass unwalked.
+// The stream kernel's walk: the same depth-first pre-order as tuv_walk (the
+// same boxes, evaluated in the same order, pruned by the same rules), with
+// the climb to the next box made in one jump instead of one level per move.
+// low[] marks the levels 0..127 at which the path took the lower child while
+// the upper sibling is still to be visited; an upper sibling outside the
+// triangle (VF, u + v > 1) is never marked, since its whole subtree is
+// skipped.  The next box after a finished subtree is the upper sibling at the
+// deepest marked level; its exponents are population counts of the pair's
+// level masks, and its keys keep the bits of the splits above it.
+//
+// Keys are 32-bit (lower corner * 2^31), so the walk takes the pairs whose
+// dimensions split at most 31 times each and whose split sequence fits the
+// masks (jump_walk_fits); a tolerance of 1e-6 needs extents near 1e3 to pass
+// that.  Other pairs are handed to the wave pass unwalked.
+constexpr uint32_t kOne31 = 1u << 31;
+constexpr uint32_t kJumpNoHit = ~0u;
+
 struct JumpWalk {
-  uint64_t key[3];
-  uint64_t best;
+  uint32_t key[3];
+  uint32_t best;  // least done t key, kJumpNoHit: none
   uint64_t low[2];
   uint32_t boxes;
   uint32_t ep;
@@ -501,15 +508,25 @@ struct JumpWalk {
   bool ready;
 };
 
+P2C_TUV_HD bool jump_walk_fits(const TuvPair& q) {
+  const uint32_t et = (uint32_t)(__builtin_popcountll(q.tmask[0]) + __builtin_popcountll(q.tmask[1]));
+  const uint32_t eu = (uint32_t)(__builtin_popcountll(q.umask[0]) + __builtin_popcountll(q.umask[1]));
+  return q.done_level <= kTuvMaskLevels && et <= 31u && eu <= 31u && q.done_level - et - eu <= 31u;
+}
+
 P2C_TUV_HD void start_jump_walk(JumpWalk& w) {
   w.key[0] = w.key[1] = w.key[2] = 0;
-  w.best = kTuvNoHit;
+  w.best = kJumpNoHit;
   w.low[0] = w.low[1] = 0;
   w.boxes = 0;
   w.ep = 0;
   w.level = 0;
   w.ready = true;
 }
+
+// The 64-bit forms of a jump walk's keys and best (tuv_walk's encoding).
+P2C_TUV_HD uint64_t jump_key64(uint32_t k) { return (uint64_t)k << 32; }
+P2C_TUV_HD uint64_t jump_best64(uint32_t b) { return b == kJumpNoHit ? kTuvNoHit : jump_key64(b); }
 
 // Moves to the lower child of a box that splits (the same lower corner, so it
 // passes every skip test its parent passed) and marks the level when the
@@ -518,13 +535,31 @@ P2C_TUV_HD void jump_descend(const TuvPair& q, JumpWalk& w) {
   const uint32_t l = w.level;  // < done_level <= kTuvMaskLevels
   const int d = tuv_mask_dim(q, l);
   w.ep += 1u << (8 * d);
-  const uint64_t half = kOne63 >> (int)((w.ep >> (8 * d)) & 0xffu);
-  const uint64_t ku = w.key[1] + (d == 1 ? half : 0ull), kv = w.key[2] + (d == 2 ? half : 0ull);
-  const bool visit = q.is_ee || ku + kv <= kOne63;
+  const uint32_t half = kOne31 >> ((w.ep >> (8 * d)) & 0xffu);
+  const uint32_t ku = w.key[1] + (d == 1 ? half : 0u), kv = w.key[2] + (d == 2 ? half : 0u);
+  const bool visit = q.is_ee || ku + kv <= kOne31;  // keys below 2^31: no overflow
   const uint64_t m = visit ? (1ull << (l & 63u)) : 0ull;
   w.low[0] |= l < 64 ? m : 0ull;
   w.low[1] |= l < 64 ? 0ull : m;
   w.level = l + 1;
+}
+
+// The upper child at marked level l (bit `bit` of the mask word holding it,
+// m0 / m1 the levels 0..l): exponents by population count, keys cut to the
+// splits on levels 0..l (bit positions >= 31 - e), then the split at l taken
+// upwards.
+P2C_TUV_HD void jump_land(const TuvPair& q, JumpWalk& w, uint32_t l, uint64_t m0, uint64_t m1, bool st, bool su) {
+  const uint32_t et = (uint32_t)(__builtin_popcountll(q.tmask[0] & m0) + __builtin_popcountll(q.tmask[1] & m1));
+  const uint32_t eu = (uint32_t)(__builtin_popcountll(q.umask[0] & m0) + __builtin_popcountll(q.umask[1] & m1));
+  const uint32_t ev = l + 1u - et - eu;
+  const bool sv = !st && !su;
+  const uint32_t s0 = 1u << (31u - et), s1 = 1u << (31u - eu), s2 = 1u << (31u - ev);
+  w.key[0] = (w.key[0] | (st ? s0 : 0u)) & (0u - s0);
+  w.key[1] = (w.key[1] | (su ? s1 : 0u)) & (0u - s1);
+  w.key[2] = (w.key[2] | (sv ? s2 : 0u)) & (0u - s2);
+  w.ep = et | (eu << 8) | (ev << 16);
+  w.level = l + 1u;
+  w.ready = w.key[0] < w.best;
 }
 
 // Moves to the next box after the current box's subtree; false when the walk
@@ -538,32 +573,32 @@ P2C_TUV_HD bool jump_next(const TuvPair& q, JumpWalk& w) {
   const uint64_t bit = 1ull << b;
   w.low[1] = hi ? (x ^ bit) : w.low[1];
   w.low[0] = hi ? w.low[0] : (x ^ bit);
-  // splits on levels 0..l, l = the marked level
   const uint64_t upto = bit | (bit - 1ull);
-  const uint64_t m0 = hi ? ~0ull : upto, m1 = hi ? upto : 0ull;
-  const uint32_t l = b + (hi ? 64u : 0u);
-  const uint32_t et = (uint32_t)(__builtin_popcountll(q.tmask[0] & m0) + __builtin_popcountll(q.tmask[1] & m1));
-  const uint32_t eu = (uint32_t)(__builtin_popcountll(q.umask[0] & m0) + __builtin_popcountll(q.umask[1] & m1));
-  const uint32_t ev = l + 1u - et - eu;
   const bool st = ((hi ? q.tmask[1] : q.tmask[0]) & bit) != 0;
   const bool su = !st && ((hi ? q.umask[1] : q.umask[0]) & bit) != 0;
-  const bool sv = !st && !su;
-  // keep each key's bits of the splits on levels 0..l (positions >= 63 - e),
-  // then take the upper child at level l
-  const uint64_t s0 = 1ull << (63u - et), s1 = 1ull << (63u - eu), s2 = 1ull << (63u - ev);
-  w.key[0] = (w.key[0] | (st ? s0 : 0ull)) & (0ull - s0);
-  w.key[1] = (w.key[1] | (su ? s1 : 0ull)) & (0ull - s1);
-  w.key[2] = (w.key[2] | (sv ? s2 : 0ull)) & (0ull - s2);
-  w.ep = et | (eu << 8) | (ev << 16);
-  w.level = l + 1u;
-  w.ready = w.key[0] < w.best;
+  jump_land(q, w, b + (hi ? 64u : 0u), hi ? ~0ull : upto, hi ? upto : 0ull, st, su);
   return true;
 }
 
-// Evaluates the box a jump walk stands on: 0 reject, 1 done, 2 split.
+// jump_next for a walk with no marks past level 63 (low[1] == 0).
+P2C_TUV_HD bool jump_next_lo(const TuvPair& q, JumpWalk& w) {
+  const uint64_t x = w.low[0];
+  if (x == 0) return false;
+  const uint32_t b = 63u - (uint32_t)__builtin_clzll(x);
+  const uint64_t bit = 1ull << b;
+  w.low[0] = x ^ bit;
+  const bool st = (q.tmask[0] & bit) != 0;
+  const bool su = !st && (q.umask[0] & bit) != 0;
+  jump_land(q, w, b, bit | (bit - 1ull), 0ull, st, su);
+  return true;
+}
+
+// Evaluates the box a jump walk stands on: 0 reject, 1 done, 2 split.  The
+// keys' values are exact in FP64 (31 significant bits).
 P2C_TUV_HD int jump_classify(const TuvPair& q, const JumpWalk& w, double tol) {
   const int e[3] = {(int)(w.ep & 0xffu), (int)((w.ep >> 8) & 0xffu), (int)((w.ep >> 16) & 0xffu)};
-  return tuv_classify_at(q, key_to_f64(w.key[0]), key_to_f64(w.key[1]), key_to_f64(w.key[2]), e, w.level, tol);
+  return tuv_classify_at(q, (double)w.key[0] * 0x1p-31, (double)w.key[1] * 0x1p-31, (double)w.key[2] * 0x1p-31, e,
+                         w.level, tol);
 }
 
 }  // namespace p2c

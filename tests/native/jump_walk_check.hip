@@ -3,9 +3,9 @@
 // tuv_walk, the plain depth-first walk it restates: the same number of boxes
 // evaluated and the same least done t_lo, on random vertex-face and
 // edge-edge pairs -- uniform, near contact (hits and deep walks), coplanar,
-// and scaled so that walks run past 64 levels.  Pairs whose split sequence
-// runs past the masks (done_level > 128) are left to the wave pass by the
-// kernel and are skipped here.
+// and scaled so that walks run past 64 levels.  Pairs the jump walk does not
+// take (jump_walk_fits) are left to the wave pass by the kernel and are
+// skipped here; scale 1e4 makes some.
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -19,7 +19,7 @@ static bool jump_walk(const TuvPair& q, double tol, uint64_t* best, uint64_t* bo
   for (;;) {
     if (w.ready) {
       if (w.boxes >= budget) {
-        *best = w.best;
+        *best = jump_best64(w.best);
         *boxes = w.boxes;
         return false;
       }
@@ -32,9 +32,9 @@ static bool jump_walk(const TuvPair& q, double tol, uint64_t* best, uint64_t* bo
       if (outcome == 1 && w.key[0] < w.best) w.best = w.key[0];
       w.ready = false;
     }
-    if (!jump_next(q, w)) break;
+    if (!(w.low[1] != 0 ? jump_next(q, w) : jump_next_lo(q, w))) break;
   }
-  *best = w.best;
+  *best = jump_best64(w.best);
   *boxes = w.boxes;
   return true;
 }
@@ -88,13 +88,13 @@ int main() {
       // coplanar: every vertex in z = 0 at both ends
       for (int r = 0; r < 8; ++r) v[r][2] = 0.0;
     } else if (mode == 4) {
-      const double scale = (i % 3 == 0) ? 40.0 : 4.0;  // deeper split sequences
+      const double scale = (i / 10) % 7 == 0 ? 1e4 : ((i % 3 == 0) ? 40.0 : 4.0);  // deeper split sequences
       for (int r = 0; r < 8; ++r)
         for (int k = 0; k < 3; ++k) v[r][k] *= scale;
     }
     TuvPair q;
     tuv_setup(&q, v, is_ee, 0.0, tol);
-    if (q.done_level > kTuvMaskLevels) {
+    if (!jump_walk_fits(q)) {
       ++skipped;
       continue;
     }
@@ -114,8 +114,8 @@ int main() {
       ++bad;
     }
   }
-  printf("jump_walk_check: %ld pairs (%ld hits, %ld past level 64, %ld over budget, %llu boxes; %ld past the masks "
+  printf("jump_walk_check: %ld pairs (%ld hits, %ld past level 64, %ld over budget, %llu boxes; %ld not taken by the jump walk, "
          "skipped), %ld mismatches\n",
          n, hits, deep, over, (unsigned long long)total_boxes, skipped, bad);
-  return bad != 0 || hits == 0 || deep == 0;
+  return bad != 0 || hits == 0 || deep == 0 || skipped == 0;
 }
