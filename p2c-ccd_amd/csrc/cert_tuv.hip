@@ -479,6 +479,9 @@ constexpr int kJumps = P2C_JUMPS;
 // walk has finished; sets *over (leaving the box unevaluated) when the box
 // budget ran out.
 __device__ inline bool jump_step(const TuvPair& q, JumpWalk& w, double tol, uint64_t budget, bool* over) {
+  // levels past 63 are rare: the wave takes the one-word descend and climb
+  // unless one of its lanes stands at level 64 or deeper or holds a mark there
+  const bool wide = __ballot(w.level >= 64 || w.low[1] != 0) != 0;
   if (w.ready) {
     if (w.boxes >= budget) {
       *over = true;
@@ -487,15 +490,13 @@ __device__ inline bool jump_step(const TuvPair& q, JumpWalk& w, double tol, uint
     ++w.boxes;
     const int outcome = jump_classify(q, w, tol);
     if (outcome == 2) {
-      jump_descend(q, w);
+      if (wide) jump_descend(q, w);
+      else jump_descend_lo(q, w);
       return false;
     }
     if (outcome == 1 && w.key[0] < w.best) w.best = w.key[0];
     w.ready = false;
   }
-  // marks past level 63 are rare: the wave takes the one-word climb unless
-  // one of its lanes has such a mark
-  const bool wide = __ballot(w.low[1] != 0) != 0;
 #pragma unroll
   for (int m = 0; m < kJumps; ++m) {
     if (!(wide ? jump_next(q, w) : jump_next_lo(q, w))) return true;
@@ -782,6 +783,13 @@ __global__ __launch_bounds__(kTuvBlock, 3) void tuv_stream_kernel(Src src, int64
   }
 }
 
+// Orders a wave's LDS accesses before and after this point.
+__device__ inline void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
 // ---- pass 2: one wave per heavy pair ------------------------------------
 // The 64 lanes of a wave share one pair's box tree.  Each lane walks a
 // subtree depth-first (one evaluation and at most kAdvanceMoves moves per
@@ -814,12 +822,6 @@ __device__ inline uint32_t wave_prefix_u8(uint32_t v, uint32_t* total) {
   }
   *total = tot;
   return pre;
-}
-
-__device__ inline void wave_sync_lds() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
 // Pending upper siblings on the path between level `root` (exponents
@@ -1072,6 +1074,203 @@ __device__ int wave_certify(const TuvPair& q, uint32_t p, WaveTask* wq, unsigned
   }
 }
 
+// ---- wave pass on the jump walk ------------------------------------------
+// For pairs the jump walk takes (jump_walk_fits): each lane runs a jump walk
+// over a subtree; a lane that finishes its subtree takes one from the wave's
+// LDS stack, and while idle lanes outnumber the stacked subtrees every lane
+// holding marks gives away its shallowest one (the largest pending subtree).
+// The least done t_lo is shared across the wave (and, for split pieces,
+// through the pair's accumulator) whenever a lane finds one.  The result is
+// the least t_lo over done boxes, whatever the order.
+constexpr int kJumpStack = 512;  // stacked subtrees per wave
+
+struct JumpBox {
+  uint32_t key[3];
+  uint32_t lep;  // level | exponents << 8
+};
+
+// A wave's LDS in the wave pass: the old walk's task queue or the jump
+// walk's subtree stack (a pair takes one of the two).
+union WaveLds {
+  WaveTask queue[kWaveQueue];
+  JumpBox stack[kJumpStack];
+};
+
+__device__ inline uint32_t wave_min_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    const uint32_t x = (uint32_t)__shfl_xor((int)v, o);
+    v = x < v ? x : v;
+  }
+  return v;
+}
+
+__device__ inline uint32_t best32_of(uint64_t b) { return b == kTuvNoHit ? kJumpNoHit : (uint32_t)(b >> 32); }
+
+// A saved walk (64-bit keys, tuv_walk's position) of a pair the jump walk
+// takes, with its marks rebuilt from the keys: lane j checks level j (and
+// j + 64).  Every lane returns the same walk.
+__device__ inline void jump_resume(const TuvPair& q, const WalkState& st, JumpWalk& w) {
+  w.key[0] = (uint32_t)(st.key[0] >> 32);
+  w.key[1] = (uint32_t)(st.key[1] >> 32);
+  w.key[2] = (uint32_t)(st.key[2] >> 32);
+  w.best = best32_of(st.best);
+  w.boxes = 0;
+  w.ep = (uint32_t)st.e[0] | ((uint32_t)st.e[1] << 8) | ((uint32_t)st.e[2] << 16);
+  w.level = st.level;
+  w.ready = st.ready != 0;
+  const uint32_t lane = threadIdx.x & 63u;
+  w.low[0] = __ballot(lane < w.level && jump_pending_at(q, w, lane));
+  w.low[1] = __ballot(lane + 64u < w.level && jump_pending_at(q, w, lane + 64u));
+}
+
+__device__ inline SplitTask jump_task(uint32_t pair, const uint32_t key[3], uint32_t level, uint32_t ep) {
+  const uint64_t k[3] = {jump_key64(key[0]), jump_key64(key[1]), jump_key64(key[2])};
+  return make_task(pair, k, level, ep, true, level, ep);
+}
+
+// The wave pass over one pair (heavy mode: `active` on lane 0 only, its walk
+// resumed or at the root) or one split piece (task mode).  Returns kWaveDone,
+// kWaveOver or kWaveSplit; w.best is the wave's least done key on return.
+template <bool kTask, bool kSplit>
+__device__ int wave_certify_jump(const TuvPair& q, uint32_t p, JumpBox* stk, JumpWalk& w, bool active,
+                                 uint64_t& boxes, double tol, uint64_t max_itr, const SplitOut& so) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t lt = (1ull << lane) - 1ull;
+  int sp = 0;  // stacked subtrees (wave-uniform)
+  uint32_t step = 0;
+  w.best = wave_min_u32(w.best);
+  for (;;) {
+    // 1. evaluate, then descend or jump to the next box of the lane's subtree
+    const bool wide = __ballot(w.level >= 64 || w.low[1] != 0) != 0;
+    bool evaluated = false, found = false;
+    if (active) {
+      bool finished = !w.ready;
+      if (w.ready) {
+        if (w.key[0] < w.best) {
+          evaluated = true;
+          const int outcome = jump_classify(q, w, tol);
+          if (outcome == 2) {
+            if (wide) jump_descend(q, w);
+            else jump_descend_lo(q, w);
+          } else {
+            finished = true;
+            if (outcome == 1) {
+              w.best = w.key[0];
+              found = true;
+            }
+          }
+        } else {
+          finished = true;
+        }
+      }
+      if (finished && !(wide ? jump_next(q, w) : jump_next_lo(q, w))) active = false;
+    }
+    boxes += (uint64_t)__popcll(__ballot(evaluated));
+    // 2. share the least done t_lo
+    const bool poll = kTask && (++step & 15u) == 0;
+    if (__ballot(found) != 0 || poll) {
+      uint32_t b = wave_min_u32(w.best);
+      if (kTask) {
+        unsigned long long g = 0;
+        if (lane == 0) g = atomicMin(&so.acc[p].best, (unsigned long long)jump_best64(b));
+        g = __shfl(g, 0);
+        const uint32_t gb = best32_of(g);
+        b = gb < b ? gb : b;
+      }
+      w.best = b;
+    }
+    if (boxes > max_itr) return kWaveOver;
+    if (kSplit && boxes > so.split_at) {
+      // every pending piece becomes a task: each lane's current box and
+      // marked siblings, and the stacked subtrees
+      const uint32_t mine = active ? (uint32_t)(__popcll(w.low[0]) + __popcll(w.low[1])) + (w.ready ? 1u : 0u) : 0u;
+      uint32_t lanes_tot = 0;
+      const uint32_t off = wave_prefix_u8(mine, &lanes_tot);  // at most 129 per lane
+      const uint32_t need = lanes_tot + (uint32_t)sp;
+      uint32_t base = 0xFFFFFFFFu;
+      if (lane == 0) {
+        uint32_t cur = __hip_atomic_load(so.task_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int tries = 0; tries < 64; ++tries) {
+          if (cur + need > so.task_cap || cur + need < cur) break;
+          if (__hip_atomic_compare_exchange_strong(so.task_count, &cur, cur + need, __ATOMIC_RELAXED,
+                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            base = cur;
+            break;
+          }
+        }
+      }
+      base = __shfl(base, 0);
+      if (base != 0xFFFFFFFFu) {
+        SplitTask* out = so.tasks + base;
+        if (active) {
+          uint32_t slot = off;
+          if (w.ready) out[slot++] = jump_task(p, w.key, w.level, w.ep);
+          while ((w.low[0] | w.low[1]) != 0) {
+            uint32_t k[3], ep, lvl;
+            jump_take_shallowest(q, w, k, &ep, &lvl);
+            out[slot++] = jump_task(p, k, lvl, ep);
+          }
+        }
+        for (int i = lane; i < sp; i += 64) {
+          const JumpBox b = stk[i];
+          out[lanes_tot + (uint32_t)i] = jump_task(p, b.key, b.lep & 0xffu, b.lep >> 8);
+        }
+        if (lane == 0) {
+          so.acc[p].best = jump_best64(w.best);
+          so.acc[p].boxes = (uint32_t)boxes;
+          so.acc[p].over = 0;
+          const uint32_t at = atomicAdd(so.split_count, 1u);
+          so.split_list[at] = p;  // at most one entry per listed pair
+        }
+        return kWaveSplit;
+      }
+    }
+    // 3. idle lanes beyond the stacked subtrees: lanes with marks give their shallowest
+    const uint64_t idle_m = __ballot(!active);
+    const int idle = __popcll(idle_m);
+    if (idle > sp && sp < kJumpStack) {
+      const bool can = active && (w.low[0] | w.low[1]) != 0;
+      const uint64_t cm = __ballot(can);
+      const int want = min(idle - sp, kJumpStack - sp);
+      const int give = min(__popcll(cm), want);
+      if (can) {
+        const int r = __popcll(cm & lt);
+        if (r < give) {
+          JumpBox b;
+          uint32_t ep, lvl;
+          jump_take_shallowest(q, w, b.key, &ep, &lvl);
+          b.lep = lvl | (ep << 8);
+          stk[sp + r] = b;
+        }
+      }
+      sp += give;
+      wave_sync_lds();
+    }
+    // 4. idle lanes take stacked subtrees, the top first
+    if (idle > 0 && sp > 0) {
+      const int take = min(idle, sp);
+      if (!active) {
+        const int r = __popcll(idle_m & lt);
+        if (r < take) {
+          const JumpBox b = stk[sp - 1 - r];
+          w.key[0] = b.key[0];
+          w.key[1] = b.key[1];
+          w.key[2] = b.key[2];
+          w.level = b.lep & 0xffu;
+          w.ep = b.lep >> 8;
+          w.low[0] = w.low[1] = 0;
+          w.ready = w.key[0] < w.best;
+          active = true;
+        }
+      }
+      sp -= take;
+      wave_sync_lds();
+    }
+    if (__ballot(active) == 0 && sp == 0) return kWaveDone;
+  }
+}
+
 // Three workgroups per CU (12 waves: 3 per SIMD; 147 KB of LDS): at most 168
 // VGPRs, a few spilled; the wave pass ran 20 % longer at 2 waves per SIMD.
 template <bool kSplit, typename Src>
@@ -1083,7 +1282,7 @@ __global__ __launch_bounds__(kTuvBlock, 3) void tuv_heavy_kernel(Src src,
                                                               uint64_t max_itr, p2c_tuv_result* __restrict__ out,
                                                               uint32_t* __restrict__ replay_list,
                                                               uint32_t* __restrict__ replay_count, SplitOut so) {
-  __shared__ WaveTask queue[kTuvBlock / 64][kWaveQueue];
+  __shared__ WaveLds lds[kTuvBlock / 64];
   __shared__ unsigned long long best_slot[kTuvBlock / 64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   uint32_t total = *heavy_count;
@@ -1095,6 +1294,43 @@ __global__ __launch_bounds__(kTuvBlock, 3) void tuv_heavy_kernel(Src src,
     if (h >= total) break;
     const uint32_t p = heavy_list[h];
     const TuvPair q = src.load(p);
+    if (jump_walk_fits(q)) {
+      JumpWalk w;
+      start_jump_walk(w);
+      uint64_t boxes = 0;
+      bool active = lane == 0;
+      if (resume != nullptr) {
+        const WalkState st = resume[p];
+        boxes = st.boxes;
+        jump_resume(q, st, w);
+        // every lane at once: lane j + 1 takes the upper sibling at marked
+        // level j < 63, lane 0 keeps the current box and the deeper marks
+        if (lane == 0) {
+          w.low[0] &= 1ull << 63;
+        } else {
+          const uint32_t j = (uint32_t)lane - 1u;
+          uint32_t k[3], ep;
+          active = ((w.low[0] >> j) & 1ull) != 0 && jump_sibling_at(q, w, j, k, &ep);
+          if (active) {
+            w.key[0] = k[0];
+            w.key[1] = k[1];
+            w.key[2] = k[2];
+            w.ep = ep;
+            w.level = j + 1u;
+            w.ready = w.key[0] < w.best;
+          }
+          w.low[0] = w.low[1] = 0;
+        }
+      }
+      const int r = wave_certify_jump<false, kSplit>(q, p, lds[wv].stack, w, active, boxes, tol, max_itr, so);
+      if (r != kWaveSplit && lane == 0) {
+        write_result(&out[p], jump_best64(w.best), boxes, r == kWaveOver ? P2C_TUV_FLAG_OVER_BUDGET : 0u);
+        if (r == kWaveOver) flag_for_replay(p, replay_list, replay_count, list_cap);
+      }
+      __builtin_amdgcn_wave_barrier();
+      continue;
+    }
+    WaveTask* queue_wv = lds[wv].queue;
     LaneWalk w;
     uint64_t boxes = 0;
     start_walk(w);
@@ -1107,7 +1343,7 @@ __global__ __launch_bounds__(kTuvBlock, 3) void tuv_heavy_kernel(Src src,
     if (lane == 0) best_slot[wv] = w.best;
     wave_sync_lds();
     const int r =
-        wave_certify<false, kSplit>(q, p, queue[wv], &best_slot[wv], w, lane == 0, 0, 0, boxes, tol, max_itr, so);
+        wave_certify<false, kSplit>(q, p, queue_wv, &best_slot[wv], w, lane == 0, 0, 0, boxes, tol, max_itr, so);
     if (r != kWaveSplit) {
       const uint64_t final_best = best_slot[wv] < w.best ? best_slot[wv] : w.best;
       if (lane == 0) {
@@ -1125,7 +1361,7 @@ template <typename Src>
 __global__ __launch_bounds__(kTuvBlock, 3) void tuv_task_kernel(Src src,
                                                              uint32_t* __restrict__ task_next, double tol,
                                                              uint64_t max_itr, SplitOut so) {
-  __shared__ WaveTask queue[kTuvBlock / 64][kWaveQueue];
+  __shared__ WaveLds lds[kTuvBlock / 64];
   __shared__ unsigned long long best_slot[kTuvBlock / 64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   uint32_t total = *so.task_count;
@@ -1138,6 +1374,28 @@ __global__ __launch_bounds__(kTuvBlock, 3) void tuv_task_kernel(Src src,
     const SplitTask t = so.tasks[h];
     const uint32_t p = t.pair;
     const TuvPair q = src.load(p);
+    if (jump_walk_fits(q)) {
+      // pieces of such pairs are whole subtrees (a box, ready, rooted at itself)
+      JumpWalk w;
+      start_jump_walk(w);
+      w.key[0] = (uint32_t)(t.key[0] >> 32);
+      w.key[1] = (uint32_t)(t.key[1] >> 32);
+      w.key[2] = (uint32_t)(t.key[2] >> 32);
+      w.level = t.level;
+      w.ep = (uint32_t)t.e[0] | ((uint32_t)t.e[1] << 8) | ((uint32_t)t.e[2] << 16);
+      w.best = best32_of(__hip_atomic_load(&so.acc[p].best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+      w.ready = w.key[0] < w.best;
+      uint64_t boxes = 0;
+      const int r = wave_certify_jump<true, false>(q, p, lds[wv].stack, w, lane == 0, boxes, tol, max_itr, so);
+      if (lane == 0) {
+        atomicMin(&so.acc[p].best, (unsigned long long)jump_best64(w.best));
+        atomicAdd(&so.acc[p].boxes, (uint32_t)(boxes < 0xFFFFFFFFull ? boxes : 0xFFFFFFFFull));
+        if (r == kWaveOver) atomicOr(&so.acc[p].over, 1u);
+      }
+      __builtin_amdgcn_wave_barrier();
+      continue;
+    }
+    WaveTask* queue_wv = lds[wv].queue;
     LaneWalk w;
     start_walk(w);
     w.best = __hip_atomic_load(&so.acc[p].best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1152,7 +1410,7 @@ __global__ __launch_bounds__(kTuvBlock, 3) void tuv_task_kernel(Src src,
     if (lane == 0) best_slot[wv] = w.best;
     wave_sync_lds();
     uint64_t boxes = 0;
-    const int r = wave_certify<true, false>(q, p, queue[wv], &best_slot[wv], w, has, t.root, root_ep, boxes, tol,
+    const int r = wave_certify<true, false>(q, p, queue_wv, &best_slot[wv], w, has, t.root, root_ep, boxes, tol,
                                             max_itr, so);
     if (lane == 0) {
       const uint64_t b = best_slot[wv] < w.best ? best_slot[wv] : w.best;

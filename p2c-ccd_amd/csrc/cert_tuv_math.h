@@ -544,22 +544,89 @@ P2C_TUV_HD void jump_descend(const TuvPair& q, JumpWalk& w) {
   w.level = l + 1;
 }
 
-// The upper child at marked level l (bit `bit` of the mask word holding it,
-// m0 / m1 the levels 0..l): exponents by population count, keys cut to the
-// splits on levels 0..l (bit positions >= 31 - e), then the split at l taken
-// upwards.
-P2C_TUV_HD void jump_land(const TuvPair& q, JumpWalk& w, uint32_t l, uint64_t m0, uint64_t m1, bool st, bool su) {
+// jump_descend for a box below level 64 (its mark lands in low[0]).
+P2C_TUV_HD void jump_descend_lo(const TuvPair& q, JumpWalk& w) {
+  const uint32_t l = w.level;
+  const uint32_t t = (uint32_t)(q.tmask[0] >> l) & 1u, u = (uint32_t)(q.umask[0] >> l) & 1u;
+  const uint32_t d = t ? 0u : (u ? 1u : 2u);
+  w.ep += 1u << (8 * d);
+  const uint32_t half = kOne31 >> ((w.ep >> (8 * d)) & 0xffu);
+  const uint32_t ku = w.key[1] + (d == 1 ? half : 0u), kv = w.key[2] + (d == 2 ? half : 0u);
+  const bool visit = q.is_ee || ku + kv <= kOne31;
+  w.low[0] |= visit ? (1ull << l) : 0ull;
+  w.level = l + 1;
+}
+
+// The upper child at marked level l of the path through key[] (m0 / m1: the
+// levels 0..l; st / su: level l splits t / u): its exponents by population
+// count, its keys cut to the splits on levels 0..l (bit positions >= 31 - e)
+// with the split at l taken upwards.
+P2C_TUV_HD void jump_sibling(const TuvPair& q, const uint32_t key[3], uint32_t l, uint64_t m0, uint64_t m1, bool st,
+                             bool su, uint32_t out[3], uint32_t* ep) {
   const uint32_t et = (uint32_t)(__builtin_popcountll(q.tmask[0] & m0) + __builtin_popcountll(q.tmask[1] & m1));
   const uint32_t eu = (uint32_t)(__builtin_popcountll(q.umask[0] & m0) + __builtin_popcountll(q.umask[1] & m1));
   const uint32_t ev = l + 1u - et - eu;
   const bool sv = !st && !su;
   const uint32_t s0 = 1u << (31u - et), s1 = 1u << (31u - eu), s2 = 1u << (31u - ev);
-  w.key[0] = (w.key[0] | (st ? s0 : 0u)) & (0u - s0);
-  w.key[1] = (w.key[1] | (su ? s1 : 0u)) & (0u - s1);
-  w.key[2] = (w.key[2] | (sv ? s2 : 0u)) & (0u - s2);
-  w.ep = et | (eu << 8) | (ev << 16);
+  out[0] = (key[0] | (st ? s0 : 0u)) & (0u - s0);
+  out[1] = (key[1] | (su ? s1 : 0u)) & (0u - s1);
+  out[2] = (key[2] | (sv ? s2 : 0u)) & (0u - s2);
+  *ep = et | (eu << 8) | (ev << 16);
+}
+
+// Moves the walk to that upper child (ready unless pruned by best).
+P2C_TUV_HD void jump_land(const TuvPair& q, JumpWalk& w, uint32_t l, uint64_t m0, uint64_t m1, bool st, bool su) {
+  uint32_t k[3], ep;
+  jump_sibling(q, w.key, l, m0, m1, st, su, k, &ep);
+  w.key[0] = k[0];
+  w.key[1] = k[1];
+  w.key[2] = k[2];
+  w.ep = ep;
   w.level = l + 1u;
   w.ready = w.key[0] < w.best;
+}
+
+// The upper child at the shallowest marked level, whose mark is cleared: the
+// largest pending subtree of the walk, for another lane to take.  The walk
+// must hold a mark.
+P2C_TUV_HD void jump_take_shallowest(const TuvPair& q, JumpWalk& w, uint32_t out[3], uint32_t* ep, uint32_t* level) {
+  const bool lo = w.low[0] != 0;
+  const uint64_t x = lo ? w.low[0] : w.low[1];
+  const uint32_t b = (uint32_t)__builtin_ctzll(x);
+  const uint64_t bit = 1ull << b;
+  w.low[0] = lo ? (x ^ bit) : w.low[0];
+  w.low[1] = lo ? w.low[1] : (x ^ bit);
+  const uint64_t upto = bit | (bit - 1ull);
+  const bool st = ((lo ? q.tmask[0] : q.tmask[1]) & bit) != 0;
+  const bool su = !st && ((lo ? q.umask[0] : q.umask[1]) & bit) != 0;
+  const uint32_t l = b + (lo ? 0u : 64u);
+  jump_sibling(q, w.key, l, lo ? upto : ~0ull, lo ? 0ull : upto, st, su, out, ep);
+  *level = l + 1u;
+}
+
+// Whether level j < w.level of the walk's path took the lower child with an
+// upper sibling to visit (not outside the triangle): the mark jump_descend
+// would have left there.  The sibling (keys k, exponents *ep, level j + 1)
+// is returned either way.  Rebuilds the marks of a walk resumed from its
+// keys.
+P2C_TUV_HD bool jump_sibling_at(const TuvPair& q, const JumpWalk& w, uint32_t j, uint32_t k[3], uint32_t* ep) {
+  const bool hi = j >= 64;
+  const uint64_t bit = 1ull << (j & 63u);
+  const uint64_t upto = bit | (bit - 1ull);
+  const bool st = ((hi ? q.tmask[1] : q.tmask[0]) & bit) != 0;
+  const bool su = !st && ((hi ? q.umask[1] : q.umask[0]) & bit) != 0;
+  jump_sibling(q, w.key, j, hi ? ~0ull : upto, hi ? upto : 0ull, st, su, k, ep);
+  // the path's own key holds the bit of the split at j when it took the upper child
+  const uint32_t d = st ? 0u : (su ? 1u : 2u);
+  const uint32_t ed = (*ep >> (8 * d)) & 0xffu;
+  const uint32_t own = d == 0 ? w.key[0] : (d == 1 ? w.key[1] : w.key[2]);
+  const bool lower = (own & (1u << (31u - ed))) == 0;
+  return lower && (q.is_ee || k[1] + k[2] <= kOne31);
+}
+
+P2C_TUV_HD bool jump_pending_at(const TuvPair& q, const JumpWalk& w, uint32_t j) {
+  uint32_t k[3], ep;
+  return jump_sibling_at(q, w, j, k, &ep);
 }
 
 // Moves to the next box after the current box's subtree; false when the walk
