@@ -469,15 +469,10 @@ __device__ inline bool lane_step(const TuvPair& q, LaneWalk& w, double tol, uint
   return false;
 }
 
-#ifndef P2C_JUMPS
-#define P2C_JUMPS 1
-#endif
-constexpr int kJumps = P2C_JUMPS;
-
 // One step of the jump walk: evaluates the box the lane stands on (when
-// ready), then descends or makes at most kJumps jumps.  Returns true when the
-// walk has finished; sets *over (leaving the box unevaluated) when the box
-// budget ran out.
+// ready), then descends or makes one jump.  Returns true when the walk has
+// finished; sets *over (leaving the box unevaluated) when the box budget ran
+// out.
 __device__ inline bool jump_step(const TuvPair& q, JumpWalk& w, double tol, uint64_t budget, bool* over) {
   // levels past 63 are rare: the wave takes the one-word descend and climb
   // unless one of its lanes stands at level 64 or deeper or holds a mark there
@@ -497,12 +492,7 @@ __device__ inline bool jump_step(const TuvPair& q, JumpWalk& w, double tol, uint
     if (outcome == 1 && w.key[0] < w.best) w.best = w.key[0];
     w.ready = false;
   }
-#pragma unroll
-  for (int m = 0; m < kJumps; ++m) {
-    if (!(wide ? jump_next(q, w) : jump_next_lo(q, w))) return true;
-    if (w.ready) return false;
-  }
-  return false;
+  return !(wide ? jump_next(q, w) : jump_next_lo(q, w));
 }
 
 // Pass 1 of the batched form.  A pair whose walk needs more than
@@ -684,7 +674,9 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_round_kernel(const TuvPair* __r
 // pair that needs more than `light_budget` boxes saves its walk and goes to
 // the heavy list.  Once no pairs are left to claim, a wave whose busy lanes
 // number `handover_at` or fewer hands them all to the heavy list too, so a
-// few long walks do not hold a nearly empty wave.
+// few long walks do not hold a nearly empty wave.  Each lane runs the jump
+// walk (one box evaluated per step, then the lower child or one jump to the
+// next box); pairs the jump walk does not take go to the heavy list unwalked.
 constexpr int kStreamChunk = 256;
 
 template <typename Src>

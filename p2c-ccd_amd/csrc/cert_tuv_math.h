@@ -11,8 +11,10 @@
 // Box encoding: lower corners as dyadic keys scaled by 2^63 (kt, ku, kv) and
 // a level.  Every box at a level has the same widths, because the split
 // dimension is a function of the widths alone; the per-pair split sequence is
-// precomputed (2 bits per level), so a box is (keys, level) and the path to
-// it can be read back from the key bits.
+// precomputed (one-hot level masks), so a box is (keys, level) and the path
+// to it can be read back from the key bits.  The stream kernel's jump walk
+// keeps 32-bit keys (scaled by 2^31) for the pairs whose dimensions split at
+// most 31 times each.
 #pragma once
 
 #include <hip/hip_runtime.h>
