@@ -367,16 +367,24 @@ int p2c_tuv_stream(const void* pairs, int64_t n, const uint64_t* totals, double 
 int p2c_tuv_wave(const void* pairs, void* counters, const void* states, const uint32_t* heavy_list,
                  uint32_t list_cap, double tolerance, int64_t max_itr, p2c_tuv_result* out, uint32_t* replay_list,
                  int grid_blocks, void* stream);
-/* The wave pass with splitting: a pair that has taken more than split_at
- * boxes (0: never) is cut into pieces (each lane's pending path siblings, its
- * current subtree or unfinished walk, the wave's queued subtrees), written to
- * `tasks` (task_cap * p2c_tuv_task_bytes() device bytes) and certified one
- * wave per piece with the pair's best shared through its `states` slot; the
- * split pairs (listed in split_list, list_cap entries) get their results
- * once all pieces are done.  A pair whose pieces do not fit in `tasks` is
- * finished by its own wave.  `counters` as for p2c_tuv_stream, with the task
- * count, task cursor and split count at bytes 52, 56 and 60.  No counterpart
- * in the reference (its certificate runs one query per CPU thread). */
+/* The wave pass with splitting (split_at > 0; 0: never).  Waves done with
+ * the listed pairs wait for pieces of the others: while waves wait, a wave
+ * that has taken split_at boxes on its pair or piece hands every other busy
+ * lane's walk (its current box and pending siblings) to the waiting waves,
+ * through a queue in `tasks` (task_cap * p2c_tuv_task_bytes() device bytes,
+ * task_cap >= 20), in the same kernel; pieces may be cut again.  The pair's
+ * best is shared through its `states` slot; split pairs (listed in
+ * split_list, list_cap entries, list_cap < 2^31) get their results once all
+ * pieces are done.  Pairs whose walks need 64-bit keys (a dimension split
+ * more than 31 times) are cut once, after split_at boxes, into pieces run by
+ * a closing pass.  The last 12 slots of `tasks` hold the queue's counters and
+ * a call number kept from call to call (the buffer is reused; its first
+ * contents do not matter); a quarter of the rest holds the 64-bit-key walk's
+ * pieces.  When the queue is full, waves keep their work.  `counters` as for
+ * p2c_tuv_stream, with the 64-bit-key piece count and cursor at bytes 32 and
+ * 44, the piece count at byte 52 (written at the end) and the split count at
+ * byte 60.  No counterpart in the reference (its certificate runs one query
+ * per CPU thread). */
 size_t p2c_tuv_task_bytes(void);
 int p2c_tuv_wave_split(const void* pairs, void* counters, void* states, const uint32_t* heavy_list,
                        uint32_t list_cap, double tolerance, int64_t max_itr, p2c_tuv_result* out,

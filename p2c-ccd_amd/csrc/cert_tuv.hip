@@ -833,21 +833,55 @@ __device__ inline uint32_t pending_siblings(const LaneWalk& w, uint32_t root, ui
 }
 
 // A piece of a split pair's box tree: a walk (key, level, exponents, ready)
-// confined to the subtree rooted at `root` (exponents root_e).
+// confined to the subtree rooted at `root` (exponents root_e).  Pieces of
+// pairs the jump walk takes are stored in the same 88-byte slots as eleven
+// tagged words (JumpTaskWords).
 struct SplitTask {
   uint64_t key[3];
   uint32_t pair;
   uint16_t level, root;
   uint8_t e[3], ready;
   uint8_t root_e[3], pad;
+  uint64_t pad2[6];
 };
-static_assert(sizeof(SplitTask) == 40, "split task layout");
+static_assert(sizeof(SplitTask) == 88, "split task layout");
 
-// A split pair's accumulators live in the key bytes of its WalkState slot
+// The wave pass runs the pieces of jump-walk pairs in the same kernel, on
+// other waves (of other XCDs, whose L2 caches are not coherent with each
+// other).  A piece's words are written and read as device-scope atomics and
+// each carries the pass's epoch (31 bits, a new one per call, never 0), so
+// that each word shows by itself whether it was written in this pass (the
+// slots are reused from call to call); no cache-wide release or acquire is
+// needed.  A piece is a lane's walk: ten words, payload in the upper half,
+// epoch in the lower (key t, u, v; pair | ready << 31; level | exponents <<
+// 8; the giver's best key; the pending-sibling marks in four halves), and a
+// control word the queue marks (a slot's waiting wave may leave; the piece
+// is done) without touching the piece.
+struct JumpTaskWords {
+  uint64_t w[11];
+};
+constexpr int kPieceWords = 10, kControlWord = 10;
+
+struct JumpPiece {
+  uint32_t key[3];
+  uint32_t pair;
+  uint32_t lep;
+  uint32_t best;
+  uint64_t low[2];
+  bool ready;
+};
+static_assert(sizeof(JumpTaskWords) == sizeof(SplitTask), "jump pieces share the task slots");
+
+__device__ inline uint64_t tag_word(uint32_t payload, uint32_t epoch) { return ((uint64_t)payload << 32) | epoch; }
+__device__ inline bool tag_ok(uint64_t v, uint32_t epoch) { return (uint32_t)v == epoch; }
+
+// A split pair's accumulators live in the first bytes of its WalkState slot
 // once the walk it held has been resumed: best (u64 at byte 0, atomicMin),
 // boxes (u32 at byte 8, atomicAdd), over-budget flag (u32 at byte 12,
-// atomicOr).  The rest of the slot, with the box count the stream kernel had
-// reached, is left as it was.
+// atomicOr).  The wave that lists the pair sets them before it writes any
+// piece of the pair, and every other wave finds the slot through a piece's
+// pair word, so each of its accesses follows the setting.  The rest of the
+// slot is left as it was.
 struct SplitAcc {
   unsigned long long best;
   uint32_t boxes;
@@ -856,15 +890,138 @@ struct SplitAcc {
 };
 static_assert(sizeof(SplitAcc) == sizeof(WalkState), "split accumulators reuse the walk state slot");
 
+__device__ inline unsigned long long acc_load_best(const SplitAcc* a) {
+  return __hip_atomic_load(&a->best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline uint32_t acc_load_over(const SplitAcc* a) {
+  return __hip_atomic_load(&a->over, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Sets a jump-walk pair's accumulators when the pair is listed (lane 0).
+// Each word is set by compare-and-swap and the loop waits for its result,
+// so the setting is done before the wave goes on to write the pair's pieces
+// (no other wave touches the slot meanwhile: each swap succeeds at once).
+__device__ inline void acc_set(SplitAcc* a, uint64_t best, uint32_t boxes) {
+  unsigned long long cur = __hip_atomic_load(&a->best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  while (!__hip_atomic_compare_exchange_strong(&a->best, &cur, (unsigned long long)best, __ATOMIC_RELAXED,
+                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+  }
+  auto* cw = reinterpret_cast<unsigned long long*>(&a->boxes);  // boxes (low half), over (high half)
+  cur = __hip_atomic_load(cw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  while (!__hip_atomic_compare_exchange_strong(cw, &cur, (unsigned long long)boxes, __ATOMIC_RELAXED,
+                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+  }
+}
+
+// The jump-walk queue's counters, each on a 128-byte line of its own (the
+// busy waves read `count` and `next` every few steps, waiting waves poll
+// their own slots), in the last kHeadSlots task slots: the epoch is kept
+// there from call to call, the rest set to 0 by the epoch kernel.
+struct alignas(128) QueueLine {
+  uint32_t v;
+  uint32_t pad[31];
+};
+struct QueueHead {
+  QueueLine epoch, count, next, pending;
+};
+constexpr uint32_t kHeadSlots = 12;  // 672 bytes: a 128-aligned QueueHead fits
+static_assert(sizeof(QueueHead) + 128 <= kHeadSlots * sizeof(SplitTask), "queue head room");
+
+// The wave pass's shared task queue.  Jump-walk pieces take slots [0,
+// jump_cap) through task_count (reserved); a waiting wave takes a ticket
+// (task_next) and runs the slot of that number once it is written.  The
+// other walk's pieces take [jump_cap, task_cap) through lw_count and run in
+// the closing task pass only.  `pending` counts the waves still on heavy
+// pairs plus the pieces given and not yet done: work comes only from work,
+// so once it reads 0 it stays 0 and the waiting waves leave.
 struct SplitOut {
-  uint64_t split_at;     // boxes after which a pair is split (0: never)
+  uint64_t split_at;     // boxes a wave takes on a pair or task before it gives pieces away (0: never)
   SplitTask* tasks;
-  uint32_t task_cap;
-  uint32_t* task_count;  // reserved task slots
+  uint32_t task_cap;     // usable slots
+  uint32_t jump_cap;     // slots of the jump-walk pieces
+  QueueHead* head;
+  uint32_t epoch;        // this pass's tag, read by each kernel at its start
+  uint32_t* task_count;  // reserved jump-walk slots
+  uint32_t* task_next;   // taken jump-walk slots
+  uint32_t* lw_count;    // reserved slots of the other walk (from jump_cap)
+  uint32_t* lw_next;     // their cursor in the closing task pass
+  uint32_t* pending;
   uint32_t* split_list;  // split pairs
   uint32_t* split_count;
+  uint32_t* report;      // the jump-walk piece count, copied here at the end
   SplitAcc* acc;         // per pair (the walk state slots)
 };
+
+// Lane 0 reserves `need` (> 0) slots of `count` below `cap`; returns the
+// first or ~0u when there is no room (the caller keeps its work).
+__device__ inline uint32_t reserve_slots(uint32_t* count, uint32_t cap, uint32_t need) {
+  uint32_t cur = __hip_atomic_load(count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (int tries = 0; tries < 64; ++tries) {
+    if (cur + need > cap || cur + need < cur) break;
+    if (__hip_atomic_compare_exchange_strong(count, &cur, cur + need, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT))
+      return cur;
+  }
+  return ~0u;
+}
+
+// Lane 0 reserves up to `want` jump-walk slots, no more than the tickets
+// taken (waves waiting for a piece, or about to), so that donors seeing the
+// same waiting waves do not all give to them; returns the first slot and
+// sets *got, or ~0u.
+__device__ inline uint32_t reserve_for_tickets(const SplitOut& so, uint32_t want, uint32_t* got) {
+  uint32_t cur = __hip_atomic_load(so.task_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (int tries = 0; tries < 16; ++tries) {
+    uint32_t lim = __hip_atomic_load(so.task_next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    lim = lim < so.jump_cap ? lim : so.jump_cap;
+    if (cur >= lim) break;
+    const uint32_t n = min(want, lim - cur);
+    if (__hip_atomic_compare_exchange_strong(so.task_count, &cur, cur + n, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT)) {
+      *got = n;
+      return cur;
+    }
+  }
+  return ~0u;
+}
+
+// One jump-walk piece, written word by word as device-scope atomics.
+__device__ inline void put_jump_task(const SplitOut& so, uint32_t slot, const JumpPiece& pc) {
+  auto* w = reinterpret_cast<JumpTaskWords*>(so.tasks + slot)->w;
+  const uint32_t payload[kPieceWords] = {pc.key[0], pc.key[1], pc.key[2], pc.pair | (pc.ready ? 1u << 31 : 0u),
+                                         pc.lep, pc.best, (uint32_t)pc.low[0], (uint32_t)(pc.low[0] >> 32),
+                                         (uint32_t)pc.low[1], (uint32_t)(pc.low[1] >> 32)};
+#pragma unroll
+  for (int i = 0; i < kPieceWords; ++i)
+    __hip_atomic_store(&w[i], tag_word(payload[i], so.epoch), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ inline JumpPiece unpack_piece(const uint32_t t[kPieceWords]) {
+  JumpPiece pc;
+  pc.key[0] = t[0];
+  pc.key[1] = t[1];
+  pc.key[2] = t[2];
+  pc.pair = t[3] & 0x7FFFFFFFu;
+  pc.ready = (t[3] >> 31) != 0;
+  pc.lep = t[4];
+  pc.best = t[5];
+  pc.low[0] = (uint64_t)t[6] | ((uint64_t)t[7] << 32);
+  pc.low[1] = (uint64_t)t[8] | ((uint64_t)t[9] << 32);
+  return pc;
+}
+
+// Reads a jump-walk piece's payload; false if any word is not yet this pass's.
+__device__ inline bool get_jump_task(const SplitOut& so, uint32_t slot, uint32_t out[kPieceWords]) {
+  auto* w = reinterpret_cast<JumpTaskWords*>(so.tasks + slot)->w;
+  bool ok = true;
+#pragma unroll
+  for (int i = 0; i < kPieceWords; ++i) {
+    const uint64_t v = __hip_atomic_load(&w[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    ok = ok && tag_ok(v, so.epoch);
+    out[i] = (uint32_t)(v >> 32);
+  }
+  return ok;
+}
 
 __device__ inline SplitTask make_task(uint32_t pair, const uint64_t k[3], uint32_t level, uint32_t ep, bool ready,
                                       uint32_t root, uint32_t root_ep) {
@@ -888,10 +1045,10 @@ __device__ inline SplitTask make_task(uint32_t pair, const uint64_t k[3], uint32
 // Heavy mode starts from the pair's saved walk (or the root); once the pair
 // has taken more than split_at boxes it may split: every pending piece (each
 // lane's path siblings, each lane's current subtree or unfinished walk, every
-// queued task) becomes a task, the pair's best and box count go to its
-// accumulators, and the pair is listed for the finishing pass.  Task mode
-// walks one piece with the pair's shared best: read from the accumulator
-// every 16 steps and lowered there on every improvement.  Every form of the
+// queued task) becomes a task for the closing task pass, the pair's best and
+// box count go to its accumulators, and the pair is listed for the finishing
+// pass.  Task mode walks one piece with the pair's shared best: read from the
+// accumulator every 16 steps and lowered there on every improvement.  Every form of the
 // result is the minimum t_lo over done boxes, so the pieces' order is free.
 enum : int { kWaveDone = 0, kWaveOver = 1, kWaveSplit = 2 };
 
@@ -950,20 +1107,10 @@ __device__ int wave_certify(const TuvPair& q, uint32_t p, WaveTask* wq, unsigned
       const uint32_t off = wave_prefix_u8(mine, &lanes_tot);
       const uint32_t need = lanes_tot + (uint32_t)qsize;
       uint32_t base = 0xFFFFFFFFu;
-      if (lane == 0) {
-        uint32_t cur = __hip_atomic_load(so.task_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        for (int tries = 0; tries < 64; ++tries) {
-          if (cur + need > so.task_cap || cur + need < cur) break;
-          if (__hip_atomic_compare_exchange_strong(so.task_count, &cur, cur + need, __ATOMIC_RELAXED,
-                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-            base = cur;
-            break;
-          }
-        }
-      }
+      if (lane == 0 && need > 0) base = reserve_slots(so.lw_count, so.task_cap - so.jump_cap, need);
       base = __shfl(base, 0);
       if (base != 0xFFFFFFFFu) {
-        SplitTask* out = so.tasks + base;
+        SplitTask* out = so.tasks + so.jump_cap + base;
         if (has) {
           uint32_t slot = off;
           if (w.ready) {
@@ -1116,17 +1263,32 @@ __device__ inline void jump_resume(const TuvPair& q, const WalkState& st, JumpWa
   w.low[1] = __ballot(lane + 64u < w.level && jump_pending_at(q, w, lane + 64u));
 }
 
-__device__ inline SplitTask jump_task(uint32_t pair, const uint32_t key[3], uint32_t level, uint32_t ep) {
-  const uint64_t k[3] = {jump_key64(key[0]), jump_key64(key[1]), jump_key64(key[2])};
-  return make_task(pair, k, level, ep, true, level, ep);
-}
+// Box accounting of one unit of work (a pair or a task), wave-uniform:
+// `boxes` evaluated so far (a resumed pair's include those of the stream
+// kernel), `flushed` of them already added to the pair's accumulator, and
+// `listed` once the pair's accumulators are live.
+struct UnitCount {
+  uint64_t boxes;
+  uint64_t flushed;
+  bool listed;
+};
 
-// The wave pass over one pair (heavy mode: `active` on lane 0 only, its walk
-// resumed or at the root) or one split piece (task mode).  Returns kWaveDone,
-// kWaveOver or kWaveSplit; w.best is the wave's least done key on return.
-template <bool kTask, bool kSplit>
+__device__ inline uint32_t clamp_u32(uint64_t v) { return v < 0xFFFFFFFFull ? (uint32_t)v : 0xFFFFFFFFu; }
+
+// The shared task queue and a listed pair's accumulators are looked at every
+// 32 steps; a wave adds its boxes to the pair's sum once 4096 have gathered.
+constexpr uint32_t kLookMask = 31u;
+constexpr uint64_t kFlushBoxes = 4096;
+
+// The wave pass over one pair (`active` on lane 0 only, its walk resumed or
+// at the root) or one task (a listed pair's piece).  With kSplit, when a look
+// finds more waiting waves than queued tasks and the wave has taken split_at
+// boxes on this unit, lanes holding marks give their shallowest (at most one
+// per lane) to the queue; the first such gift lists the pair.  Returns
+// kWaveDone or kWaveOver; w.best is the wave's least done key on return.
+template <bool kSplit>
 __device__ int wave_certify_jump(const TuvPair& q, uint32_t p, JumpBox* stk, JumpWalk& w, bool active,
-                                 uint64_t& boxes, double tol, uint64_t max_itr, const SplitOut& so) {
+                                 UnitCount& uc, double tol, uint64_t max_itr, const SplitOut& so) {
   const int lane = threadIdx.x & 63;
   const uint64_t lt = (1ull << lane) - 1ull;
   int sp = 0;  // stacked subtrees (wave-uniform)
@@ -1158,67 +1320,107 @@ __device__ int wave_certify_jump(const TuvPair& q, uint32_t p, JumpBox* stk, Jum
       }
       if (finished && !(wide ? jump_next(q, w) : jump_next_lo(q, w))) active = false;
     }
-    boxes += (uint64_t)__popcll(__ballot(evaluated));
-    // 2. share the least done t_lo
-    const bool poll = kTask && (++step & 15u) == 0;
-    if (__ballot(found) != 0 || poll) {
+    uc.boxes += (uint64_t)__popcll(__ballot(evaluated));
+    // 2. share the least done t_lo: a find lowers the pair's accumulator,
+    //    a look reads it; a look also adds the boxes (once kFlushBoxes have
+    //    gathered; the sum decides the budget) and reads the task queue
+    const bool look = (kSplit || uc.listed) && (++step & kLookMask) == 0;
+    const bool any_found = __ballot(found) != 0;
+    bool over = uc.boxes > max_itr;
+    int deficit = 0;
+    if (any_found || look) {
       uint32_t b = wave_min_u32(w.best);
-      if (kTask) {
-        unsigned long long g = 0;
-        if (lane == 0) g = atomicMin(&so.acc[p].best, (unsigned long long)jump_best64(b));
-        g = __shfl(g, 0);
-        const uint32_t gb = best32_of(g);
-        b = gb < b ? gb : b;
+      const bool flush = look && uc.listed && uc.boxes - uc.flushed >= kFlushBoxes;
+      if (uc.listed || look) {
+        uint32_t g = kJumpNoHit, flags = 0;
+        int d = 0;
+        if (lane == 0) {
+          if (uc.listed) {
+            SplitAcc* a = &so.acc[p];
+            g = best32_of(any_found ? atomicMin(&a->best, (unsigned long long)jump_best64(b)) : acc_load_best(a));
+            if (flush) {
+              const uint32_t add = clamp_u32(uc.boxes - uc.flushed);
+              const uint64_t tot = (uint64_t)atomicAdd(&a->boxes, add) + add;
+              flags = tot > max_itr || acc_load_over(a) != 0 ? 1u : 0u;
+            }
+          }
+          if (kSplit && look) {
+            // tickets beyond the written slots: waves waiting for a piece
+            const uint32_t count = __hip_atomic_load(so.task_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const uint32_t next = __hip_atomic_load(so.task_next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            d = next > count ? (int)min(next - count, 64u) : 0;
+          }
+        }
+        g = (uint32_t)__shfl((int)g, 0);
+        b = g < b ? g : b;
+        if (flush) {
+          uc.flushed = uc.boxes;
+          over = over || __shfl((int)flags, 0) != 0;
+        }
+        deficit = __shfl(d, 0);
       }
       w.best = b;
     }
-    if (boxes > max_itr) return kWaveOver;
-    if (kSplit && boxes > so.split_at) {
-      // every pending piece becomes a task: each lane's current box and
-      // marked siblings, and the stacked subtrees
-      const uint32_t mine = active ? (uint32_t)(__popcll(w.low[0]) + __popcll(w.low[1])) + (w.ready ? 1u : 0u) : 0u;
-      uint32_t lanes_tot = 0;
-      const uint32_t off = wave_prefix_u8(mine, &lanes_tot);  // at most 129 per lane
-      const uint32_t need = lanes_tot + (uint32_t)sp;
-      uint32_t base = 0xFFFFFFFFu;
-      if (lane == 0) {
-        uint32_t cur = __hip_atomic_load(so.task_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        for (int tries = 0; tries < 64; ++tries) {
-          if (cur + need > so.task_cap || cur + need < cur) break;
-          if (__hip_atomic_compare_exchange_strong(so.task_count, &cur, cur + need, __ATOMIC_RELAXED,
-                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-            base = cur;
-            break;
-          }
-        }
-      }
-      base = __shfl(base, 0);
-      if (base != 0xFFFFFFFFu) {
-        SplitTask* out = so.tasks + base;
-        if (active) {
-          uint32_t slot = off;
-          if (w.ready) out[slot++] = jump_task(p, w.key, w.level, w.ep);
-          while ((w.low[0] | w.low[1]) != 0) {
-            uint32_t k[3], ep, lvl;
-            jump_take_shallowest(q, w, k, &ep, &lvl);
-            out[slot++] = jump_task(p, k, lvl, ep);
-          }
-        }
-        for (int i = lane; i < sp; i += 64) {
-          const JumpBox b = stk[i];
-          out[lanes_tot + (uint32_t)i] = jump_task(p, b.key, b.lep & 0xffu, b.lep >> 8);
-        }
+    if (over) return kWaveOver;
+    // 3. pieces for waiting waves: every other active lane hands its whole
+    //    walk (current box and pending siblings) over, so that the wave's
+    //    remaining work about halves whatever the tree's shape; a lone lane
+    //    gives its shallowest pending sibling
+    if (kSplit && deficit > 0 && uc.boxes >= so.split_at) {
+      const uint64_t am = __ballot(active);
+      const uint32_t n_active = (uint32_t)__popcll(am);
+      const bool lone = n_active == 1;
+      const bool lone_can = lone && active && (w.low[0] | w.low[1]) != 0;
+      uint32_t give = lone ? (__ballot(lone_can) != 0 ? 1u : 0u) : min(n_active >> 1, (uint32_t)deficit);
+      if (give > 0) {
+        uint32_t base = ~0u;
         if (lane == 0) {
-          so.acc[p].best = jump_best64(w.best);
-          so.acc[p].boxes = (uint32_t)boxes;
-          so.acc[p].over = 0;
-          const uint32_t at = atomicAdd(so.split_count, 1u);
-          so.split_list[at] = p;  // at most one entry per listed pair
+          base = reserve_for_tickets(so, give, &give);
+          if (base != ~0u) atomicAdd(so.pending, give);
         }
-        return kWaveSplit;
+        base = (uint32_t)__shfl((int)base, 0);
+        give = (uint32_t)__shfl((int)give, 0);
+        if (base != ~0u) {
+          if (!uc.listed) {
+            // the pair's accumulators take the wave's best and boxes so far
+            if (lane == 0) {
+              acc_set(&so.acc[p], jump_best64(w.best), clamp_u32(uc.boxes));
+              const uint32_t at = atomicAdd(so.split_count, 1u);
+              so.split_list[at] = p;  // at most one entry per listed pair
+            }
+            uc.flushed = uc.boxes;
+            uc.listed = true;
+          }
+          const uint32_t rank = (uint32_t)__popcll(am & lt);
+          JumpPiece pc;
+          pc.pair = p;
+          pc.best = w.best;
+          if (lone) {
+            if (lone_can) {
+              uint32_t lvl, ep;
+              jump_take_shallowest(q, w, pc.key, &ep, &lvl);
+              pc.lep = lvl | (ep << 8);
+              pc.low[0] = pc.low[1] = 0;
+              pc.ready = true;
+              put_jump_task(so, base, pc);
+            }
+          } else if (active && (rank & 1u) != 0 && (rank >> 1) < give) {
+            pc.key[0] = w.key[0];
+            pc.key[1] = w.key[1];
+            pc.key[2] = w.key[2];
+            pc.lep = w.level | (w.ep << 8);
+            pc.low[0] = w.low[0];
+            pc.low[1] = w.low[1];
+            pc.ready = w.ready;
+            put_jump_task(so, base + (rank >> 1), pc);
+            active = false;
+            w.low[0] = w.low[1] = 0;
+            w.ready = false;
+          }
+        }
       }
     }
-    // 3. idle lanes beyond the stacked subtrees: lanes with marks give their shallowest
+    // 4. idle lanes beyond the stacked subtrees: lanes with marks give their shallowest
     const uint64_t idle_m = __ballot(!active);
     const int idle = __popcll(idle_m);
     if (idle > sp && sp < kJumpStack) {
@@ -1229,28 +1431,28 @@ __device__ int wave_certify_jump(const TuvPair& q, uint32_t p, JumpBox* stk, Jum
       if (can) {
         const int r = __popcll(cm & lt);
         if (r < give) {
-          JumpBox b;
+          JumpBox bx;
           uint32_t ep, lvl;
-          jump_take_shallowest(q, w, b.key, &ep, &lvl);
-          b.lep = lvl | (ep << 8);
-          stk[sp + r] = b;
+          jump_take_shallowest(q, w, bx.key, &ep, &lvl);
+          bx.lep = lvl | (ep << 8);
+          stk[sp + r] = bx;
         }
       }
       sp += give;
       wave_sync_lds();
     }
-    // 4. idle lanes take stacked subtrees, the top first
+    // 5. idle lanes take stacked subtrees, the top first
     if (idle > 0 && sp > 0) {
       const int take = min(idle, sp);
       if (!active) {
         const int r = __popcll(idle_m & lt);
         if (r < take) {
-          const JumpBox b = stk[sp - 1 - r];
-          w.key[0] = b.key[0];
-          w.key[1] = b.key[1];
-          w.key[2] = b.key[2];
-          w.level = b.lep & 0xffu;
-          w.ep = b.lep >> 8;
+          const JumpBox bx = stk[sp - 1 - r];
+          w.key[0] = bx.key[0];
+          w.key[1] = bx.key[1];
+          w.key[2] = bx.key[2];
+          w.level = bx.lep & 0xffu;
+          w.ep = bx.lep >> 8;
           w.low[0] = w.low[1] = 0;
           w.ready = w.key[0] < w.best;
           active = true;
@@ -1263,8 +1465,162 @@ __device__ int wave_certify_jump(const TuvPair& q, uint32_t p, JumpBox* stk, Jum
   }
 }
 
+// A listed pair's unit is done (lane 0): its best and its boxes not yet
+// added go to the accumulators, and an over-budget unit marks the pair.
+__device__ inline void close_unit(const SplitOut& so, uint32_t p, uint32_t best, const UnitCount& uc, bool over) {
+  if ((threadIdx.x & 63) == 0) {
+    atomicMin(&so.acc[p].best, (unsigned long long)jump_best64(best));
+    atomicAdd(&so.acc[p].boxes, clamp_u32(uc.boxes - uc.flushed));
+    if (over) atomicOr(&so.acc[p].over, 1u);
+  }
+}
+
+// One jump-walk piece (a lane's walk), by one wave (kSplit: it may give
+// pieces of its own).  A piece of a pair already over budget is dropped: the
+// pair goes to the canonical replay.
+template <bool kSplit, typename Src>
+__device__ void run_jump_task(const Src& src, const SplitOut& so, const JumpPiece& pc, WaveLds& lds, double tol,
+                              uint64_t max_itr) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t p = pc.pair;
+  SplitAcc* a = &so.acc[p];
+  if (acc_load_over(a) != 0) return;
+  const TuvPair q = src.load(p);
+  JumpWalk w;
+  start_jump_walk(w);
+  w.key[0] = pc.key[0];
+  w.key[1] = pc.key[1];
+  w.key[2] = pc.key[2];
+  w.level = pc.lep & 0xffu;
+  w.ep = pc.lep >> 8;
+  const uint32_t g = best32_of(acc_load_best(a));
+  w.best = g < pc.best ? g : pc.best;
+  w.ready = pc.ready && w.key[0] < w.best;
+  if (lane == 0) {
+    w.low[0] = pc.low[0];
+    w.low[1] = pc.low[1];
+  }
+  UnitCount uc{0, 0, true};
+  const int r = wave_certify_jump<kSplit>(q, p, lds.stack, w, lane == 0, uc, tol, max_itr, so);
+  close_unit(so, p, w.best, uc, r == kWaveOver);
+}
+
+// One piece of a 64-bit-key walk (closing task pass only).
+template <typename Src>
+__device__ void run_lw_task(const Src& src, const SplitOut& so, const SplitTask& t, WaveLds& lds,
+                            unsigned long long* best_slot, double tol, uint64_t max_itr) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t p = t.pair;
+  if (so.acc[p].over != 0) return;
+  const TuvPair q = src.load(p);
+  LaneWalk w;
+  start_walk(w);
+  w.best = so.acc[p].best;
+  w.key[0] = t.key[0];
+  w.key[1] = t.key[1];
+  w.key[2] = t.key[2];
+  w.level = t.level;
+  w.ep = (uint32_t)t.e[0] | ((uint32_t)t.e[1] << 8) | ((uint32_t)t.e[2] << 16);
+  w.ready = t.ready != 0;
+  const uint32_t root_ep = (uint32_t)t.root_e[0] | ((uint32_t)t.root_e[1] << 8) | ((uint32_t)t.root_e[2] << 16);
+  const bool has = lane == 0 && (!w.ready || !box_skipped(q, w));
+  if (lane == 0) *best_slot = w.best;
+  wave_sync_lds();
+  uint64_t boxes = 0;
+  const int r =
+      wave_certify<true, false>(q, p, lds.queue, best_slot, w, has, t.root, root_ep, boxes, tol, max_itr, so);
+  if (lane == 0) {
+    const uint64_t b = *best_slot < w.best ? *best_slot : w.best;
+    atomicMin(&so.acc[p].best, (unsigned long long)b);
+    atomicAdd(&so.acc[p].boxes, clamp_u32(boxes));
+    if (r == kWaveOver) atomicOr(&so.acc[p].over, 1u);
+  }
+}
+
+// A wave done with the heavy pairs takes a ticket and runs the jump-walk
+// piece of that slot once all ten of its words are this pass's, then takes
+// the next ticket.  A waiting wave polls only its own slot, so waiting waves
+// load no shared line.  The wave that takes `pending` to 0 (no work left, and
+// none can come: work comes only from work) writes a leave mark into the
+// first word of every slot a ticket waits on; a wave whose ticket it missed
+// (taken meanwhile) sees `pending` at 0 on one of its rare checks.  Marks go
+// to the slots' control words and never touch a piece.  The
+// pieces and counters are device-scope atomics, read without cache-wide
+// maintenance (an acquire invalidates the L2 of the wave's XCD, which a few
+// thousand polling waves would keep empty).  A done slot is marked in its
+// control word; after kServeSpins polls a waiting wave leaves anyway, and the
+// closing task pass runs every written slot not marked done.
+constexpr uint32_t kServeSpins = 1u << 20;
+constexpr uint32_t kSlotDone = 1u, kSlotLeave = 2u;  // control word marks
+
+__device__ inline void mark_slot(const SplitOut& so, uint32_t slot, uint32_t mark) {
+  auto* w = reinterpret_cast<JumpTaskWords*>(so.tasks + slot)->w;
+  __hip_atomic_store(&w[kControlWord], tag_word(mark, so.epoch), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Lane 0 took `pending` to 0: leave marks for the waiting tickets.
+__device__ inline void release_waiters(const SplitOut& so) {
+  const int lane = threadIdx.x & 63;
+  uint32_t lo = 0, hi = 0;
+  if (lane == 0) {
+    lo = __hip_atomic_load(so.task_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    hi = __hip_atomic_load(so.task_next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  lo = (uint32_t)__shfl((int)lo, 0);
+  hi = min((uint32_t)__shfl((int)hi, 0), so.jump_cap);
+  for (uint32_t t = lo + (uint32_t)lane; t < hi; t += 64) mark_slot(so, t, kSlotLeave);
+}
+
+// Lane 0 lowers `pending`; every lane returns whether it reached 0.
+__device__ inline bool drop_pending(const SplitOut& so) {
+  uint32_t last = 0;
+  if ((threadIdx.x & 63) == 0) last = atomicSub(so.pending, 1u) == 1u ? 1u : 0u;
+  return __shfl((int)last, 0) != 0;
+}
+
+template <typename Src>
+__device__ void serve_tasks(const Src& src, const SplitOut& so, WaveLds& lds, double tol, uint64_t max_itr) {
+  const int lane = threadIdx.x & 63;
+  if (drop_pending(so)) release_waiters(so);  // this wave's heavy pairs are done
+  for (;;) {
+    uint32_t ticket = 0;
+    if (lane == 0) ticket = atomicAdd(so.task_next, 1u);
+    ticket = (uint32_t)__shfl((int)ticket, 0);
+    if (ticket >= so.jump_cap) return;
+    uint32_t t[kPieceWords] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    uint32_t state = 0;  // 1: piece read, 2: leave
+    for (uint32_t spins = 0; state == 0; ++spins) {
+      if (lane == 0) {
+        const auto* w = reinterpret_cast<JumpTaskWords*>(so.tasks + ticket)->w;
+        const uint64_t w0 = __hip_atomic_load(&w[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint64_t wc = __hip_atomic_load(&w[kControlWord], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (tag_ok(w0, so.epoch) && get_jump_task(so, ticket, t)) {
+          state = 1;
+        } else if (tag_ok(wc, so.epoch) && (uint32_t)(wc >> 32) == kSlotLeave) {
+          state = 2;
+        } else if ((spins & 255u) == 255u &&
+                   __hip_atomic_load(so.pending, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u &&
+                   __hip_atomic_load(so.task_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= ticket) {
+          state = 2;
+        }
+        if (state == 0 && spins + 1 >= kServeSpins) state = 2;
+      }
+      state = (uint32_t)__shfl((int)state, 0);
+      if (state == 0) __builtin_amdgcn_s_sleep(8);
+    }
+    if (state == 2) return;
+#pragma unroll
+    for (int i = 0; i < kPieceWords; ++i) t[i] = (uint32_t)__shfl((int)t[i], 0);
+    run_jump_task<true>(src, so, unpack_piece(t), lds, tol, max_itr);
+    __builtin_amdgcn_wave_barrier();
+    if (lane == 0) mark_slot(so, ticket, kSlotDone);
+    if (drop_pending(so)) release_waiters(so);
+  }
+}
+
 // Three workgroups per CU (12 waves: 3 per SIMD; 147 KB of LDS): at most 168
 // VGPRs, a few spilled; the wave pass ran 20 % longer at 2 waves per SIMD.
+// With kSplit, a wave done with the heavy pairs serves the task queue.
 template <bool kSplit, typename Src>
 __global__ __launch_bounds__(kTuvBlock, 3) void tuv_heavy_kernel(Src src,
                                                               const uint32_t* __restrict__ heavy_list,
@@ -1273,10 +1629,16 @@ __global__ __launch_bounds__(kTuvBlock, 3) void tuv_heavy_kernel(Src src,
                                                               const WalkState* __restrict__ resume, double tol,
                                                               uint64_t max_itr, p2c_tuv_result* __restrict__ out,
                                                               uint32_t* __restrict__ replay_list,
-                                                              uint32_t* __restrict__ replay_count, SplitOut so) {
+                                                              uint32_t* __restrict__ replay_count,
+                                                              SplitOut so_in) {
   __shared__ WaveLds lds[kTuvBlock / 64];
   __shared__ unsigned long long best_slot[kTuvBlock / 64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  SplitOut so = so_in;
+  if constexpr (kSplit) {
+    so.epoch = so.head->epoch.v;
+    if (lane == 0) atomicAdd(so.pending, 1u);
+  }
   uint32_t total = *heavy_count;
   if (total > list_cap) total = list_cap;
   for (;;) {
@@ -1289,11 +1651,11 @@ __global__ __launch_bounds__(kTuvBlock, 3) void tuv_heavy_kernel(Src src,
     if (jump_walk_fits(q)) {
       JumpWalk w;
       start_jump_walk(w);
-      uint64_t boxes = 0;
       bool active = lane == 0;
+      UnitCount uc{0, 0, false};
       if (resume != nullptr) {
         const WalkState st = resume[p];
-        boxes = st.boxes;
+        uc.boxes = st.boxes;
         jump_resume(q, st, w);
         // every lane at once: lane j + 1 takes the upper sibling at marked
         // level j < 63, lane 0 keeps the current box and the deeper marks
@@ -1314,9 +1676,11 @@ __global__ __launch_bounds__(kTuvBlock, 3) void tuv_heavy_kernel(Src src,
           w.low[0] = w.low[1] = 0;
         }
       }
-      const int r = wave_certify_jump<false, kSplit>(q, p, lds[wv].stack, w, active, boxes, tol, max_itr, so);
-      if (r != kWaveSplit && lane == 0) {
-        write_result(&out[p], jump_best64(w.best), boxes, r == kWaveOver ? P2C_TUV_FLAG_OVER_BUDGET : 0u);
+      const int r = wave_certify_jump<kSplit>(q, p, lds[wv].stack, w, active, uc, tol, max_itr, so);
+      if (uc.listed) {
+        close_unit(so, p, w.best, uc, r == kWaveOver);
+      } else if (lane == 0) {
+        write_result(&out[p], jump_best64(w.best), uc.boxes, r == kWaveOver ? P2C_TUV_FLAG_OVER_BUDGET : 0u);
         if (r == kWaveOver) flag_for_replay(p, replay_list, replay_count, list_cap);
       }
       __builtin_amdgcn_wave_barrier();
@@ -1345,81 +1709,54 @@ __global__ __launch_bounds__(kTuvBlock, 3) void tuv_heavy_kernel(Src src,
     }
     __builtin_amdgcn_wave_barrier();
   }
+  if constexpr (kSplit) serve_tasks(src, so, lds[wv], tol, max_itr);
 }
 
-// Split tasks, one wave per task; boxes and an over-budget mark go to the
-// pair's accumulators.
+// The closing task pass, one wave per piece, without further pieces: the
+// written jump-walk pieces not marked done (normally none; read here as
+// plain memory, written before this kernel began), then every piece of the
+// 64-bit-key walk.
 template <typename Src>
-__global__ __launch_bounds__(kTuvBlock, 3) void tuv_task_kernel(Src src,
-                                                             uint32_t* __restrict__ task_next, double tol,
-                                                             uint64_t max_itr, SplitOut so) {
+__global__ __launch_bounds__(kTuvBlock, 3) void tuv_task_kernel(Src src, double tol, uint64_t max_itr,
+                                                             SplitOut so_in) {
   __shared__ WaveLds lds[kTuvBlock / 64];
   __shared__ unsigned long long best_slot[kTuvBlock / 64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  uint32_t total = *so.task_count;
-  if (total > so.task_cap) total = so.task_cap;
+  SplitOut so = so_in;
+  so.epoch = so.head->epoch.v;
+  uint32_t jump_total = *so.task_count;
+  if (jump_total > so.jump_cap) jump_total = so.jump_cap;
+  const uint32_t waves = gridDim.x * (kTuvBlock / 64);
+  for (uint32_t h = blockIdx.x * (kTuvBlock / 64) + (uint32_t)wv; h < jump_total; h += waves) {
+    const auto* words = reinterpret_cast<const JumpTaskWords*>(so.tasks + h)->w;
+    const uint64_t c = words[kControlWord];
+    if (tag_ok(c, so.epoch) && (uint32_t)(c >> 32) == kSlotDone) continue;
+    uint32_t t[kPieceWords];
+#pragma unroll
+    for (int i = 0; i < kPieceWords; ++i) t[i] = (uint32_t)(words[i] >> 32);
+    run_jump_task<false>(src, so, unpack_piece(t), lds[wv], tol, max_itr);
+    __builtin_amdgcn_wave_barrier();
+  }
+  uint32_t lw_total = *so.lw_count;
+  if (lw_total > so.task_cap - so.jump_cap) lw_total = so.task_cap - so.jump_cap;
   for (;;) {
     uint32_t h = 0;
-    if (lane == 0) h = atomicAdd(task_next, 1u);
+    if (lane == 0) h = atomicAdd(so.lw_next, 1u);
     h = __shfl(h, 0);
-    if (h >= total) break;
-    const SplitTask t = so.tasks[h];
-    const uint32_t p = t.pair;
-    const TuvPair q = src.load(p);
-    if (jump_walk_fits(q)) {
-      // pieces of such pairs are whole subtrees (a box, ready, rooted at itself)
-      JumpWalk w;
-      start_jump_walk(w);
-      w.key[0] = (uint32_t)(t.key[0] >> 32);
-      w.key[1] = (uint32_t)(t.key[1] >> 32);
-      w.key[2] = (uint32_t)(t.key[2] >> 32);
-      w.level = t.level;
-      w.ep = (uint32_t)t.e[0] | ((uint32_t)t.e[1] << 8) | ((uint32_t)t.e[2] << 16);
-      w.best = best32_of(__hip_atomic_load(&so.acc[p].best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-      w.ready = w.key[0] < w.best;
-      uint64_t boxes = 0;
-      const int r = wave_certify_jump<true, false>(q, p, lds[wv].stack, w, lane == 0, boxes, tol, max_itr, so);
-      if (lane == 0) {
-        atomicMin(&so.acc[p].best, (unsigned long long)jump_best64(w.best));
-        atomicAdd(&so.acc[p].boxes, (uint32_t)(boxes < 0xFFFFFFFFull ? boxes : 0xFFFFFFFFull));
-        if (r == kWaveOver) atomicOr(&so.acc[p].over, 1u);
-      }
-      __builtin_amdgcn_wave_barrier();
-      continue;
-    }
-    WaveTask* queue_wv = lds[wv].queue;
-    LaneWalk w;
-    start_walk(w);
-    w.best = __hip_atomic_load(&so.acc[p].best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    w.key[0] = t.key[0];
-    w.key[1] = t.key[1];
-    w.key[2] = t.key[2];
-    w.level = t.level;
-    w.ep = (uint32_t)t.e[0] | ((uint32_t)t.e[1] << 8) | ((uint32_t)t.e[2] << 16);
-    w.ready = t.ready != 0;
-    const uint32_t root_ep = (uint32_t)t.root_e[0] | ((uint32_t)t.root_e[1] << 8) | ((uint32_t)t.root_e[2] << 16);
-    const bool has = lane == 0 && (!w.ready || !box_skipped(q, w));
-    if (lane == 0) best_slot[wv] = w.best;
-    wave_sync_lds();
-    uint64_t boxes = 0;
-    const int r = wave_certify<true, false>(q, p, queue_wv, &best_slot[wv], w, has, t.root, root_ep, boxes, tol,
-                                            max_itr, so);
-    if (lane == 0) {
-      const uint64_t b = best_slot[wv] < w.best ? best_slot[wv] : w.best;
-      atomicMin(&so.acc[p].best, (unsigned long long)b);
-      atomicAdd(&so.acc[p].boxes, (uint32_t)(boxes < 0xFFFFFFFFull ? boxes : 0xFFFFFFFFull));
-      if (r == kWaveOver) atomicOr(&so.acc[p].over, 1u);
-    }
+    if (h >= lw_total) break;
+    run_lw_task(src, so, so.tasks[so.jump_cap + h], lds[wv], &best_slot[wv], tol, max_itr);
     __builtin_amdgcn_wave_barrier();
   }
 }
 
 // Results of the split pairs once all their tasks are done.
-__global__ __launch_bounds__(kTuvBlock) void tuv_split_finish_kernel(SplitOut so, uint64_t max_itr,
+__global__ __launch_bounds__(kTuvBlock) void tuv_split_finish_kernel(SplitOut so_in, uint64_t max_itr,
                                                                      p2c_tuv_result* __restrict__ out,
                                                                      uint32_t* __restrict__ replay_list,
                                                                      uint32_t* __restrict__ replay_count,
                                                                      uint32_t list_cap) {
+  const SplitOut& so = so_in;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *so.report = *so.task_count;
   uint32_t total = *so.split_count;
   if (total > list_cap) total = list_cap;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
@@ -1428,6 +1765,18 @@ __global__ __launch_bounds__(kTuvBlock) void tuv_split_finish_kernel(SplitOut so
     const bool over = a.over != 0 || a.boxes > max_itr;
     write_result(&out[p], a.best, a.boxes, over ? P2C_TUV_FLAG_OVER_BUDGET : 0u);
     if (over) flag_for_replay(p, replay_list, replay_count, list_cap);
+  }
+}
+
+// The pass's epoch, kept in the queue head between calls: one more than
+// the last (31 bits, never 0); the queue's counters start at 0.
+__global__ void tuv_split_epoch_kernel(QueueHead* head) {
+  if (threadIdx.x == 0) {
+    const uint32_t e = (head->epoch.v + 1u) & 0x7FFFFFFFu;
+    head->epoch.v = e == 0 ? 1u : e;
+    head->count.v = 0;
+    head->next.v = 0;
+    head->pending.v = 0;
   }
 }
 
@@ -1592,20 +1941,32 @@ int launch_wave_split(const Src& src, void* counters, void* states, const uint32
   if (grid_blocks <= 0) return set_error(P2C_EINVAL, "grid_blocks must be positive");
   if (split_at < 0) return set_error(P2C_EINVAL, "split_at must be non-negative");
   const bool split = split_at > 0;
-  if (split && (tasks == nullptr || task_cap == 0 || split_list == nullptr))
-    return set_error(P2C_EINVAL, "splitting needs a task buffer and a split list");
+  if (split && (tasks == nullptr || task_cap < kHeadSlots + 4 || split_list == nullptr))
+    return set_error(P2C_EINVAL, "splitting needs a task buffer of at least 20 slots and a split list");
+  if (split && list_cap > 0x7FFFFFFFu) return set_error(P2C_EINVAL, "split lists hold pair ids below 2^31");
   auto* c32 = static_cast<uint32_t*>(counters);
   hipStream_t s = as_stream(stream);
   SplitOut so{};
   if (split) {
-    // counters (u32): [13] reserved task slots, [14] task cursor, [15] split pairs
+    // counters (u32): [8] 64-bit-walk pieces, [11] their cursor, [13] jump-walk pieces (reported
+    // at the end), [15] split pairs; the last kHeadSlots task slots hold the jump-walk queue's
+    // head (its epoch kept between calls); a quarter of the others holds the 64-bit walk's pieces
     so.split_at = (uint64_t)split_at;
     so.tasks = static_cast<SplitTask*>(tasks);
-    so.task_cap = task_cap;
-    so.task_count = c32 + 13;
+    so.task_cap = task_cap - kHeadSlots;
+    so.jump_cap = so.task_cap - so.task_cap / 4;
+    const uintptr_t head = (reinterpret_cast<uintptr_t>(so.tasks + so.task_cap) + 127u) & ~uintptr_t(127);
+    so.head = reinterpret_cast<QueueHead*>(head);
+    so.task_count = &so.head->count.v;
+    so.task_next = &so.head->next.v;
+    so.report = c32 + 13;
+    so.lw_count = c32 + 8;
+    so.lw_next = c32 + 11;
+    so.pending = &so.head->pending.v;
     so.split_list = split_list;
     so.split_count = c32 + 15;
     so.acc = static_cast<SplitAcc*>(states);
+    hipLaunchKernelGGL(tuv_split_epoch_kernel, dim3(1), dim3(64), 0, s, so.head);
   }
   const auto* resume = static_cast<const WalkState*>(states);
   if (split)
@@ -1615,7 +1976,7 @@ int launch_wave_split(const Src& src, void* counters, void* states, const uint32
     hipLaunchKernelGGL((tuv_heavy_kernel<false, Src>), dim3(grid_blocks), dim3(kTuvBlock), 0, s, src, heavy_list,
                        c32 + 3, list_cap, c32 + 4, resume, tolerance, (uint64_t)max_itr, out, replay_list, c32 + 2, so);
   if (split) {
-    hipLaunchKernelGGL(tuv_task_kernel<Src>, dim3(grid_blocks), dim3(kTuvBlock), 0, s, src, c32 + 14, tolerance,
+    hipLaunchKernelGGL(tuv_task_kernel<Src>, dim3(grid_blocks), dim3(kTuvBlock), 0, s, src, tolerance,
                        (uint64_t)max_itr, so);
     hipLaunchKernelGGL(tuv_split_finish_kernel, dim3(16), dim3(kTuvBlock), 0, s, so, (uint64_t)max_itr, out,
                        replay_list, c32 + 2, list_cap);

@@ -30,8 +30,9 @@ SPACE = {
     "handover_at": (0, 8, 16),
     "blocks_per_cu": (2, 3, 4),
 }
-# tried after the grid, with the grid's winner: when the wave pass cuts a pair into pieces
-SPLIT_CHOICES = (0, 16384, 65536)
+# tried after the grid, with the grid's winner: boxes a wave takes on a pair before it hands
+# walks to waiting waves (0: never)
+SPLIT_CHOICES = (0, 1024, 2048, 4096, 16384)
 
 
 @dataclass

@@ -80,7 +80,7 @@ class TuvConfig:
     light_budget: int = 1024     # boxes a pair may take in the stream/persistent pass before the wave pass
     refill_at: int = 8           # stream: idle lanes of a wave that trigger a refill
     handover_at: int = 8         # stream: busy lanes of a drained wave at or below which all go to the wave pass
-    split_at: int = 16384        # wave pass: boxes after which a pair is cut into pieces for many waves (0: never)
+    split_at: int = 4096         # wave pass: boxes a wave takes on a pair before it hands walks to waiting waves (0: never)
     small_batch_budget: int = 0   # stream: light budget of a batch filling at most 1/8 of the lanes (0: off)
     split_tasks: int = 1 << 16   # capacity of the piece buffer
     round_budgets: tuple = (8, 32, 128, 512)  # walk steps per pair per round before the wave pass
@@ -259,8 +259,8 @@ class TuvCertifier:
 
     def split_counts(self) -> tuple:
         """(pieces written, pairs split) by the last stream-form wave pass."""
-        v = int(self.counters[7].item())
-        return int(self.counters[6].item() >> 32) & 0xFFFFFFFF, (v >> 32) & 0xFFFFFFFF
+        c = self.counters.cpu().numpy().view(np.uint32)
+        return int(c[13]) + int(c[8]), int(c[15])
 
     def heavy_count(self) -> int:
         return int((self.counters[1].item() >> 32) & 0xFFFFFFFF)

@@ -132,15 +132,32 @@ def test_split_wave_pass_matches_oracle(gpu, split_at, tasks):
         assert split > 0 and 0 < pieces <= tasks and cert.replay_count() == 0
 
 
+def test_split_wave_pass_deep_pairs_match_oracle(gpu):
+    # Every third pair scaled by 1e4 splits some dimension more than 31
+    # times: such pairs take the 64-bit-key wave walk and its split, their
+    # pieces sharing the task queue with the jump walk's; two per family
+    # run over max_itr and go to the canonical replay.
+    qs = workloads.make_query_set(1500, 1500, seed=59)
+    cfg = TuvConfig(light_budget=8, split_at=32, max_itr=20000, heap_cap=1 << 16)
+    cert = TuvCertifier(cfg)
+    for is_ee, x in ((False, qs.vf.copy()), (True, qs.ee.copy())):
+        x[::3] *= 1e4
+        got = cert.certify_positions(x, is_ee, "stream")
+        ref = oracle.tuv_cert(is_ee, x, max_itr=20000, heap_cap=1 << 16, threads=8)
+        _check(got, ref, is_ee)
+        assert cert.split_counts()[1] > 0 and cert.replay_count() > 0
+
+
 def test_split_wave_pass_forced_replay(gpu):
     # Split pairs whose pieces run out of budget (or whose total does) go to
-    # the canonical replay like any other over-budget pair.
+    # the canonical replay like any other over-budget pair (about 150 pairs
+    # per family exceed 500 boxes).
     qs = workloads.make_query_set(2000, 2000, seed=31)
-    cfg = TuvConfig(max_itr=40, heap_cap=1 << 12, replay_slots=16, light_budget=4, split_at=8)
+    cfg = TuvConfig(max_itr=500, heap_cap=1 << 12, replay_slots=16, light_budget=4, split_at=8)
     cert = TuvCertifier(cfg)
     for is_ee, x in ((False, qs.vf), (True, qs.ee)):
         got = cert.certify_positions(x, is_ee, "stream")
-        ref = oracle.tuv_cert(is_ee, x, max_itr=40, heap_cap=1 << 12, threads=8)
+        ref = oracle.tuv_cert(is_ee, x, max_itr=500, heap_cap=1 << 12, threads=8)
         _check(got, ref, is_ee)
         replayed = (got["flags"] & FLAG_REPLAYED) != 0
         assert replayed.sum() == cert.replay_count() > 0
