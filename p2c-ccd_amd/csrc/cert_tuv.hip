@@ -939,8 +939,7 @@ struct SplitOut {
   SplitTask* tasks;
   uint32_t task_cap;     // usable slots
   uint32_t jump_cap;     // slots of the jump-walk pieces
-  QueueHead* head;
-  uint32_t epoch;        // this pass's tag, read by each kernel at its start
+  QueueHead* head;       // also this pass's tag (its epoch line, set before the pass's kernels)
   uint32_t* task_count;  // reserved jump-walk slots
   uint32_t* task_next;   // taken jump-walk slots
   uint32_t* lw_count;    // reserved slots of the other walk (from jump_cap)
@@ -951,6 +950,11 @@ struct SplitOut {
   uint32_t* report;      // the jump-walk piece count, copied here at the end
   SplitAcc* acc;         // per pair (the walk state slots)
 };
+
+// The pass's tag: written by the epoch kernel, read-only during the pass.
+// Kernels take the queue's descriptor straight from their arguments (a copy
+// in registers spilled in the wave pass).
+__device__ inline uint32_t pass_epoch(const SplitOut& so) { return so.head->epoch.v; }
 
 // Lane 0 reserves `need` (> 0) slots of `count` below `cap`; returns the
 // first or ~0u when there is no room (the caller keeps its work).
@@ -993,7 +997,7 @@ __device__ inline void put_jump_task(const SplitOut& so, uint32_t slot, const Ju
                                          (uint32_t)pc.low[1], (uint32_t)(pc.low[1] >> 32)};
 #pragma unroll
   for (int i = 0; i < kPieceWords; ++i)
-    __hip_atomic_store(&w[i], tag_word(payload[i], so.epoch), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&w[i], tag_word(payload[i], pass_epoch(so)), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ inline JumpPiece unpack_piece(const uint32_t t[kPieceWords]) {
@@ -1017,7 +1021,7 @@ __device__ inline bool get_jump_task(const SplitOut& so, uint32_t slot, uint32_t
 #pragma unroll
   for (int i = 0; i < kPieceWords; ++i) {
     const uint64_t v = __hip_atomic_load(&w[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    ok = ok && tag_ok(v, so.epoch);
+    ok = ok && tag_ok(v, pass_epoch(so));
     out[i] = (uint32_t)(v >> 32);
   }
   return ok;
@@ -1321,9 +1325,12 @@ __device__ int wave_certify_jump(const TuvPair& q, uint32_t p, JumpBox* stk, Jum
       if (finished && !(wide ? jump_next(q, w) : jump_next_lo(q, w))) active = false;
     }
     uc.boxes += (uint64_t)__popcll(__ballot(evaluated));
-    // 2. share the least done t_lo: a find lowers the pair's accumulator,
-    //    a look reads it; a look also adds the boxes (once kFlushBoxes have
-    //    gathered; the sum decides the budget) and reads the task queue
+    // 2. share the least done t_lo: a find is shared across the wave at
+    //    once; a look lowers the pair's accumulator with the wave's best and
+    //    takes the other waves' (one atomic: a find's own atomic, under the
+    //    contention of hundreds of waves on one pair, cost more than a
+    //    step), adds the boxes (once kFlushBoxes have gathered; the sum
+    //    decides the budget) and reads the task queue
     const bool look = (kSplit || uc.listed) && (++step & kLookMask) == 0;
     const bool any_found = __ballot(found) != 0;
     bool over = uc.boxes > max_itr;
@@ -1331,13 +1338,13 @@ __device__ int wave_certify_jump(const TuvPair& q, uint32_t p, JumpBox* stk, Jum
     if (any_found || look) {
       uint32_t b = wave_min_u32(w.best);
       const bool flush = look && uc.listed && uc.boxes - uc.flushed >= kFlushBoxes;
-      if (uc.listed || look) {
+      if (look) {
         uint32_t g = kJumpNoHit, flags = 0;
         int d = 0;
         if (lane == 0) {
           if (uc.listed) {
             SplitAcc* a = &so.acc[p];
-            g = best32_of(any_found ? atomicMin(&a->best, (unsigned long long)jump_best64(b)) : acc_load_best(a));
+            g = best32_of(atomicMin(&a->best, (unsigned long long)jump_best64(b)));
             if (flush) {
               const uint32_t add = clamp_u32(uc.boxes - uc.flushed);
               const uint64_t tot = (uint64_t)atomicAdd(&a->boxes, add) + add;
@@ -1555,7 +1562,7 @@ constexpr uint32_t kSlotDone = 1u, kSlotLeave = 2u;  // control word marks
 
 __device__ inline void mark_slot(const SplitOut& so, uint32_t slot, uint32_t mark) {
   auto* w = reinterpret_cast<JumpTaskWords*>(so.tasks + slot)->w;
-  __hip_atomic_store(&w[kControlWord], tag_word(mark, so.epoch), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(&w[kControlWord], tag_word(mark, pass_epoch(so)), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Lane 0 took `pending` to 0: leave marks for the waiting tickets.
@@ -1594,9 +1601,9 @@ __device__ void serve_tasks(const Src& src, const SplitOut& so, WaveLds& lds, do
         const auto* w = reinterpret_cast<JumpTaskWords*>(so.tasks + ticket)->w;
         const uint64_t w0 = __hip_atomic_load(&w[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const uint64_t wc = __hip_atomic_load(&w[kControlWord], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (tag_ok(w0, so.epoch) && get_jump_task(so, ticket, t)) {
+        if (tag_ok(w0, pass_epoch(so)) && get_jump_task(so, ticket, t)) {
           state = 1;
-        } else if (tag_ok(wc, so.epoch) && (uint32_t)(wc >> 32) == kSlotLeave) {
+        } else if (tag_ok(wc, pass_epoch(so)) && (uint32_t)(wc >> 32) == kSlotLeave) {
           state = 2;
         } else if ((spins & 255u) == 255u &&
                    __hip_atomic_load(so.pending, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u &&
@@ -1634,9 +1641,8 @@ __global__ __launch_bounds__(kTuvBlock, 3) void tuv_heavy_kernel(Src src,
   __shared__ WaveLds lds[kTuvBlock / 64];
   __shared__ unsigned long long best_slot[kTuvBlock / 64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  SplitOut so = so_in;
+  const SplitOut& so = so_in;
   if constexpr (kSplit) {
-    so.epoch = so.head->epoch.v;
     if (lane == 0) atomicAdd(so.pending, 1u);
   }
   uint32_t total = *heavy_count;
@@ -1722,15 +1728,14 @@ __global__ __launch_bounds__(kTuvBlock, 3) void tuv_task_kernel(Src src, double 
   __shared__ WaveLds lds[kTuvBlock / 64];
   __shared__ unsigned long long best_slot[kTuvBlock / 64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  SplitOut so = so_in;
-  so.epoch = so.head->epoch.v;
+  const SplitOut& so = so_in;
   uint32_t jump_total = *so.task_count;
   if (jump_total > so.jump_cap) jump_total = so.jump_cap;
   const uint32_t waves = gridDim.x * (kTuvBlock / 64);
   for (uint32_t h = blockIdx.x * (kTuvBlock / 64) + (uint32_t)wv; h < jump_total; h += waves) {
     const auto* words = reinterpret_cast<const JumpTaskWords*>(so.tasks + h)->w;
     const uint64_t c = words[kControlWord];
-    if (tag_ok(c, so.epoch) && (uint32_t)(c >> 32) == kSlotDone) continue;
+    if (tag_ok(c, pass_epoch(so)) && (uint32_t)(c >> 32) == kSlotDone) continue;
     uint32_t t[kPieceWords];
 #pragma unroll
     for (int i = 0; i < kPieceWords; ++i) t[i] = (uint32_t)(words[i] >> 32);
