@@ -16,6 +16,7 @@ Prints one JSON line.  `python bench.py --gpus N`: only N = 1 is supported.
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -572,19 +573,20 @@ def rows_suite() -> int:
     return 0
 
 
-def served_form(scene, requests: int, want: tuple) -> dict:
+def served_form(scene, requests: int, want: tuple, cfg=None) -> dict:
     """The same step served through serve.Server (continuous batching over
     two lanes): requests alternate between two resident copies of the scene,
     so one lane's broad phase overlaps the other's certificate.  First all
     queued at once (saturated throughput), then arriving as a Poisson stream
     at 80% of that rate (latency).  want: (n_vf, n_ee, hits) of the batched
-    pipeline on the same step, which every served request must reproduce."""
+    pipeline on the same step, which every served request must reproduce;
+    cfg: the certificate parameters (the batched step's tuned ones)."""
     import numpy as np
 
     from p2cccd import serve
 
     budget = 4 * serve.ResidentScene.estimate_bytes(scene)
-    srv = serve.Server({0: scene, 1: scene}, budget)
+    srv = serve.Server({0: scene, 1: scene}, budget, cfg=cfg)
 
     def req(k, t=0.0):
         return serve.Request(k, k % 2, scene.x0, scene.x1, t)
@@ -695,7 +697,8 @@ def main() -> int:
 
     hits = int(r["vf_res"]["hit"].sum() + r["ee_res"]["hit"].sum())
     log("served form")
-    served = served_form(scene, 2 * args.steps, (pipe.last.n_vf, pipe.last.n_ee, hits))
+    served = served_form(scene, 2 * args.steps, (pipe.last.n_vf, pipe.last.n_ee, hits),
+                         dataclasses.replace(pipe.cert.cfg))
     served["speedup_vs_plain"] = served["pairs_per_s_saturated"] / plain_rate
     log(f"served: {served['pairs_per_s_saturated']:.3e} pairs/s saturated; at {served['rate_hz']:.1f} Hz "
         f"p50 {served['p50_ms']:.2f} ms p99 {served['p99_ms']:.2f} ms; identical {served['identical_to_batched']}")

@@ -31,7 +31,7 @@ SPACE = {
     "blocks_per_cu": (2, 3, 4),
 }
 # tried after the grid, with the grid's winner: boxes a wave takes on a pair before it hands
-# walks to waiting waves (0: never)
+# walks to waiting waves (0: the small-batch default, or never for a batch larger than the grid)
 SPLIT_CHOICES = (0, 1024, 2048, 4096, 16384)
 
 

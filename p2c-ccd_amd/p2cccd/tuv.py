@@ -80,7 +80,9 @@ class TuvConfig:
     light_budget: int = 1024     # boxes a pair may take in the stream/persistent pass before the wave pass
     refill_at: int = 8           # stream: idle lanes of a wave that trigger a refill
     handover_at: int = 8         # stream: busy lanes of a drained wave at or below which all go to the wave pass
-    split_at: int = 4096         # wave pass: boxes a wave takes on a pair before it hands walks to waiting waves (0: never)
+    split_at: int = 0            # wave pass: boxes a wave takes on a pair before it hands walks to waiting waves
+    small_batch_split_at: int = 4096  # split_at of a batch of at most one pair per grid lane when split_at is 0
+                                      # (0: off); large batches keep their waves busy without splitting
     small_batch_budget: int = 0   # stream: light budget of a batch filling at most 1/8 of the lanes (0: off)
     split_tasks: int = 1 << 16   # capacity of the piece buffer
     round_budgets: tuple = (8, 32, 128, 512)  # walk steps per pair per round before the wave pass
@@ -195,9 +197,12 @@ class TuvCertifier:
             stream_args = (n, tp, self.cfg.tolerance, self.cfg.max_itr, budget, self.cfg.refill_at,
                            self.cfg.handover_at, self.results.data_ptr(), c.data_ptr(), self.states.data_ptr(),
                            self.heavy_list.data_ptr(), self._cap, self.grid_blocks, s)
+            split_at = self.cfg.split_at
+            if split_at == 0 and n <= self.grid_blocks * 256:
+                split_at = self.cfg.small_batch_split_at
             wave_args = (c.data_ptr(), self.states.data_ptr(), self.heavy_list.data_ptr(), self._cap,
                          self.cfg.tolerance, self.cfg.max_itr, self.results.data_ptr(), self.replay_list.data_ptr(),
-                         self.cfg.split_at, self.tasks.data_ptr(), self.cfg.split_tasks, self.split_list.data_ptr(),
+                         split_at, self.tasks.data_ptr(), self.cfg.split_tasks, self.split_list.data_ptr(),
                          self.grid_blocks, s)
             if self.lite_active:
                 nat.call("p2c_tuv_stream_lite", self.lite.data_ptr(), self.verts.ctypes.data, *stream_args)

@@ -31,7 +31,7 @@ def test_static_certificate_matches_stream_entry_points(gpu):
     # the replay (heaps in the static array) both run; call 2: 3k EE pairs
     for is_ee, x, max_itr in ((False, qs.vf, 40), (True, qs.ee, 1_000_000)):
         cfg = TuvConfig(max_itr=max_itr, heap_cap=1 << 16, replay_slots=8, light_budget=1024, refill_at=8,
-                        handover_at=8, blocks_per_cu=3, split_at=0)
+                        handover_at=8, blocks_per_cu=3, split_at=0, small_batch_split_at=0)
         cert = TuvCertifier(cfg, device=gpu)
         xd = torch.from_numpy(np.ascontiguousarray(x)).to(gpu)
         n = cert.setup_positions(xd, is_ee)
