@@ -339,6 +339,106 @@ P2C_TUV_HD void tuv_setup_geometry(TuvPair* q, const double v[8][3], int is_ee, 
   }
 }
 
+// Sets the bits {start + p * r : 0 <= r < count} (p = 1, 2 or 3) of a
+// 128-level mask; levels from 128 on are not kept here.
+P2C_TUV_HD void set_periodic(uint64_t m[2], uint32_t start, uint32_t p, uint32_t count) {
+  if (count == 0) return;
+  const uint64_t base = p == 1 ? ~0ull : (p == 2 ? 0x5555555555555555ull : 0x9249249249249249ull);
+  const uint32_t end = start + p * (count - 1u) + 1u;  // one past the last bit
+#pragma unroll
+  for (uint32_t w = 0; w < 2; ++w) {
+    const uint32_t lo = 64u * w, hi = lo + 64u;
+    if (end <= lo || start >= hi) continue;
+    const uint32_t s0 = start > lo ? start - lo : 0u;
+    const uint32_t e0 = end < hi ? end - lo : 64u;
+    const uint32_t phase = start >= lo ? start - lo : (p - (lo - start) % p) % p;
+    const uint64_t range = (e0 == 64u ? ~0ull : ((1ull << e0) - 1ull)) & ~((1ull << s0) - 1ull);
+    m[w] |= (base << phase) & range;
+  }
+}
+
+// The level masks, high codes and length of tuv_split_sequence's sequence,
+// built without the 2-bit words: between two consecutive changes of the set
+// of dimensions that split in a round (at most six), the rounds repeat one
+// pattern of period p (the number of those dimensions), so each dimension's
+// levels there are every p-th level from its place in the round.  Inputs the
+// rounds do not cover (inf, NaN, subnormal) take tuv_split_sequence.  The
+// host check tests/native/split_sequence_check.hip compares the two.
+P2C_TUV_HD void tuv_split_masks_of(const double inv_tol[3], uint64_t tmask[2], uint64_t umask[2],
+                                   uint64_t* split_hi, uint32_t* done_level) {
+  bool regular = true;
+  int E[3], n[3];
+  uint64_t mant[3];
+  for (int k = 0; k < 3; ++k) {
+    const uint64_t bits = __builtin_bit_cast(uint64_t, inv_tol[k]);
+    const int bexp = (int)((bits >> 52) & 0x7FF);
+    mant[k] = bits & ((1ull << 52) - 1ull);
+    E[k] = bexp - 1023;
+    const bool zero = (bits << 1) == 0;
+    regular = regular && (zero || (bexp != 0 && bexp != 0x7FF)) && !(bits >> 63);
+    int c = zero ? 0 : E[k] + (mant[k] != 0 ? 1 : 0);
+    c = c < 0 ? 0 : (c > kTuvMaxExp ? kTuvMaxExp : c);
+    n[k] = c;
+  }
+  if (!regular) {
+    uint64_t split[5];
+    tuv_split_sequence(inv_tol, split, done_level);
+    tuv_split_masks(split, tmask, umask, split_hi);
+    return;
+  }
+  int o[3] = {0, 1, 2};  // m descending, then k
+  auto before = [&](int a, int b) { return mant[a] > mant[b] || (mant[a] == mant[b] && a < b); };
+  if (before(o[1], o[0])) { const int t = o[0]; o[0] = o[1]; o[1] = t; }
+  if (before(o[2], o[1])) { const int t = o[1]; o[1] = o[2]; o[2] = t; }
+  if (before(o[1], o[0])) { const int t = o[0]; o[0] = o[1]; o[1] = t; }
+  int hi[3], lo[3];
+  int vmax = -100000, vmin = 100000;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const int k = o[i];
+    const int nk = get3(n, k), ek = get3(E, k);
+    hi[i] = nk > 0 ? ek : -100000;
+    lo[i] = nk > 0 ? ek - nk + 1 : 100000;
+    vmax = nk > 0 && ek > vmax ? ek : vmax;
+    vmin = nk > 0 && lo[i] < vmin ? lo[i] : vmin;
+  }
+  tmask[0] = tmask[1] = umask[0] = umask[1] = 0;
+  uint64_t codes_hi = ~0ull;  // levels 128..159, 3 = no split
+  uint32_t level = 0;
+  for (int v = vmax; v >= vmin;) {
+    // the dimensions splitting in round v, in order, and the next round
+    // below v at which that set changes
+    int nxt = vmin - 1;
+    uint32_t p = 0;
+    int act[3] = {0, 0, 0};
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      if (lo[i] <= v && v <= hi[i]) {
+        act[p++] = o[i];
+        nxt = lo[i] - 1 > nxt ? lo[i] - 1 : nxt;
+      } else if (hi[i] < v && hi[i] > nxt) {
+        nxt = hi[i];
+      }
+    }
+    const uint32_t count = (uint32_t)(v - nxt);
+    for (uint32_t i = 0; i < p; ++i) {
+      const int k = act[i];
+      if (k == 0) set_periodic(tmask, level + i, p, count);
+      if (k == 1) set_periodic(umask, level + i, p, count);
+    }
+    if (level + p * count > kTuvMaskLevels) {  // past the masks: 2-bit codes (rare)
+      for (uint32_t l = level > kTuvMaskLevels ? level : kTuvMaskLevels; l < level + p * count; ++l) {
+        const uint32_t sh = (l - kTuvMaskLevels) * 2u;
+        if (sh < 64u) codes_hi = (codes_hi & ~(3ull << sh)) | ((uint64_t)act[(l - level) % p] << sh);
+      }
+    }
+    level += p * count;
+    v = nxt;
+  }
+  *split_hi = codes_hi;
+  *done_level = level;
+}
+
 // Per-pair constants from the 8 positions (rows 0-3 at t=0, 4-7 at t=1).
 P2C_TUV_HD void tuv_setup(TuvPair* q, const double v[8][3], int is_ee, double ms, double tolerance) {
   tuv_setup_geometry(q, v, is_ee, ms);
@@ -375,9 +475,7 @@ P2C_TUV_HD void tuv_setup(TuvPair* q, const double v[8][3], int is_ee, double ms
         L[2] = dv > L[2] ? dv : L[2];
       }
   for (int k = 0; k < 3; ++k) q->inv_tol[k] = (3.0 * L[k]) / tolerance;
-  uint64_t split[5];
-  tuv_split_sequence(q->inv_tol, split, &q->done_level);
-  tuv_split_masks(split, q->tmask, q->umask, &q->split_hi);
+  tuv_split_masks_of(q->inv_tol, q->tmask, q->umask, &q->split_hi, &q->done_level);
 }
 
 // Box outcome: 0 = no zero inside, 1 = done (certifies a hit at its t_lo),

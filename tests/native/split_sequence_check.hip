@@ -54,6 +54,18 @@ int main() {
     TuvPair q; memset(&q, 0, sizeof q);
     tuv_split_masks(w, q.tmask, q.umask, &q.split_hi);
     q.done_level = dln;
+    // the setup's direct builder: the same masks, high codes and length
+    {
+      uint64_t tm[2], um[2], sh;
+      uint32_t dl;
+      tuv_split_masks_of(it, tm, um, &sh, &dl);
+      if (dl != dln || tm[0] != q.tmask[0] || tm[1] != q.tmask[1] || um[0] != q.umask[0] || um[1] != q.umask[1] ||
+          sh != q.split_hi) {
+        if (bad < 5) printf("direct mask mismatch %g %g %g\n", a, b, c);
+        ++bad;
+        return;
+      }
+    }
     int e[3] = {0, 0, 0};
     for (uint32_t l = 0; l <= dln; ++l) {
       int ex[3];
