@@ -186,7 +186,9 @@ class Lane:
     def __init__(self, device, cfg: TuvConfig | None, cache_slots: int):
         self.device = device
         self.stream = torch.cuda.Stream(device)
-        self.cert = TuvCertifier(cfg, device)
+        # lanes' kernels overlap on the device, and a splitting wave pass keeps its waves resident while they
+        # wait for pieces: by default a lane's batches are not split
+        self.cert = TuvCertifier(cfg if cfg is not None else TuvConfig(small_batch_split_at=0), device)
         self.bp = BatchBroadPhase(device, cache_slots)
         self.batch: _Batch | None = None
         self.stage = torch.empty(0, dtype=torch.float64).pin_memory()  # pinned staging of the batch's positions
